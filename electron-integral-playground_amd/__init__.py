@@ -1,0 +1,23 @@
+"""MI355X-native McMurchie-Davidson integral path (overlap, nuclear attraction, (P|Q), (ij|P)).
+
+Host layer mirrors the reference package's interface (readers, basis_utility, pair_utility,
+molecular_integral.*); the integrals run in the HIP library ``lib/libmolecular_integral.so``.
+"""
+
+import ctypes
+import pathlib
+
+PACKAGE_DIR = pathlib.Path(__file__).resolve().parent
+LIBRARY_PATH = PACKAGE_DIR / "lib" / "libmolecular_integral.so"
+
+_lib = None
+
+
+def library() -> ctypes.CDLL:
+    """Load the HIP library.  There is no CPU fallback: a missing library is an error."""
+    global _lib
+    if _lib is None:
+        if not LIBRARY_PATH.exists():
+            raise RuntimeError(f"HIP library {LIBRARY_PATH} is not built; run build() first")
+        _lib = ctypes.CDLL(str(LIBRARY_PATH))
+    return _lib

@@ -1,0 +1,470 @@
+"""Emit straight-line HIP kernels, one per angular class, for the McMurchie-Davidson path.
+
+In the style of the reference's offline generator (gen/equation_mcmurchie_davidson.py:4-98), which
+turns the MD recursions into straight-line C, but targeting gfx950 and restructured for it:
+
+* ``pairE_<la><lb>``: one thread per primitive pair.  Builds the pair's Hermite expansion E^{ij}_t by
+  the vertical recurrence and the binomial horizontal transfer (reference
+  mcmurchie_davidson_E_term.hpp:22-45, 166-199), contracts it straight into the *output* basis of the
+  pair (real solid harmonics, or normalised Cartesians) and folds the pair coefficient in.  Entries are
+  stored t-major, in exactly the order the three-centre kernels consume them, so those kernels read
+  them as wave-uniform scalar loads.
+* ``j3c_<la><lb><lc>_<m0>_<m1>``: one wave per contracted shell pair, one lane per auxiliary shell.
+  Each lane loops over the shell pair's primitive pairs (uniform) and its own auxiliary primitives,
+  evaluates the Boys function and the Hermite integrals R_tuv by the MD recursion in registers, applies
+  the auxiliary shell's Hermite-to-output transform (for spherical output only the |s| = lc terms
+  survive -- Hobson's theorem -- so it is a small constant matrix) and accumulates
+  out[ij, m] += E[ij, t] * V[t, m] in registers.  Components m0..m1 of the auxiliary shell are handled
+  by one kernel, so large classes are split over several kernels to bound register use.  The result is
+  identical to the reference's per-primitive Cartesian evaluation followed by normalisation and the
+  spherical transform (three_center.cpp:27-289) up to rounding.
+* ``v_<la><lb>``: the same structure with point charges in place of auxiliary shells (lc = 0,
+  zeta = p or p w^2/(p + w^2)), reference nuclear_attraction.cpp:23-171.
+* ``j2c_<li><lj>``: one lane per auxiliary shell pair; both sides single-centre
+  (reference two_center.cpp:25-212).
+"""
+
+from fractions import Fraction
+from math import comb
+
+from .tables import (aux_transform, cart_list, fmt, herm_list, n_out, ncart, pair_out_rows,
+                     pair_support, SqrtRational)
+
+
+# ---------------------------------------------------------------------------------------------
+# Layout of the packed pair matrix
+# ---------------------------------------------------------------------------------------------
+
+def pair_layout(la, lb, spherical):
+    """Entries (t, row) of the packed pair matrix in storage order (t-major)."""
+    sup = pair_support(la, lb, spherical)
+    ts = sorted({t for s in sup for t in s}, key=lambda t: (sum(t), -t[0], -t[1]))
+    entries = []
+    for t in ts:
+        for row, s in enumerate(sup):
+            if t in s:
+                entries.append((t, row))
+    return entries
+
+
+def pair_nnz(la, lb, spherical):
+    return len(pair_layout(la, lb, spherical))
+
+
+# ---------------------------------------------------------------------------------------------
+# Hermite E (pair) code: VRR + HRR per axis
+# ---------------------------------------------------------------------------------------------
+
+def _emit_E_axis(lines, ax, la, lb, ty="double"):
+    """Emit E{ax}_{i}_{j}_{t} for i <= la, j <= lb, t <= i + j."""
+    L = la + lb
+    lines.append(f"    const {ty} E0{ax}_0_0 = 1.0;")
+    for i in range(L):
+        for t in range(i + 2):
+            terms = []
+            if t >= 1:
+                terms.append(f"o2p * E0{ax}_{i}_{t - 1}")
+            if t <= i:
+                terms.append(f"PA{ax} * E0{ax}_{i}_{t}")
+            if t + 1 <= i:
+                terms.append(f"{fmt(t + 1)} * E0{ax}_{i}_{t + 1}")
+            lines.append(f"    const {ty} E0{ax}_{i + 1}_{t} = {' + '.join(terms)};")
+    if lb > 0:
+        for k in range(1, lb + 1):
+            lines.append(f"    const {ty} AB{ax}_p{k} = " + (f"AB{ax};" if k == 1 else f"AB{ax}_p{k - 1} * AB{ax};"))
+    for i in range(la + 1):
+        for j in range(lb + 1):
+            for t in range(i + j + 1):
+                terms = []
+                for m in range(j, -1, -1):
+                    if t > i + m:
+                        continue
+                    c = comb(j, j - m)
+                    pw = j - m
+                    base = f"E0{ax}_{i + m}_{t}"
+                    if pw == 0:
+                        terms.append(base if c == 1 else f"{fmt(c)} * {base}")
+                    else:
+                        terms.append(f"{fmt(c)} * AB{ax}_p{pw} * {base}")
+                lines.append(f"    const {ty} E{ax}_{i}_{j}_{t} = {' + '.join(terms)};")
+
+
+def emit_pairE(la, lb, spherical):
+    name = f"pairE_{la}{lb}_{'s' if spherical else 'c'}"
+    ca, cb = cart_list(la), cart_list(lb)
+    rows = pair_out_rows(la, lb, spherical)
+    layout = pair_layout(la, lb, spherical)
+    nnz = len(layout)
+    L = ["// pair matrix of class (%d,%d) %s: %d entries per primitive pair" %
+         (la, lb, "spherical" if spherical else "cartesian", nnz)]
+    L.append(f'extern "C" __global__ void __launch_bounds__(256) {name}(const PairPrepArgs a)')
+    L.append("{")
+    L.append("    const int pp = blockIdx.x * blockDim.x + threadIdx.x;")
+    L.append("    if (pp >= a.n_pp) return;")
+    L.append("    const double* __restrict__ Pp = a.P_p + 4 * (size_t) pp;")
+    L.append("    const double* __restrict__ Aa = a.A_a + 4 * (size_t) pp;")
+    L.append("    const double* __restrict__ Bb = a.B_b + 4 * (size_t) pp;")
+    L.append("    const double c = a.coef[pp];")
+    L.append("    double Px, Py, Pz, p;")
+    L.append("    if (a.recompute_P) {   // overlap ignores the stored P (reference overlap.cpp:22-29)")
+    L.append("        p = Aa[3] + Bb[3];")
+    L.append("        const double f = -Bb[3] / p;")
+    L.append("        Px = Aa[0] + f * (Aa[0] - Bb[0]); Py = Aa[1] + f * (Aa[1] - Bb[1]); Pz = Aa[2] + f * (Aa[2] - Bb[2]);")
+    L.append("    } else { Px = Pp[0]; Py = Pp[1]; Pz = Pp[2]; p = Pp[3]; }")
+    L.append("    const double o2p = 0.5 / p;")
+    L.append("    const double PAx = Px - Aa[0], PAy = Py - Aa[1], PAz = Pz - Aa[2];")
+    L.append("    const double ABx = Aa[0] - Bb[0], ABy = Aa[1] - Bb[1], ABz = Aa[2] - Bb[2];")
+    L.append("    (void) PAx; (void) PAy; (void) PAz; (void) ABx; (void) ABy; (void) ABz; (void) o2p;")
+    for ax in "xyz":
+        _emit_E_axis(L, ax, la, lb)
+    L.append("    double* __restrict__ out = a.E + (size_t) pp * %d;" % nnz)
+    L.append("    float* __restrict__ out32 = a.E32 ? a.E32 + (size_t) pp * %d : nullptr;" % nnz)
+    for k, (t, row) in enumerate(layout):
+        r, terms = rows[row]
+        pieces = []
+        for (ci, cj), q in terms.items():
+            ix, iy, iz = ca[ci]
+            jx, jy, jz = cb[cj]
+            if t[0] > ix + jx or t[1] > iy + jy or t[2] > iz + jz:
+                continue
+            coef = SqrtRational(r, q).to_float()
+            prod = f"Ex_{ix}_{jx}_{t[0]} * Ey_{iy}_{jy}_{t[1]} * Ez_{iz}_{jz}_{t[2]}"
+            pieces.append(prod if coef == 1.0 else f"{fmt(coef)} * ({prod})")
+        expr = " + ".join(pieces) if pieces else "0.0"
+        L.append(f"    {{ const double v = c * ({expr}); out[{k}] = v; if (out32) out32[{k}] = (float) v; }}")
+    L.append("}")
+    return name, nnz, "\n".join(L)
+
+
+# ---------------------------------------------------------------------------------------------
+# Hermite integrals R_tuv by the MD recursion (straight-line, only what is needed)
+# ---------------------------------------------------------------------------------------------
+
+def _r_name(n, t):
+    return f"r{n}_{t[0]}_{t[1]}_{t[2]}"
+
+
+def _emit_R(lines, L, needed0, ty):
+    """Emit R^0_t for t in needed0 (plus dependencies), from R^n_000 named R000_n."""
+    need = set((0, t) for t in needed0)
+    stack = list(need)
+    deps = {}
+    while stack:
+        n, t = stack.pop()
+        tx, ty_, tz = t
+        if t == (0, 0, 0):
+            deps[(n, t)] = []
+            continue
+        if tx > 0:
+            d = [(n + 1, (tx - 1, ty_, tz))] + ([(n + 1, (tx - 2, ty_, tz))] if tx > 1 else [])
+        elif ty_ > 0:
+            d = [(n + 1, (tx, ty_ - 1, tz))] + ([(n + 1, (tx, ty_ - 2, tz))] if ty_ > 1 else [])
+        else:
+            d = [(n + 1, (tx, ty_, tz - 1))] + ([(n + 1, (tx, ty_, tz - 2))] if tz > 1 else [])
+        deps[(n, t)] = d
+        for x in d:
+            if x not in need:
+                need.add(x)
+                stack.append(x)
+    order = sorted(need, key=lambda x: (-x[0], sum(x[1]), x[1]))
+    for n, t in order:
+        tx, ty_, tz = t
+        if t == (0, 0, 0):
+            lines.append(f"        const {ty} {_r_name(n, t)} = R000_{n};")
+            continue
+        if tx > 0:
+            a, b, X, k = (tx - 1, ty_, tz), (tx - 2, ty_, tz), "PQx", tx - 1
+        elif ty_ > 0:
+            a, b, X, k = (tx, ty_ - 1, tz), (tx, ty_ - 2, tz), "PQy", ty_ - 1
+        else:
+            a, b, X, k = (tx, ty_, tz - 1), (tx, ty_, tz - 2), "PQz", tz - 1
+        expr = f"{X} * {_r_name(n + 1, a)}"
+        if k > 0:
+            expr += f" + {fmt(k)}{'f' if ty == 'float' else ''} * {_r_name(n + 1, b)}"
+        lines.append(f"        const {ty} {_r_name(n, t)} = {expr};")
+
+
+# ---------------------------------------------------------------------------------------------
+# Three-centre kernel
+# ---------------------------------------------------------------------------------------------
+
+def emit_j3c(la, lb, lc, m0, m1, precision="f64"):
+    """Spherical three-centre kernel for aux components [m0, m1) of class (la lb | lc)."""
+    spherical = True
+    f32 = precision == "f32"
+    ty = "float" if f32 else "double"
+    lit = (lambda v: fmt(v) + "f") if f32 else fmt
+    name = f"j3c_{la}{lb}{lc}_{m0}_{m1}" + ("_f32" if f32 else "")
+    L = la + lb + lc
+    layout = pair_layout(la, lb, spherical)
+    nnz = len(layout)
+    nij = n_out(la, spherical) * n_out(lb, spherical)
+    nj = n_out(lb, spherical)
+    aux_rows = aux_transform(lc, spherical)
+    ms = list(range(m0, m1))
+    # Hermite t of the pair used by the layout, and R_{t+s} needed
+    ts = []
+    for t, _ in layout:
+        if t not in ts:
+            ts.append(t)
+    needed = set()
+    for t in ts:
+        for m in ms:
+            for s in aux_rows[m][1]:
+                needed.add((t[0] + s[0], t[1] + s[1], t[2] + s[2]))
+    out = [f"// ({la}{lb}|{lc}) aux components [{m0},{m1}) {precision}: L = {L}, {nnz} pair entries"]
+    out.append(f'extern "C" __global__ void __launch_bounds__(64) {name}(const J3cArgs a)')
+    out.append("{")
+    out.append("    const int lb_ = eip::xcd_remap(blockIdx.x, gridDim.x);")
+    out.append("    const int item = a.item_begin + lb_;")
+    out.append("    if (item >= a.item_end) return;")
+    out.append("    const int sp = a.items[2 * item];")
+    out.append("    const int ks_first = a.items[2 * item + 1];")
+    out.append("    const int ks_raw = ks_first + (int) threadIdx.x;")
+    out.append("    const int ks_last = a.items_ks_end ? a.items_ks_end[item] : a.n_ks;")
+    out.append("    const bool active = ks_raw < ks_last;")
+    out.append("    const int ks = active ? ks_raw : ks_last - 1;")
+    out.append("    const int ksh = a.ks_perm ? a.ks_perm[ks] : ks;")
+    out.append("    const int kp0 = a.ks_prim_off[ksh], kp1 = a.ks_prim_off[ksh + 1];")
+    out.append("    const int pp0 = a.sp_pp_off[sp], pp1 = a.sp_pp_off[sp + 1];")
+    for ij in range(nij):
+        for m in ms:
+            out.append(f"    {ty} acc_{ij}_{m} = 0;")
+    out.append("    for (int pp = pp0; pp < pp1; ++pp) {")
+    out.append(f"      const {ty}* __restrict__ E = a.{'E32' if f32 else 'E'} + (size_t) pp * {nnz};")
+    out.append("      const double Px = a.pp_P[4 * pp + 0], Py = a.pp_P[4 * pp + 1], Pz = a.pp_P[4 * pp + 2], p = a.pp_P[4 * pp + 3];")
+    out.append("      for (int kp = kp0; kp < kp1; ++kp) {")
+    out.append("        const double q = a.ka_C[4 * kp + 3];")
+    out.append("        const double ck = a.ka_coef[kp];")
+    # geometry and prefactor in double always (cheap), then cast
+    out.append("        const double dPQx = Px - a.ka_C[4 * kp + 0], dPQy = Py - a.ka_C[4 * kp + 1], dPQz = Pz - a.ka_C[4 * kp + 2];")
+    out.append("        double zeta, pf;")
+    out.append("        if (a.omega == 0.0) {")
+    out.append("            zeta = p * q / (p + q);")
+    out.append("            pf = ck * eip::kTwoPi25 / (p * q * sqrt(p + q));")
+    out.append("        } else {")
+    out.append("            const double w2 = a.omega * a.omega;")
+    out.append("            zeta = 1.0 / (1.0 / p + 1.0 / q + 1.0 / w2);")
+    out.append("            pf = ck * eip::kTwoPi25 * a.omega / (p * q * sqrt(p * q + (p + q) * w2));")
+    out.append("        }")
+    if lc > 0:
+        # common factor (-1)^lc (1/2q)^lc of the auxiliary side
+        out.append("        {")
+        out.append("            const double h = 0.5 / q;")
+        out.append("            double hp = h;")
+        for _ in range(lc - 1):
+            out.append("            hp *= h;")
+        out.append(f"            pf *= {'-' if lc % 2 else ''}hp;")
+        out.append("        }")
+    out.append(f"        const {ty} PQx = ({ty}) dPQx, PQy = ({ty}) dPQy, PQz = ({ty}) dPQz;")
+    out.append(f"        const {ty} T = ({ty}) (zeta * (dPQx * dPQx + dPQy * dPQy + dPQz * dPQz));")
+    out.append(f"        {ty} F[{L + 1}];")
+    out.append(f"        eip::boys{'32' if f32 else '64'}<{L}>(T, F);")
+    out.append(f"        const {ty} m2z = ({ty}) (-2.0 * zeta);")
+    out.append(f"        const {ty} R000_0 = ({ty}) pf * F[0];")
+    for n in range(1, L + 1):
+        out.append(f"        const {ty} pf_{n} = {'(' + ty + ') pf' if n == 1 else f'pf_{n - 1}'} * m2z;")
+        out.append(f"        const {ty} R000_{n} = pf_{n} * F[{n}];")
+    _emit_R(out, L, needed, ty)
+    # contraction, t-major
+    k = 0
+    for t in ts:
+        vnames = {}
+        for m in ms:
+            r, terms = aux_rows[m]
+            pieces = []
+            for s, (qf, j) in terms.items():
+                assert j == 0
+                coef = SqrtRational(r, qf).to_float()
+                rn = _r_name(0, (t[0] + s[0], t[1] + s[1], t[2] + s[2]))
+                pieces.append(rn if coef == 1.0 else f"{lit(coef)} * {rn}")
+            vn = f"V_{t[0]}_{t[1]}_{t[2]}_{m}"
+            out.append(f"        const {ty} {vn} = {' + '.join(pieces)};")
+            vnames[m] = vn
+        while k < len(layout) and layout[k][0] == t:
+            row = layout[k][1]
+            out.append(f"        {{ const {ty} e = E[{k}];")
+            for m in ms:
+                out.append(f"          acc_{row}_{m} = fma{'f' if f32 else ''}(e, {vnames[m]}, acc_{row}_{m});")
+            out.append("        }")
+            k += 1
+    out.append("      }")
+    out.append("    }")
+    out.append("    if (!active) return;")
+    out.append("    const int col = a.ks_col[ksh];")
+    out.append("    const int I = a.sp_i[sp], J = a.sp_j[sp];")
+    out.append("    double* __restrict__ o = a.out;")
+    out.append("    if (a.layout == 0) {")
+    for ij in range(nij):
+        mi, mj = divmod(ij, nj)
+        out.append(f"      if (I != J || {mi} >= {mj}) {{ const long long row = eip::packed_row(I + {mi}, J + {mj}) * a.ld + col;")
+        for m in ms:
+            out.append(f"        o[row + {m}] = (double) acc_{ij}_{m};")
+        out.append("      }")
+    out.append("    } else {")
+    out.append(f"      const long long base = (long long) sp * {nij};")
+    for ij in range(nij):
+        out.append(f"      {{ const long long row = (base + {ij}) * a.ld + col;")
+        for m in ms:
+            out.append(f"        o[row + {m}] = (double) acc_{ij}_{m};")
+        out.append("      }")
+    out.append("    }")
+    out.append("}")
+    return name, "\n".join(out)
+
+
+# ---------------------------------------------------------------------------------------------
+# Nuclear attraction kernel: lanes over point charges
+# ---------------------------------------------------------------------------------------------
+
+def emit_nuc(la, lb):
+    spherical = True
+    name = f"v_{la}{lb}"
+    L = la + lb
+    layout = pair_layout(la, lb, spherical)
+    nnz = len(layout)
+    nij = n_out(la, spherical) * n_out(lb, spherical)
+    ts = []
+    for t, _ in layout:
+        if t not in ts:
+            ts.append(t)
+    out = [f"// nuclear attraction ({la}{lb}): {nnz} pair entries"]
+    out.append(f'extern "C" __global__ void __launch_bounds__(64) {name}(const NucArgs a)')
+    out.append("{")
+    out.append("    const int blk = eip::xcd_remap(blockIdx.x, gridDim.x);")
+    out.append("    const int nchunk = (a.n_charge + 63) / 64;")
+    out.append("    const int sp = blk / nchunk;")
+    out.append("    if (sp >= a.n_sp) return;")
+    out.append("    const int c_raw = (blk - sp * nchunk) * 64 + (int) threadIdx.x;")
+    out.append("    const bool active = c_raw < a.n_charge;")
+    out.append("    const int ic = active ? c_raw : a.n_charge - 1;")
+    out.append("    const double Cx = a.charge[3 * ic + 0], Cy = a.charge[3 * ic + 1], Cz = a.charge[3 * ic + 2];")
+    out.append("    const int pp0 = a.sp_pp_off[sp], pp1 = a.sp_pp_off[sp + 1];")
+    for ij in range(nij):
+        out.append(f"    double acc_{ij} = 0;")
+    out.append("    for (int pp = pp0; pp < pp1; ++pp) {")
+    out.append(f"        const double* __restrict__ E = a.E + (size_t) pp * {nnz};")
+    out.append("        const double p = a.pp_P[4 * pp + 3];")
+    out.append("        const double PQx = a.pp_P[4 * pp + 0] - Cx, PQy = a.pp_P[4 * pp + 1] - Cy, PQz = a.pp_P[4 * pp + 2] - Cz;")
+    out.append("        double zeta, pf;")
+    out.append("        if (a.omega == 0.0) { zeta = p; pf = eip::kTwoPi / p; }")
+    out.append("        else { const double w2 = a.omega * a.omega; zeta = p * w2 / (p + w2); pf = eip::kTwoPi * a.omega / (p * sqrt(p + w2)); }")
+    out.append("        const double T = zeta * (PQx * PQx + PQy * PQy + PQz * PQz);")
+    out.append(f"        double F[{L + 1}];")
+    out.append(f"        eip::boys64<{L}>(T, F);")
+    out.append("        const double m2z = -2.0 * zeta;")
+    out.append("        const double R000_0 = pf * F[0];")
+    for n in range(1, L + 1):
+        out.append(f"        const double pf_{n} = {'pf' if n == 1 else f'pf_{n - 1}'} * m2z;")
+        out.append(f"        const double R000_{n} = pf_{n} * F[{n}];")
+    _emit_R(out, L, set(ts), "double")
+    for k, (t, row) in enumerate(layout):
+        out.append(f"        acc_{row} = fma(E[{k}], {_r_name(0, t)}, acc_{row});")
+    out.append("    }")
+    out.append("    if (!active) return;")
+    out.append("    const long long base = (long long) sp * %d;" % nij)
+    for ij in range(nij):
+        out.append(f"    a.out[(base + {ij}) * a.ld + ic] = acc_{ij};")
+    out.append("}")
+    return name, "\n".join(out)
+
+
+# ---------------------------------------------------------------------------------------------
+# Two-centre kernel: lanes over auxiliary shells J for one shell I per wave
+# ---------------------------------------------------------------------------------------------
+
+def emit_j2c(li, lj):
+    spherical = True
+    name = f"j2c_{li}{lj}"
+    L = li + lj
+    ri = aux_transform(li, spherical)
+    rj = aux_transform(lj, spherical)
+    ni, nj = len(ri), len(rj)
+    out = [f"// two-centre ({li}|{lj})"]
+    out.append(f'extern "C" __global__ void __launch_bounds__(64) {name}(const J2cArgs a)')
+    out.append("{")
+    out.append("    const int blk = eip::xcd_remap(blockIdx.x, gridDim.x);")
+    out.append("    const int nchunk = (a.n_j + 63) / 64;")
+    out.append("    const int si = blk / nchunk;")
+    out.append("    if (si >= a.n_i) return;")
+    out.append("    const int sj_raw = (blk - si * nchunk) * 64 + (int) threadIdx.x;")
+    out.append("    const bool active = sj_raw < a.n_j;")
+    out.append("    const int sj = active ? sj_raw : a.n_j - 1;")
+    out.append("    const int ip0 = a.i_prim_off[si], ip1 = a.i_prim_off[si + 1];")
+    out.append("    const int jp0 = a.j_prim_off[sj], jp1 = a.j_prim_off[sj + 1];")
+    for mi in range(ni):
+        for mj in range(nj):
+            out.append(f"    double acc_{mi}_{mj} = 0;")
+    out.append("    for (int ip = ip0; ip < ip1; ++ip) {")
+    out.append("      const double p = a.i_A[4 * ip + 3], ci = a.i_coef[ip];")
+    out.append("      const double Ax = a.i_A[4 * ip + 0], Ay = a.i_A[4 * ip + 1], Az = a.i_A[4 * ip + 2];")
+    out.append("      for (int jp = jp0; jp < jp1; ++jp) {")
+    out.append("        const double q = a.j_A[4 * jp + 3];")
+    out.append("        const double PQx = Ax - a.j_A[4 * jp + 0], PQy = Ay - a.j_A[4 * jp + 1], PQz = Az - a.j_A[4 * jp + 2];")
+    out.append("        const double coef = ci * a.j_coef[jp];")
+    out.append("        double zeta, pf;")
+    out.append("        if (a.omega == 0.0) { zeta = p * q / (p + q); pf = coef * eip::kTwoPi25 / (p * q * sqrt(p + q)); }")
+    out.append("        else { const double w2 = a.omega * a.omega; zeta = 1.0 / (1.0 / p + 1.0 / q + 1.0 / w2);")
+    out.append("               pf = coef * eip::kTwoPi25 * a.omega / (p * q * sqrt(p * q + (p + q) * w2)); }")
+    for side, l, ex in (("i", li, "p"), ("j", lj, "q")):
+        if l > 0:
+            out.append(f"        {{ const double h = 0.5 / {ex}; double hp = h;")
+            for _ in range(l - 1):
+                out.append("          hp *= h;")
+            sign = "-" if (side == "j" and l % 2) else ""
+            out.append(f"          pf *= {sign}hp; }}")
+    out.append("        const double T = zeta * (PQx * PQx + PQy * PQy + PQz * PQz);")
+    out.append(f"        double F[{L + 1}];")
+    out.append(f"        eip::boys64<{L}>(T, F);")
+    out.append("        const double m2z = -2.0 * zeta;")
+    out.append("        const double R000_0 = pf * F[0];")
+    for n in range(1, L + 1):
+        out.append(f"        const double pf_{n} = {'pf' if n == 1 else f'pf_{n - 1}'} * m2z;")
+        out.append(f"        const double R000_{n} = pf_{n} * F[{n}];")
+    needed = set()
+    for _, ti in ri:
+        for t in ti:
+            for _, sj in rj:
+                for s in sj:
+                    needed.add((t[0] + s[0], t[1] + s[1], t[2] + s[2]))
+    _emit_R(out, L, needed, "double")
+    for mi, (r1, t1) in enumerate(ri):
+        for mj, (r2, t2) in enumerate(rj):
+            pieces = []
+            for t, (q1, j1) in t1.items():
+                for s, (q2, j2) in t2.items():
+                    assert j1 == 0 and j2 == 0
+                    coef = SqrtRational(r1 * r2, q1 * q2).to_float()
+                    rn = _r_name(0, (t[0] + s[0], t[1] + s[1], t[2] + s[2]))
+                    pieces.append(rn if coef == 1.0 else f"{fmt(coef)} * {rn}")
+            out.append(f"        acc_{mi}_{mj} += {' + '.join(pieces)};")
+    out.append("      }")
+    out.append("    }")
+    out.append("    if (!active) return;")
+    out.append("    const int I = a.i_col[si], J = a.j_col[sj];")
+    out.append("    if (a.same_class && I > J) return;   // reference two_center.cpp:138")
+    out.append("    double* __restrict__ o = a.out;")
+    for mi in range(ni):
+        for mj in range(nj):
+            out.append(f"    o[(long long) (I + {mi}) * a.ld + J + {mj}] = acc_{mi}_{mj};")
+    out.append("    if (I != J && a.mirror) {")
+    for mi in range(ni):
+        for mj in range(nj):
+            out.append(f"      o[(long long) (J + {mj}) * a.ld + I + {mi}] = acc_{mi}_{mj};")
+    out.append("    }")
+    out.append("}")
+    return name, "\n".join(out)
+
+
+# ---------------------------------------------------------------------------------------------
+# Work split of a class over kernels (register budget)
+# ---------------------------------------------------------------------------------------------
+
+def m_chunks(la, lb, lc, budget=96):
+    """Split the 2lc+1 auxiliary components so acc registers stay near 'budget' values."""
+    nij = (2 * la + 1) * (2 * lb + 1)
+    nk = 2 * lc + 1
+    per = max(1, min(nk, budget // nij))
+    n = -(-nk // per)
+    per = -(-nk // n)          # balance
+    return [(m, min(nk, m + per)) for m in range(0, nk, per)]

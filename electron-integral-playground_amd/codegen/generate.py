@@ -1,0 +1,194 @@
+"""Write every generated source of the HIP library into csrc/generated/.
+
+Usage: python -m codegen.generate <out_dir>   (run from the package directory), or generate(out_dir).
+
+Specialised straight-line kernels cover the classes the workloads use (orbital l <= 3, auxiliary
+l <= 4, spherical output); every other class the reference's C ABI accepts (orbital l <= 6, auxiliary
+l <= 9, Cartesian output) goes to the hand-written generic kernels in csrc/generic_kernels.hip.
+"""
+
+import os
+import sys
+
+from . import boys_table
+from .gen_kernels import (emit_j2c, emit_j3c, emit_nuc, emit_pairE, m_chunks, pair_layout)
+
+SPEC_MAX_L = 3
+SPEC_MAX_AUX_L = 4
+
+HEADER = """// Generated by codegen/generate.py -- do not edit.
+#include "../md_device.hpp"
+#include "../md_args.hpp"
+"""
+
+
+def _write(path, text):
+    old = None
+    if os.path.exists(path):
+        with open(path) as f:
+            old = f.read()
+    if old != text:                       # keep mtimes stable for incremental builds
+        with open(path, "w") as f:
+            f.write(text)
+
+
+def generate(out_dir):
+    os.makedirs(out_dir, exist_ok=True)
+    boys_path = os.path.join(out_dir, "boys_table.inc")
+    if not os.path.exists(boys_path):
+        boys_table.emit(boys_path)
+
+    pair_classes = [(la, lb) for lb in range(SPEC_MAX_L + 1) for la in range(lb + 1)]
+    reg = ['// Generated by codegen/generate.py -- host-side dispatch to the specialised kernels.',
+           '#include "../md_args.hpp"',
+           f'#define EIP_SPEC_MAX_L {SPEC_MAX_L}',
+           f'#define EIP_SPEC_MAX_AUX_L {SPEC_MAX_AUX_L}']
+    disp = ['// Generated by codegen/generate.py -- host-side dispatch to the specialised kernels.',
+            '#include "registry.hpp"', '']
+
+    # ---- pair preparation ----
+    src = [HEADER]
+    nnz_table = {}
+    cases = []
+    for la, lb in pair_classes:
+        name, nnz, code = emit_pairE(la, lb, True)
+        nnz_table[(la, lb)] = nnz
+        src.append(code)
+        cases.append(f"        case {la * 10 + lb}: hipLaunchKernelGGL({name}, dim3(nblocks), dim3(256), 0, s, a); return 0;")
+    src.append("int eip_launch_pairE(int la, int lb, const PairPrepArgs& a, int nblocks, hipStream_t s)\n{\n"
+               "    switch (la * 10 + lb) {\n" + "\n".join(cases) + "\n    }\n    return -1;\n}\n")
+    _write(os.path.join(out_dir, "pairE.hip"), "\n".join(src))
+    reg.append("int eip_launch_pairE(int la, int lb, const PairPrepArgs& a, int nblocks, hipStream_t s);")
+    disp.append("int eip_pair_nnz(int la, int lb)\n{\n    switch (la * 10 + lb) {")
+    for (la, lb), nnz in nnz_table.items():
+        disp.append(f"        case {la * 10 + lb}: return {nnz};")
+    disp.append("    }\n    return -1;\n}\n")
+    reg.append("int eip_pair_nnz(int la, int lb);")
+
+    # ---- three-centre ----
+    chunk_rows = []
+    for la, lb in pair_classes:
+        for prec in ("f64", "f32"):
+            src = [HEADER]
+            cases = []
+            for lc in range(SPEC_MAX_AUX_L + 1):
+                for ci, (m0, m1) in enumerate(m_chunks(la, lb, lc)):
+                    name, code = emit_j3c(la, lb, lc, m0, m1, prec)
+                    src.append(code)
+                    cases.append(f"        case {lc * 100 + ci}: hipLaunchKernelGGL({name}, dim3(nblocks), dim3(64), 0, s, a); return 0;")
+                    if prec == "f64":
+                        chunk_rows.append((la, lb, lc, ci, m0, m1))
+            fn = f"eip_launch_j3c_{la}{lb}_{prec}"
+            src.append(f"int {fn}(int lc, int chunk, const J3cArgs& a, int nblocks, hipStream_t s)\n{{\n"
+                       "    switch (lc * 100 + chunk) {\n" + "\n".join(cases) + "\n    }\n    return -1;\n}\n")
+            _write(os.path.join(out_dir, f"j3c_{la}{lb}_{prec}.hip"), "\n".join(src))
+            reg.append(f"int {fn}(int lc, int chunk, const J3cArgs& a, int nblocks, hipStream_t s);")
+    reg.append("int eip_launch_j3c(int la, int lb, int lc, int chunk, int f32, const J3cArgs& a, int nblocks, hipStream_t s);")
+    reg.append("int eip_j3c_nchunks(int la, int lb, int lc);")
+    reg.append("int eip_j3c_chunk(int la, int lb, int lc, int chunk, int* m0, int* m1);")
+    disp.append("int eip_launch_j3c(int la, int lb, int lc, int chunk, int f32, const J3cArgs& a, int nblocks, hipStream_t s)\n{")
+    disp.append("    switch (la * 10 + lb) {")
+    for la, lb in pair_classes:
+        disp.append(f"        case {la * 10 + lb}: return f32 ? eip_launch_j3c_{la}{lb}_f32(lc, chunk, a, nblocks, s)"
+                    f" : eip_launch_j3c_{la}{lb}_f64(lc, chunk, a, nblocks, s);")
+    disp.append("    }\n    return -1;\n}\n")
+    disp.append("static const short kChunks[][6] = {")
+    for row in chunk_rows:
+        disp.append("    {%d, %d, %d, %d, %d, %d}," % row)
+    disp.append("};")
+    disp.append("int eip_j3c_nchunks(int la, int lb, int lc)\n{\n    int n = 0;\n"
+                "    for (const auto& r : kChunks) n += (r[0] == la && r[1] == lb && r[2] == lc);\n    return n;\n}\n")
+    disp.append("int eip_j3c_chunk(int la, int lb, int lc, int chunk, int* m0, int* m1)\n{\n"
+                "    for (const auto& r : kChunks)\n"
+                "        if (r[0] == la && r[1] == lb && r[2] == lc && r[3] == chunk) { *m0 = r[4]; *m1 = r[5]; return 0; }\n"
+                "    return -1;\n}\n")
+
+    # ---- nuclear attraction ----
+    src = [HEADER]
+    cases = []
+    for la, lb in pair_classes:
+        name, code = emit_nuc(la, lb)
+        src.append(code)
+        cases.append(f"        case {la * 10 + lb}: hipLaunchKernelGGL({name}, dim3(nblocks), dim3(64), 0, s, a); return 0;")
+    src.append("int eip_launch_nuc(int la, int lb, const NucArgs& a, int nblocks, hipStream_t s)\n{\n"
+               "    switch (la * 10 + lb) {\n" + "\n".join(cases) + "\n    }\n    return -1;\n}\n")
+    _write(os.path.join(out_dir, "nuc.hip"), "\n".join(src))
+    reg.append("int eip_launch_nuc(int la, int lb, const NucArgs& a, int nblocks, hipStream_t s);")
+
+    # ---- two-centre ----
+    src = [HEADER]
+    cases = []
+    for lj in range(SPEC_MAX_AUX_L + 1):
+        for li in range(lj + 1):
+            name, code = emit_j2c(li, lj)
+            src.append(code)
+            cases.append(f"        case {li * 10 + lj}: hipLaunchKernelGGL({name}, dim3(nblocks), dim3(64), 0, s, a); return 0;")
+    src.append("int eip_launch_j2c(int li, int lj, const J2cArgs& a, int nblocks, hipStream_t s)\n{\n"
+               "    switch (li * 10 + lj) {\n" + "\n".join(cases) + "\n    }\n    return -1;\n}\n")
+    _write(os.path.join(out_dir, "j2c.hip"), "\n".join(src))
+    reg.append("int eip_launch_j2c(int li, int lj, const J2cArgs& a, int nblocks, hipStream_t s);")
+
+    _write(os.path.join(out_dir, "registry.hpp"), "\n".join(reg) + "\n")
+    _write(os.path.join(out_dir, "registry.cpp"), "\n".join(disp) + "\n")
+    emit_generic_tables(out_dir)
+    return sorted(f for f in os.listdir(out_dir) if f.endswith((".hip", ".cpp")))
+
+
+def emit_generic_tables(out_dir):
+    """Tables for the runtime-generic kernels (every l <= 9, spherical and Cartesian).
+
+    * basis rows: for (l, spherical) the output rows as lists of (cartesian index, coefficient);
+    * auxiliary rows: for (l, spherical) the Hermite-to-output transform rows as lists of
+      (sx, sy, sz, j, coefficient), see tables.aux_transform;
+    * reverse Hermite coefficients H(t, m) = t! / ((2m-t)! (t-m)! 2^(t-m)) for the closed form of R.
+    """
+    from math import factorial
+    from .tables import SqrtRational, aux_transform, out_basis_rows
+
+    lmax = 9
+    brow_off, bterm_c, bterm_v = [], [], []
+    brow_base = []          # per (l, sph): first row index into brow_off
+    arow_off, aterm, aterm_v = [], [], []
+    arow_base = []
+    for sph in (0, 1):
+        for l in range(lmax + 1):
+            brow_base.append(len(brow_off))
+            for r, terms in out_basis_rows(l, bool(sph)):
+                brow_off.append(len(bterm_c))
+                for c, q in sorted(terms.items()):
+                    bterm_c.append(c)
+                    bterm_v.append(SqrtRational(r, q).to_float())
+            arow_base.append(len(arow_off))
+            for r, terms in aux_transform(l, bool(sph)):
+                arow_off.append(len(aterm))
+                for s, (q, j) in sorted(terms.items()):
+                    aterm.append(s[0] | (s[1] << 5) | (s[2] << 10) | (j << 15))
+                    aterm_v.append(SqrtRational(r, q).to_float())
+    brow_off.append(len(bterm_c))
+    arow_off.append(len(aterm))
+    H = []
+    for t in range(25):
+        row = []
+        for m in range(25):
+            if m < (t + 1) // 2 or m > t:
+                row.append(0.0)
+            else:
+                row.append(factorial(t) / (factorial(2 * m - t) * factorial(t - m) * 2 ** (t - m)))
+        H.append(row)
+
+    def arr(ctype, name, vals, f=str):
+        return f"__device__ const {ctype} {name}[{len(vals)}] = {{" + ", ".join(f(v) for v in vals) + "};"
+
+    from .tables import fmt
+    lines = ["// Generated by codegen/generate.py -- tables for the generic kernels.", "#pragma once",
+             "#define GEN_LMAX 9",
+             arr("int", "gen_brow_base", brow_base), arr("int", "gen_brow_off", brow_off),
+             arr("int", "gen_bterm_c", bterm_c), arr("double", "gen_bterm_v", bterm_v, fmt),
+             arr("int", "gen_arow_base", arow_base), arr("int", "gen_arow_off", arow_off),
+             arr("int", "gen_aterm", aterm), arr("double", "gen_aterm_v", aterm_v, fmt),
+             arr("double", "gen_herm", [v for row in H for v in row], fmt)]
+    _write(os.path.join(out_dir, "generic_tables.inc"), "\n".join(lines) + "\n")
+
+
+if __name__ == "__main__":
+    print(generate(sys.argv[1]))

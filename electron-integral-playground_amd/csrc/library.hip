@@ -1,0 +1,484 @@
+// libmolecular_integral.so -- MI355X (gfx950) McMurchie-Davidson integral library.
+//
+// Two groups of C entry points (declared in include/molecular_integral.h):
+//
+// 1. The reference's four C-ABI functions, same names, signatures, accumulate (+=) semantics, mirror
+//    rule and return codes, so the reference's ctypes callers work unchanged
+//    (reference c_library/molecular_integral/{overlap,nuclear_attraction,two_center,three_center}.cpp).
+//    Each call copies its inputs to the GPU, groups primitive pairs by contracted shell pair and
+//    auxiliary primitives by auxiliary shell (the reference evaluates each primitive triplet on its own
+//    and accumulates with atomics; here each output block has exactly one owner on the device), runs
+//    the class's kernels into a compact block buffer, copies it back and adds it into the caller's
+//    array, writing the transposed element too when the two AO offsets differ.
+//
+// 2. Device-resident entry points (eip_*) over grouped data already in HBM: pair-matrix preparation,
+//    three-centre over a work-item list, two-centre and nuclear attraction.  They take device
+//    pointers and a stream, allocate nothing and never synchronise.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <thread>
+#include <vector>
+
+#include "md_args.hpp"
+#include "generic.hpp"
+#include "generated/registry.hpp"
+
+namespace {
+
+constexpr int kMaxL = 6;       // orbital shells (reference angular.h:4)
+constexpr int kMaxAuxL = 9;    // auxiliary shells (reference angular.h:5)
+
+inline int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+inline int nout(int l, bool sph) { return sph ? 2 * l + 1 : ncart(l); }
+
+#define EIP_CHECK(expr)                                                                    \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            fprintf(stderr, "libmolecular_integral: %s failed: %s (%s:%d)\n", #expr,       \
+                    hipGetErrorString(e_), __FILE__, __LINE__);                            \
+            return 2;                                                                      \
+        }                                                                                  \
+    } while (0)
+
+// One non-blocking stream per host thread: concurrent ctypes calls (the GIL is released) never
+// share a stream or a buffer.
+hipStream_t thread_stream()
+{
+    thread_local hipStream_t s = nullptr;
+    if (!s && hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
+    return s;
+}
+
+// Device buffer owned by one call, released on scope exit.
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() { if (p) (void) hipFree(p); }
+    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 8); }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+template <class T>
+hipError_t upload(DevBuf& b, const std::vector<T>& v, hipStream_t s)
+{
+    hipError_t e = b.alloc(v.size() * sizeof(T));
+    if (e != hipSuccess) return e;
+    return hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
+}
+
+// Stable grouping of items by an integer key pair.  Returns the permutation (items in group order),
+// group offsets (CSR) and the key pair of each group.
+struct Grouping {
+    std::vector<int> order, off, key_a, key_b;
+};
+
+Grouping group_by(const int* a, const int* b, int n)
+{
+    Grouping g;
+    g.order.resize(n);
+    std::iota(g.order.begin(), g.order.end(), 0);
+    std::stable_sort(g.order.begin(), g.order.end(), [&](int x, int y) {
+        const int bx = b ? b[x] : 0, by = b ? b[y] : 0;
+        return a[x] != a[y] ? a[x] < a[y] : bx < by;
+    });
+    for (int k = 0; k < n; ++k) {
+        const int i = g.order[k];
+        const int kb = b ? b[i] : 0;
+        if (k == 0 || a[i] != g.key_a.back() || kb != g.key_b.back()) {
+            g.off.push_back(k);
+            g.key_a.push_back(a[i]);
+            g.key_b.push_back(kb);
+        }
+    }
+    g.off.push_back(n);
+    return g;
+}
+
+template <class T>
+std::vector<T> gather_rows(const T* src, const std::vector<int>& order, int width)
+{
+    std::vector<T> out(order.size() * (size_t) width);
+    for (size_t k = 0; k < order.size(); ++k)
+        std::memcpy(&out[k * width], src + (size_t) order[k] * width, sizeof(T) * width);
+    return out;
+}
+
+// Run f(begin, end) over [0, n) on the host's cores.
+template <class F>
+void parallel_for(long long n, F f)
+{
+    unsigned nt = std::max(1u, std::min(std::thread::hardware_concurrency(), 32u));
+    if (n < 4096 || nt == 1) { f(0LL, n); return; }
+    std::vector<std::thread> th;
+    const long long per = (n + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+        const long long b = t * per, e = std::min(n, b + per);
+        if (b < e) th.emplace_back(f, b, e);
+    }
+    for (auto& t : th) t.join();
+}
+
+bool unsupported(const char* fn, int i_L, int j_L, int k_L = -1)
+{
+    if (k_L >= 0) printf("%s function does not support angular i_L = %d, j_L = %d, k_L = %d\n", fn, i_L, j_L, k_L);
+    else printf("%s function does not support angular i_L = %d, j_L = %d\n", fn, i_L, j_L);
+    fflush(stdout);
+    return true;
+}
+
+// Grouped primitive pairs of one orbital class, resident on the device.
+struct DevPairs {
+    Grouping g;
+    DevBuf P, A, B, coef, off, E;
+    int n_pp = 0, n_sp = 0;
+    bool spec = false;     // packed specialised pair matrix (else dense generic)
+    int nnz = 0;           // entries per primitive pair
+};
+
+int prepare_pairs(DevPairs& d, int la, int lb, bool sph, const double* P_p, const double* A_a, const double* B_b,
+                  const double* coef, const int* i_start, const int* j_start, int n, bool recompute_P, hipStream_t s)
+{
+    d.g = group_by(i_start, j_start, n);
+    d.n_pp = n;
+    d.n_sp = (int) d.g.key_a.size();
+    std::vector<double> hP = P_p ? gather_rows(P_p, d.g.order, 4) : std::vector<double>(4 * (size_t) n, 0.0);
+    std::vector<double> hA = gather_rows(A_a, d.g.order, 4), hB = gather_rows(B_b, d.g.order, 4);
+    std::vector<double> hc = gather_rows(coef, d.g.order, 1);
+    EIP_CHECK(upload(d.P, hP, s));
+    EIP_CHECK(upload(d.A, hA, s));
+    EIP_CHECK(upload(d.B, hB, s));
+    EIP_CHECK(upload(d.coef, hc, s));
+    EIP_CHECK(upload(d.off, d.g.off, s));
+    d.spec = sph && la <= EIP_SPEC_MAX_L && lb <= EIP_SPEC_MAX_L;
+    if (d.spec) {
+        d.nnz = eip_pair_nnz(la, lb);
+        EIP_CHECK(d.E.alloc(sizeof(double) * (size_t) n * d.nnz));
+        PairPrepArgs a{d.P.as<double>(), d.A.as<double>(), d.B.as<double>(), d.coef.as<double>(),
+                       d.E.as<double>(), nullptr, n, recompute_P ? 1 : 0};
+        if (eip_launch_pairE(la, lb, a, (n + 255) / 256, s) != 0) return 3;
+    } else {
+        const int L = la + lb;
+        d.nnz = nout(la, sph) * nout(lb, sph) * (L + 1) * (L + 2) * (L + 3) / 6;
+        EIP_CHECK(d.E.alloc(sizeof(double) * (size_t) n * d.nnz));
+        GenPairArgs a{la, lb, sph ? 1 : 0, recompute_P ? 1 : 0, d.P.as<double>(), d.A.as<double>(),
+                      d.B.as<double>(), d.coef.as<double>(), n, d.E.as<double>()};
+        EIP_CHECK((hipError_t) eip_gen_pair_matrix(a, s));
+    }
+    EIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------------------------------------
+// Reference C ABI (drop-in)
+// ---------------------------------------------------------------------------------------------
+
+int overlap(int i_L, int j_L, const double* pair_A_a, const double* pair_B_b, const double* pair_coefficient,
+            const int* pair_i_ao_start, const int* pair_j_ao_start, int n_pair, double* S_matrix, int n_ao,
+            bool spherical)
+{
+    if (!(i_L >= 0 && j_L >= 0 && i_L <= j_L && j_L <= kMaxL) && unsupported(__func__, i_L, j_L)) return 1;
+    if (n_pair <= 0) return 0;
+    hipStream_t s = thread_stream();
+    DevPairs d;
+    if (int rc = prepare_pairs(d, i_L, j_L, spherical, nullptr, pair_A_a, pair_B_b, pair_coefficient,
+                               pair_i_ao_start, pair_j_ao_start, n_pair, true, s)) return rc;
+    const int ni = nout(i_L, spherical), nj = nout(j_L, spherical), nrow = ni * nj;
+    DevBuf out;
+    EIP_CHECK(out.alloc(sizeof(double) * (size_t) d.n_sp * nrow));
+    const int L = i_L + j_L;
+    GenOverlapArgs a{d.off.as<int>(), d.A.as<double>(), d.B.as<double>(), d.E.as<double>(),
+                     d.nnz, d.spec ? 1 : (long long) (L + 1) * (L + 2) * (L + 3) / 6, d.n_sp, nrow, out.as<double>()};
+    EIP_CHECK((hipError_t) eip_gen_overlap(a, s));
+    std::vector<double> h((size_t) d.n_sp * nrow);
+    EIP_CHECK(hipMemcpyAsync(h.data(), out.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    EIP_CHECK(hipStreamSynchronize(s));
+    for (int sp = 0; sp < d.n_sp; ++sp) {
+        const int I = d.g.key_a[sp], J = d.g.key_b[sp];
+        for (int i = 0; i < ni; ++i)
+            for (int j = 0; j < nj; ++j) {
+                const double v = h[(size_t) sp * nrow + i * nj + j];
+                S_matrix[(size_t) (I + i) * n_ao + (J + j)] += v;
+                if (I != J) S_matrix[(size_t) (I + i) + (size_t) (J + j) * n_ao] += v;
+            }
+    }
+    return 0;
+}
+
+int nuclear_attraction(int i_L, int j_L, const double* pair_P_p, const double* pair_A_a, const double* pair_B_b,
+                       const double* pair_coefficient, const int* pair_i_ao_start, const int* pair_j_ao_start,
+                       int n_pair, const double* charge_position, int n_charge, double* V_tensor, int n_ao,
+                       bool spherical, double omega)
+{
+    if (!(i_L >= 0 && j_L >= 0 && i_L <= j_L && j_L <= kMaxL) && unsupported(__func__, i_L, j_L)) return 1;
+    if (n_pair <= 0 || n_charge <= 0) return 0;
+    hipStream_t s = thread_stream();
+    DevPairs d;
+    if (int rc = prepare_pairs(d, i_L, j_L, spherical, pair_P_p, pair_A_a, pair_B_b, pair_coefficient,
+                               pair_i_ao_start, pair_j_ao_start, n_pair, false, s)) return rc;
+    const int ni = nout(i_L, spherical), nj = nout(j_L, spherical), nrow = ni * nj;
+    DevBuf charge, out;
+    EIP_CHECK(charge.alloc(sizeof(double) * 3 * (size_t) n_charge));
+    EIP_CHECK(hipMemcpyAsync(charge.p, charge_position, sizeof(double) * 3 * (size_t) n_charge, hipMemcpyHostToDevice, s));
+    EIP_CHECK(out.alloc(sizeof(double) * (size_t) d.n_sp * nrow * n_charge));
+    if (d.spec) {
+        NucArgs a{d.off.as<int>(), d.P.as<double>(), d.E.as<double>(), charge.as<double>(), d.n_sp, n_charge,
+                  out.as<double>(), n_charge, omega};
+        const int nchunk = (n_charge + 63) / 64;
+        if (eip_launch_nuc(i_L, j_L, a, d.n_sp * nchunk, s) != 0) return 3;
+    } else {
+        GenNucArgs a{i_L, j_L, spherical ? 1 : 0, d.off.as<int>(), d.P.as<double>(), d.E.as<double>(), d.n_sp,
+                     charge.as<double>(), n_charge, out.as<double>(), n_charge, omega};
+        EIP_CHECK((hipError_t) eip_gen_nuclear(a, s));
+    }
+    EIP_CHECK(hipGetLastError());
+    std::vector<double> h((size_t) d.n_sp * nrow * n_charge);
+    EIP_CHECK(hipMemcpyAsync(h.data(), out.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    EIP_CHECK(hipStreamSynchronize(s));
+    for (int sp = 0; sp < d.n_sp; ++sp) {
+        const int I = d.g.key_a[sp], J = d.g.key_b[sp];
+        for (int i = 0; i < ni; ++i)
+            for (int j = 0; j < nj; ++j) {
+                const double* v = &h[((size_t) sp * nrow + i * nj + j) * n_charge];
+                double* o = V_tensor + ((size_t) (I + i) * n_ao + (J + j)) * n_charge;
+                for (int c = 0; c < n_charge; ++c) o[c] += v[c];
+                if (I != J) {
+                    double* m = V_tensor + ((size_t) (I + i) + (size_t) (J + j) * n_ao) * n_charge;
+                    for (int c = 0; c < n_charge; ++c) m[c] += v[c];
+                }
+            }
+    }
+    return 0;
+}
+
+int two_center(int i_L, int j_L, const double* aux_A_a, const double* aux_i_coefficient, const int* aux_i_aux_start,
+               int n_aux_primitive_i, const double* aux_B_b, const double* aux_j_coefficient,
+               const int* aux_j_aux_start, int n_aux_primitive_j, double* J2c_matrix, int n_aux, bool spherical,
+               double omega)
+{
+    if (!(i_L >= 0 && j_L >= 0 && i_L <= j_L && j_L <= kMaxAuxL) && unsupported(__func__, i_L, j_L)) return 1;
+    if (n_aux_primitive_i <= 0 || n_aux_primitive_j <= 0) return 0;
+    hipStream_t s = thread_stream();
+    Grouping gi = group_by(aux_i_aux_start, nullptr, n_aux_primitive_i);
+    Grouping gj = group_by(aux_j_aux_start, nullptr, n_aux_primitive_j);
+    const int n_i = (int) gi.key_a.size(), n_j = (int) gj.key_a.size();
+    const int ni = nout(i_L, spherical), nj = nout(j_L, spherical);
+    DevBuf iA, ic, io, jA, jc, jo, icol, jcol, out;
+    EIP_CHECK(upload(iA, gather_rows(aux_A_a, gi.order, 4), s));
+    EIP_CHECK(upload(ic, gather_rows(aux_i_coefficient, gi.order, 1), s));
+    EIP_CHECK(upload(io, gi.off, s));
+    EIP_CHECK(upload(jA, gather_rows(aux_B_b, gj.order, 4), s));
+    EIP_CHECK(upload(jc, gather_rows(aux_j_coefficient, gj.order, 1), s));
+    EIP_CHECK(upload(jo, gj.off, s));
+    std::vector<int> hic(n_i), hjc(n_j);
+    for (int k = 0; k < n_i; ++k) hic[k] = k * ni;
+    for (int k = 0; k < n_j; ++k) hjc[k] = k * nj;
+    EIP_CHECK(upload(icol, hic, s));
+    EIP_CHECK(upload(jcol, hjc, s));
+    const long long ld = (long long) n_j * nj;
+    EIP_CHECK(out.alloc(sizeof(double) * (size_t) n_i * ni * ld));
+    if (spherical && j_L <= EIP_SPEC_MAX_AUX_L) {
+        J2cArgs a{io.as<int>(), iA.as<double>(), ic.as<double>(), icol.as<int>(), jo.as<int>(), jA.as<double>(),
+                  jc.as<double>(), jcol.as<int>(), n_i, n_j, 0, 0, out.as<double>(), ld, omega};
+        if (eip_launch_j2c(i_L, j_L, a, n_i * ((n_j + 63) / 64), s) != 0) return 3;
+    } else {
+        GenJ2cArgs a{i_L, j_L, spherical ? 1 : 0, io.as<int>(), iA.as<double>(), ic.as<double>(), n_i,
+                     jo.as<int>(), jA.as<double>(), jc.as<double>(), n_j, out.as<double>(), ld, omega};
+        EIP_CHECK((hipError_t) eip_gen_two_center(a, s));
+    }
+    EIP_CHECK(hipGetLastError());
+    std::vector<double> h((size_t) n_i * ni * ld);
+    EIP_CHECK(hipMemcpyAsync(h.data(), out.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    EIP_CHECK(hipStreamSynchronize(s));
+    for (int si = 0; si < n_i; ++si)
+        for (int sj = 0; sj < n_j; ++sj) {
+            const int I = gi.key_a[si], J = gj.key_a[sj];
+            if (i_L == j_L && I > J) continue;                 // reference two_center.cpp:138
+            for (int i = 0; i < ni; ++i)
+                for (int j = 0; j < nj; ++j) {
+                    const double v = h[((size_t) si * ni + i) * ld + (size_t) sj * nj + j];
+                    J2c_matrix[(size_t) (I + i) * n_aux + (J + j)] += v;
+                    if (I != J) J2c_matrix[(size_t) (I + i) + (size_t) (J + j) * n_aux] += v;
+                }
+        }
+    return 0;
+}
+
+int three_center(int i_L, int j_L, int k_L, const double* pair_P_p, const double* pair_A_a, const double* pair_B_b,
+                 const double* pair_coefficient, const int* pair_i_ao_start, const int* pair_j_ao_start, int n_pair_ij,
+                 const double* aux_A_a, const double* aux_coefficient, const int* aux_i_aux_start,
+                 int n_aux_primitive_k, double* J3c_matrix, int n_ao, int n_aux, bool spherical, double omega)
+{
+    if (!(i_L >= 0 && j_L >= 0 && i_L <= j_L && j_L <= kMaxL && k_L >= 0 && k_L <= kMaxAuxL)
+        && unsupported(__func__, i_L, j_L, k_L)) return 1;
+    (void) n_aux;
+    if (n_pair_ij <= 0 || n_aux_primitive_k <= 0) return 0;
+    hipStream_t s = thread_stream();
+    DevPairs d;
+    if (int rc = prepare_pairs(d, i_L, j_L, spherical, pair_P_p, pair_A_a, pair_B_b, pair_coefficient,
+                               pair_i_ao_start, pair_j_ao_start, n_pair_ij, false, s)) return rc;
+    Grouping gk = group_by(aux_i_aux_start, nullptr, n_aux_primitive_k);
+    const int n_ks = (int) gk.key_a.size();
+    const int ni = nout(i_L, spherical), nj = nout(j_L, spherical), nk = nout(k_L, spherical), nrow = ni * nj;
+    DevBuf kC, kc, ko, kcol;
+    EIP_CHECK(upload(kC, gather_rows(aux_A_a, gk.order, 4), s));
+    EIP_CHECK(upload(kc, gather_rows(aux_coefficient, gk.order, 1), s));
+    EIP_CHECK(upload(ko, gk.off, s));
+    std::vector<int> hkcol(n_ks);
+    for (int k = 0; k < n_ks; ++k) hkcol[k] = k * nk;
+    EIP_CHECK(upload(kcol, hkcol, s));
+    const long long ld = (long long) n_ks * nk;
+    // Bound the compact buffer: process shell pairs in batches of at most ~256 MB of output.
+    const long long per_sp = (long long) nrow * ld;
+    const int batch = (int) std::max(1LL, std::min<long long>(d.n_sp, (256LL << 20) / 8 / std::max(1LL, per_sp)));
+    const bool spec = d.spec && k_L <= EIP_SPEC_MAX_AUX_L;
+    DevBuf out, items, spi, spj;
+    EIP_CHECK(out.alloc(sizeof(double) * (size_t) batch * per_sp));
+    std::vector<double> h((size_t) batch * per_sp);
+    if (spec) {
+        EIP_CHECK(upload(spi, d.g.key_a, s));
+        EIP_CHECK(upload(spj, d.g.key_b, s));
+    }
+    const int kblocks = (n_ks + 63) / 64;
+    for (int sp0 = 0; sp0 < d.n_sp; sp0 += batch) {
+        const int nb = std::min(batch, d.n_sp - sp0);
+        if (spec) {
+            // items: (sp, first aux shell) with compact rows relative to sp0
+            std::vector<int> hit;
+            hit.reserve((size_t) nb * kblocks * 2);
+            for (int sp = sp0; sp < sp0 + nb; ++sp)
+                for (int kb = 0; kb < kblocks; ++kb) { hit.push_back(sp); hit.push_back(kb * 64); }
+            DevBuf it;
+            EIP_CHECK(upload(it, hit, s));
+            // out pointer shifted so that row (sp * nrow + ij) lands at (sp - sp0) * nrow + ij
+            J3cArgs a{};
+            a.items = it.as<int>();
+            a.items_ks_end = nullptr;
+            a.item_begin = 0;
+            a.item_end = nb * kblocks;
+            a.sp_pp_off = d.off.as<int>();
+            a.sp_i = spi.as<int>();
+            a.sp_j = spj.as<int>();
+            a.pp_P = d.P.as<double>();
+            a.E = d.E.as<double>();
+            a.E32 = nullptr;
+            a.ks_perm = nullptr;
+            a.ks_prim_off = ko.as<int>();
+            a.ks_col = kcol.as<int>();
+            a.ka_C = kC.as<double>();
+            a.ka_coef = kc.as<double>();
+            a.n_ks = n_ks;
+            a.out = out.as<double>() - (long long) sp0 * nrow * ld;
+            a.ld = ld;
+            a.layout = 1;
+            a.omega = omega;
+            const int nchunks = eip_j3c_nchunks(i_L, j_L, k_L);
+            for (int c = 0; c < nchunks; ++c)
+                if (eip_launch_j3c(i_L, j_L, k_L, c, 0, a, nb * kblocks, s) != 0) return 3;
+            EIP_CHECK(hipGetLastError());
+            EIP_CHECK(hipStreamSynchronize(s));        // 'it' is released at scope exit
+        } else {
+            GenJ3cArgs a{i_L, j_L, k_L, spherical ? 1 : 0, d.off.as<int>() + sp0, d.P.as<double>(), d.E.as<double>(), nb,
+                         ko.as<int>(), kC.as<double>(), kc.as<double>(), n_ks, out.as<double>(), ld, omega};
+            EIP_CHECK((hipError_t) eip_gen_three_center(a, s));
+        }
+        EIP_CHECK(hipMemcpyAsync(h.data(), out.p, sizeof(double) * (size_t) nb * per_sp, hipMemcpyDeviceToHost, s));
+        EIP_CHECK(hipStreamSynchronize(s));
+        // Scatter-add: threads own disjoint auxiliary shells, hence disjoint rows of J3c.
+        const size_t nao2 = (size_t) n_ao * n_ao;
+        parallel_for(n_ks, [&](long long k0, long long k1) {
+            for (long long ks = k0; ks < k1; ++ks) {
+                const int K = gk.key_a[ks];
+                for (int sp = 0; sp < nb; ++sp) {
+                    const int I = d.g.key_a[sp0 + sp], J = d.g.key_b[sp0 + sp];
+                    for (int i = 0; i < ni; ++i)
+                        for (int j = 0; j < nj; ++j) {
+                            const double* v = &h[((size_t) sp * nrow + i * nj + j) * ld + (size_t) ks * nk];
+                            for (int k = 0; k < nk; ++k) {
+                                double* row = J3c_matrix + (size_t) (K + k) * nao2;
+                                row[(size_t) (I + i) * n_ao + (J + j)] += v[k];
+                                if (I != J) row[(size_t) (I + i) + (size_t) (J + j) * n_ao] += v[k];
+                            }
+                        }
+                }
+            }
+        });
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Device-resident entry points
+// ---------------------------------------------------------------------------------------------
+
+int eip_pair_matrix_nnz(int i_L, int j_L)
+{
+    if (!(i_L >= 0 && j_L >= 0 && i_L <= j_L && j_L <= EIP_SPEC_MAX_L)) return -1;
+    return eip_pair_nnz(i_L, j_L);
+}
+
+int eip_three_center_nchunks(int i_L, int j_L, int k_L)
+{
+    if (!(i_L >= 0 && j_L >= 0 && i_L <= j_L && j_L <= EIP_SPEC_MAX_L && k_L >= 0 && k_L <= EIP_SPEC_MAX_AUX_L)) return -1;
+    return eip_j3c_nchunks(i_L, j_L, k_L);
+}
+
+int eip_pair_matrix(int i_L, int j_L, const double* d_P_p, const double* d_A_a, const double* d_B_b,
+                    const double* d_coef, int n_pp, double* d_E, float* d_E32, void* stream)
+{
+    if (eip_pair_matrix_nnz(i_L, j_L) < 0) return 1;
+    if (n_pp <= 0) return 0;
+    PairPrepArgs a{d_P_p, d_A_a, d_B_b, d_coef, d_E, d_E32, n_pp, 0};
+    if (eip_launch_pairE(i_L, j_L, a, (n_pp + 255) / 256, (hipStream_t) stream) != 0) return 3;
+    return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int eip_three_center_items(int i_L, int j_L, int k_L, int use_f32, const int* d_items, const int* d_items_ks_end,
+                           int n_items, const int* d_sp_pp_off, const int* d_sp_i, const int* d_sp_j,
+                           const double* d_pp_P, const double* d_E, const float* d_E32, const int* d_ks_perm,
+                           const int* d_ks_prim_off, const int* d_ks_col, const double* d_ka_C,
+                           const double* d_ka_coef, int n_ks, double* d_out, long long ld, int layout,
+                           double omega, void* stream)
+{
+    const int nchunks = eip_three_center_nchunks(i_L, j_L, k_L);
+    if (nchunks < 0) return 1;
+    if (n_items <= 0) return 0;
+    J3cArgs a{d_items, d_items_ks_end, 0, n_items, d_sp_pp_off, d_sp_i, d_sp_j, d_pp_P, d_E, d_E32,
+              d_ks_perm, d_ks_prim_off, d_ks_col, d_ka_C, d_ka_coef, n_ks, d_out, ld, layout, omega};
+    for (int c = 0; c < nchunks; ++c)
+        if (eip_launch_j3c(i_L, j_L, k_L, c, use_f32, a, n_items, (hipStream_t) stream) != 0) return 3;
+    return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int eip_two_center_device(int i_L, int j_L, const int* d_i_prim_off, const double* d_i_A, const double* d_i_coef,
+                          const int* d_i_col, int n_i, const int* d_j_prim_off, const double* d_j_A,
+                          const double* d_j_coef, const int* d_j_col, int n_j, int same_class, int mirror,
+                          double* d_out, long long ld, double omega, void* stream)
+{
+    if (!(i_L >= 0 && j_L >= 0 && i_L <= j_L && j_L <= EIP_SPEC_MAX_AUX_L)) return 1;
+    if (n_i <= 0 || n_j <= 0) return 0;
+    J2cArgs a{d_i_prim_off, d_i_A, d_i_coef, d_i_col, d_j_prim_off, d_j_A, d_j_coef, d_j_col, n_i, n_j,
+              same_class, mirror, d_out, ld, omega};
+    if (eip_launch_j2c(i_L, j_L, a, n_i * ((n_j + 63) / 64), (hipStream_t) stream) != 0) return 3;
+    return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int eip_nuclear_device(int i_L, int j_L, const int* d_sp_pp_off, const double* d_pp_P, const double* d_E,
+                       int n_sp, const double* d_charge, int n_charge, double* d_out, long long ld, double omega,
+                       void* stream)
+{
+    if (eip_pair_matrix_nnz(i_L, j_L) < 0) return 1;
+    if (n_sp <= 0 || n_charge <= 0) return 0;
+    NucArgs a{d_sp_pp_off, d_pp_P, d_E, d_charge, n_sp, n_charge, d_out, ld, omega};
+    if (eip_launch_nuc(i_L, j_L, a, n_sp * ((n_charge + 63) / 64), (hipStream_t) stream) != 0) return 3;
+    return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+}  // extern "C"

@@ -1,0 +1,76 @@
+// Kernel argument blocks shared by the generated kernels and the host library.
+// All pointers are device pointers.  Passed by value (kernarg segment).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+// Primitive-pair preparation: builds the packed pair matrix (Hermite coefficients already
+// contracted into the output basis, pair coefficient folded in) for n_pp primitive pairs.
+struct PairPrepArgs {
+    const double* P_p;     // [n_pp][4] x, y, z, p
+    const double* A_a;     // [n_pp][4] x, y, z, a
+    const double* B_b;     // [n_pp][4] x, y, z, b
+    const double* coef;    // [n_pp]
+    double* E;             // [n_pp][nnz]
+    float* E32;            // optional float copy, or nullptr
+    int n_pp;
+    int recompute_P;       // 1: P and p from A, B (overlap)
+};
+
+// Three-centre (ij|P) for one (la, lb, lc) class over a list of work items.
+// Work item w = (sp, ks_first): the wave handles shell pair sp and auxiliary shells
+// ks_first .. ks_first + 63 (clipped at items_ks_end[w] or n_ks) of the (optionally permuted) list.
+struct J3cArgs {
+    const int* items;         // [n_items][2]
+    const int* items_ks_end;  // [n_items] or nullptr (then n_ks)
+    int item_begin, item_end;
+    // shell pairs
+    const int* sp_pp_off;     // [n_sp + 1] CSR offsets into the primitive-pair arrays
+    const int* sp_i;          // [n_sp] AO offset of shell i
+    const int* sp_j;          // [n_sp] AO offset of shell j
+    const double* pp_P;       // [n_pp][4]
+    const double* E;          // [n_pp][nnz]
+    const float* E32;         // [n_pp][nnz] (float path)
+    // auxiliary shells
+    const int* ks_perm;       // [n_ks] or nullptr: position -> shell index
+    const int* ks_prim_off;   // [n_ks + 1]
+    const int* ks_col;        // [n_ks] output column of the shell's first function
+    const double* ka_C;       // [n_kprim][4]
+    const double* ka_coef;    // [n_kprim]
+    int n_ks;
+    // output
+    double* out;
+    long long ld;             // row stride (in doubles)
+    int layout;               // 0: rows = packed lower triangle (i >= j); 1: rows = sp * nij + ij
+    double omega;
+};
+
+// Nuclear attraction: lanes over point charges, output [(sp * nij + ij) * ld + charge].
+struct NucArgs {
+    const int* sp_pp_off;
+    const double* pp_P;
+    const double* E;
+    const double* charge;     // [n_charge][3]
+    int n_sp, n_charge;
+    double* out;
+    long long ld;
+    double omega;
+};
+
+// Two-centre (P|Q): one wave per i shell, lanes over j shells.
+struct J2cArgs {
+    const int* i_prim_off;    // [n_i + 1]
+    const double* i_A;        // [n_iprim][4]
+    const double* i_coef;
+    const int* i_col;         // [n_i] output row of the shell's first function
+    const int* j_prim_off;
+    const double* j_A;
+    const double* j_coef;
+    const int* j_col;
+    int n_i, n_j;
+    int same_class;           // skip blocks with i_col > j_col
+    int mirror;               // also write the transposed block when i_col != j_col
+    double* out;
+    long long ld;
+    double omega;
+};

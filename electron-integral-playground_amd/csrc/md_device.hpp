@@ -1,0 +1,99 @@
+// Device-side building blocks shared by every generated McMurchie-Davidson kernel (gfx950).
+//
+// * Boys function F_0..F_L: Taylor expansion of F_L about the nearest point of a 1/16 grid on [0, 36]
+//   followed by the stable downward recursion; above the grid F_0 = sqrt(pi/T)/2 and the upward
+//   recursion (stable there since 2T > 2m+1 for every order used).  The float64 version is accurate to
+//   a few ulp; the float32 version to a few float ulp.  (Reference: boys_function.hpp:102-176 uses a
+//   five-branch scheme with a Chebyshev table; both agree to ~1e-15 relative.)
+// * XCD-aware block numbering: the dispatcher hands consecutive workgroups to the 8 XCDs round-robin;
+//   remapping puts consecutive logical blocks (which share shell-pair data) on the same XCD and L2.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "generated/boys_table.inc"
+
+namespace eip {
+
+constexpr double kTwoPi25 = 34.986836655249725;     // 2 pi^(5/2)
+constexpr double kTwoPi = 6.283185307179586;        // 2 pi
+constexpr double kPi = 3.141592653589793;
+
+__device__ __forceinline__ int xcd_remap(int b, int nblk)
+{
+    // Blocks are dealt to XCDs round-robin: physical block b runs on XCD b % 8.  Give XCD x the
+    // logical range [x * per, (x+1) * per) so neighbours in logical order share one L2.
+    // XCD x owns per (+1 when x < rem) blocks; its logical range starts at x*per + min(x, rem).
+    constexpr int kXcd = 8;
+    const int per = nblk / kXcd;
+    const int rem = nblk - per * kXcd;
+    const int x = b % kXcd;
+    const int i = b / kXcd;
+    return x * per + (x < rem ? x : rem) + i;
+}
+
+template <int L>
+__device__ __forceinline__ void boys64(const double T, double* __restrict__ F)
+{
+    if (T < BOYS_T_SWITCH) {
+        const int k = __double2int_rn(T * BOYS_H_INV);
+        const double x = k * (1.0 / BOYS_H_INV) - T;     // -(T - T0)
+        const double* __restrict__ row = boys_grid64 + k * BOYS_NCOL + L;
+        double acc = row[BOYS_K64];
+#pragma unroll
+        for (int j = BOYS_K64 - 1; j >= 0; --j) acc = fma(x * (1.0 / (j + 1)), acc, row[j]);
+        F[L] = acc;
+        if constexpr (L > 0) {
+            const double eT = exp(-T);
+            const double twoT = T + T;
+#pragma unroll
+            for (int m = L - 1; m >= 0; --m) F[m] = fma(twoT, F[m + 1], eT) * (1.0 / (2 * m + 1));
+        }
+    } else {
+        const double invT = 1.0 / T;
+        F[0] = 0.5 * sqrt(kPi * invT);
+        if constexpr (L > 0) {
+            const double eT = exp(-T);
+            const double half_invT = 0.5 * invT;
+#pragma unroll
+            for (int m = 0; m < L; ++m) F[m + 1] = fma((double) (2 * m + 1), F[m], -eT) * half_invT;
+        }
+    }
+}
+
+template <int L>
+__device__ __forceinline__ void boys32(const float T, float* __restrict__ F)
+{
+    if (T < (float) BOYS_T_SWITCH) {
+        const int k = __float2int_rn(T * (float) BOYS_H_INV);
+        const float x = k * (1.0f / BOYS_H_INV) - T;
+        const float* __restrict__ row = boys_grid32 + k * BOYS_NCOL + L;
+        float acc = row[BOYS_K32];
+#pragma unroll
+        for (int j = BOYS_K32 - 1; j >= 0; --j) acc = fmaf(x * (1.0f / (j + 1)), acc, row[j]);
+        F[L] = acc;
+        if constexpr (L > 0) {
+            const float eT = __expf(-T);
+            const float twoT = T + T;
+#pragma unroll
+            for (int m = L - 1; m >= 0; --m) F[m] = fmaf(twoT, F[m + 1], eT) * (1.0f / (2 * m + 1));
+        }
+    } else {
+        const float invT = 1.0f / T;
+        F[0] = 0.5f * sqrtf((float) kPi * invT);
+        if constexpr (L > 0) {
+            const float eT = __expf(-T);
+            const float half_invT = 0.5f * invT;
+#pragma unroll
+            for (int m = 0; m < L; ++m) F[m + 1] = fmaf((float) (2 * m + 1), F[m], -eT) * half_invT;
+        }
+    }
+}
+
+// Row of the i >= j packed lower triangle holding element (i, j) of a symmetric matrix.
+__device__ __forceinline__ long long packed_row(int i, int j)
+{
+    return i >= j ? (long long) i * (i + 1) / 2 + j : (long long) j * (j + 1) / 2 + i;
+}
+
+}  // namespace eip

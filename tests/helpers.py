@@ -1,0 +1,121 @@
+"""Shared test helpers: oracle loading, small random molecules, oracle-side entry points.
+
+The oracle (oracle/build/libmd_oracle.so) is the checker; the functions here call it with exactly the
+arrays and loop structure the reference's Python entry points use
+(reference molecular_integral/{overlap,nuclear_attraction,two_center,three_center}.py).
+"""
+
+import ctypes
+import pathlib
+import sys
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+import __graft_entry__  # noqa: E402
+
+__graft_entry__.load_package()
+
+from electron_integral_playground_amd.basis_utility import (  # noqa: E402
+    attach_auxiliary_basis_to_molecule, attach_basis_to_molecule)
+from electron_integral_playground_amd.data_structure import GaussianShell, Molecule  # noqa: E402
+
+ORACLE_SO = ROOT / "oracle" / "build" / "libmd_oracle.so"
+REF_SO = ROOT / "oracle" / "_ref" / "libmolecular_integral_ref.so"
+
+D = ctypes.POINTER(ctypes.c_double)
+I = ctypes.POINTER(ctypes.c_int)
+
+
+def dp(a):
+    return a.ctypes.data_as(D)
+
+
+def ip(a):
+    return a.ctypes.data_as(I)
+
+
+_oracle = None
+
+
+def oracle_lib():
+    global _oracle
+    if _oracle is None:
+        _oracle = ctypes.CDLL(str(ORACLE_SO))
+        _oracle.oracle_cart_to_sph_row.restype = ctypes.c_double
+        _oracle.oracle_boys.argtypes = [ctypes.c_int, ctypes.c_double, D]
+    return _oracle
+
+
+def random_basis(rng, ls, nprim=(1, 3), exp_range=(0.1, 5.0)):
+    shells = []
+    for l in ls:
+        n = int(rng.integers(nprim[0], nprim[1] + 1))
+        exps = np.exp(rng.uniform(np.log(exp_range[0]), np.log(exp_range[1]), n))
+        coefs = rng.normal(size=n)
+        shells.append(GaussianShell(l, -1, -1, exps, coefs))
+    return shells
+
+
+def random_molecule(n_atom=3, seed=0, orbital_l=(0, 1), aux_l=(0, 1), box=3.0, spherical=True,
+                    nprim=(1, 3), aux_nprim=(1, 2)):
+    """Atoms in a box (Bohr) sharing one random element 'X' basis; contracted shells."""
+    rng = np.random.default_rng(seed)
+    geom = rng.uniform(-box / 2, box / 2, size=(n_atom, 3))
+    mol = Molecule(elements=["X"] * n_atom, geometry=geom, basis_shells=None, n_ao=0,
+                   auxiliary_basis_shells=None, n_aux=0, spherical_basis=spherical)
+    attach_basis_to_molecule(mol, {"X": random_basis(rng, orbital_l, nprim)})
+    attach_auxiliary_basis_to_molecule(mol, {"X": random_basis(rng, aux_l, aux_nprim, (0.2, 8.0))})
+    return mol
+
+
+def run_three_center_oracle(lib, pair_data, aux_data, mol, omega=0.0, fn="oracle_three_center"):
+    J = np.zeros((mol.n_aux, mol.n_ao, mol.n_ao))
+    for (i_L, j_L), pd in pair_data.items():
+        for k_L, pk in aux_data.items():
+            rc = getattr(lib, fn)(i_L, j_L, k_L, dp(pd.P_p), dp(pd.A_a), dp(pd.B_b), dp(pd.coefficient),
+                                  ip(pd.i_ao_start), ip(pd.j_ao_start), pd.P_p.shape[0], dp(pk.A_a),
+                                  dp(pk.coefficient), ip(pk.i_ao_start), pk.A_a.shape[0], dp(J), mol.n_ao,
+                                  mol.n_aux, ctypes.c_bool(mol.spherical_basis), ctypes.c_double(omega))
+            assert rc == 0
+    return J
+
+
+def run_two_center_oracle(lib, aux_data, mol, omega=0.0, fn="oracle_two_center"):
+    J = np.zeros((mol.n_aux, mol.n_aux))
+    for i_L, pi in aux_data.items():
+        for j_L, pj in aux_data.items():
+            if i_L > j_L:
+                continue
+            rc = getattr(lib, fn)(i_L, j_L, dp(pi.A_a), dp(pi.coefficient), ip(pi.i_ao_start), pi.A_a.shape[0],
+                                  dp(pj.A_a), dp(pj.coefficient), ip(pj.i_ao_start), pj.A_a.shape[0], dp(J),
+                                  mol.n_aux, ctypes.c_bool(mol.spherical_basis), ctypes.c_double(omega))
+            assert rc == 0
+    return J
+
+
+def run_overlap_oracle(lib, pair_data, mol, fn="oracle_overlap"):
+    S = np.zeros((mol.n_ao, mol.n_ao))
+    for (i_L, j_L), pd in pair_data.items():
+        rc = getattr(lib, fn)(i_L, j_L, dp(pd.A_a), dp(pd.B_b), dp(pd.coefficient), ip(pd.i_ao_start),
+                              ip(pd.j_ao_start), pd.A_a.shape[0], dp(S), mol.n_ao, ctypes.c_bool(mol.spherical_basis))
+        assert rc == 0
+    return S
+
+
+def run_nuclear_oracle(lib, pair_data, mol, charges, omega=0.0, fn="oracle_nuclear_attraction"):
+    charges = np.ascontiguousarray(charges, dtype=np.float64)
+    V = np.zeros((mol.n_ao, mol.n_ao, charges.shape[0]))
+    for (i_L, j_L), pd in pair_data.items():
+        rc = getattr(lib, fn)(i_L, j_L, dp(pd.P_p), dp(pd.A_a), dp(pd.B_b), dp(pd.coefficient), ip(pd.i_ao_start),
+                              ip(pd.j_ao_start), pd.A_a.shape[0], dp(charges), charges.shape[0], dp(V), mol.n_ao,
+                              ctypes.c_bool(mol.spherical_basis), ctypes.c_double(omega))
+        assert rc == 0
+    return V
+
+
+# Parity bar of the floating-point integrals: |HIP - oracle| <= 1e-12 + 1e-10 |oracle|.
+ATOL = 1e-12
+RTOL = 1e-10
