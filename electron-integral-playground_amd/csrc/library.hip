@@ -139,8 +139,10 @@ struct DevPairs {
     int nnz = 0;           // entries per primitive pair
 };
 
+// allow_spec: the consumer has a specialised kernel for this class (else the dense generic layout).
 int prepare_pairs(DevPairs& d, int la, int lb, bool sph, const double* P_p, const double* A_a, const double* B_b,
-                  const double* coef, const int* i_start, const int* j_start, int n, bool recompute_P, hipStream_t s)
+                  const double* coef, const int* i_start, const int* j_start, int n, bool recompute_P, bool allow_spec,
+                  hipStream_t s)
 {
     d.g = group_by(i_start, j_start, n);
     d.n_pp = n;
@@ -153,7 +155,7 @@ int prepare_pairs(DevPairs& d, int la, int lb, bool sph, const double* P_p, cons
     EIP_CHECK(upload(d.B, hB, s));
     EIP_CHECK(upload(d.coef, hc, s));
     EIP_CHECK(upload(d.off, d.g.off, s));
-    d.spec = sph && la <= EIP_SPEC_MAX_L && lb <= EIP_SPEC_MAX_L;
+    d.spec = allow_spec && sph && la <= EIP_SPEC_MAX_L && lb <= EIP_SPEC_MAX_L;
     if (d.spec) {
         d.nnz = eip_pair_nnz(la, lb);
         EIP_CHECK(d.E.alloc(sizeof(double) * (size_t) n * d.nnz));
@@ -189,7 +191,7 @@ int overlap(int i_L, int j_L, const double* pair_A_a, const double* pair_B_b, co
     hipStream_t s = thread_stream();
     DevPairs d;
     if (int rc = prepare_pairs(d, i_L, j_L, spherical, nullptr, pair_A_a, pair_B_b, pair_coefficient,
-                               pair_i_ao_start, pair_j_ao_start, n_pair, true, s)) return rc;
+                               pair_i_ao_start, pair_j_ao_start, n_pair, true, true, s)) return rc;
     const int ni = nout(i_L, spherical), nj = nout(j_L, spherical), nrow = ni * nj;
     DevBuf out;
     EIP_CHECK(out.alloc(sizeof(double) * (size_t) d.n_sp * nrow));
@@ -222,7 +224,7 @@ int nuclear_attraction(int i_L, int j_L, const double* pair_P_p, const double* p
     hipStream_t s = thread_stream();
     DevPairs d;
     if (int rc = prepare_pairs(d, i_L, j_L, spherical, pair_P_p, pair_A_a, pair_B_b, pair_coefficient,
-                               pair_i_ao_start, pair_j_ao_start, n_pair, false, s)) return rc;
+                               pair_i_ao_start, pair_j_ao_start, n_pair, false, true, s)) return rc;
     const int ni = nout(i_L, spherical), nj = nout(j_L, spherical), nrow = ni * nj;
     DevBuf charge, out;
     EIP_CHECK(charge.alloc(sizeof(double) * 3 * (size_t) n_charge));
@@ -323,7 +325,8 @@ int three_center(int i_L, int j_L, int k_L, const double* pair_P_p, const double
     hipStream_t s = thread_stream();
     DevPairs d;
     if (int rc = prepare_pairs(d, i_L, j_L, spherical, pair_P_p, pair_A_a, pair_B_b, pair_coefficient,
-                               pair_i_ao_start, pair_j_ao_start, n_pair_ij, false, s)) return rc;
+                               pair_i_ao_start, pair_j_ao_start, n_pair_ij, false,
+                               k_L <= EIP_SPEC_MAX_AUX_L, s)) return rc;
     Grouping gk = group_by(aux_i_aux_start, nullptr, n_aux_primitive_k);
     const int n_ks = (int) gk.key_a.size();
     const int ni = nout(i_L, spherical), nj = nout(j_L, spherical), nk = nout(k_L, spherical), nrow = ni * nj;
@@ -338,7 +341,7 @@ int three_center(int i_L, int j_L, int k_L, const double* pair_P_p, const double
     // Bound the compact buffer: process shell pairs in batches of at most ~256 MB of output.
     const long long per_sp = (long long) nrow * ld;
     const int batch = (int) std::max(1LL, std::min<long long>(d.n_sp, (256LL << 20) / 8 / std::max(1LL, per_sp)));
-    const bool spec = d.spec && k_L <= EIP_SPEC_MAX_AUX_L;
+    const bool spec = d.spec;
     DevBuf out, items, spi, spj;
     EIP_CHECK(out.alloc(sizeof(double) * (size_t) batch * per_sp));
     std::vector<double> h((size_t) batch * per_sp);
