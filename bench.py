@@ -1,0 +1,177 @@
+"""Benchmark: three-centre (ij|P) integrals per second on one MI355X, plain FP64 path.
+
+Workload (one "step"): every (ij|P) of the 80-atom plain workload -- 80 atoms in a 14 A box, orbital
+basis 3s2p1d with 1-6 primitives per shell, auxiliary s-f with 8 even-tempered exponents per l
+(n_ao = 1120, n_aux = 10240) -- written into the device-resident packed output
+J3c_packed[i >= j, P] (51.4 GB).  Inputs (primitive pairs, pair matrices, auxiliary shells) are resident
+in HBM before timing starts.  value = n_aux * n_ao (n_ao + 1) / 2 / (seconds per step).
+
+Also reported: the roofline of the dominant kernel (HIP-event time of its launches on the launch
+stream, algorithmic flops from the fixed per-triplet model) and a CPU baseline -- the reference's
+OpenMP C++ (oracle/_ref) on a bounded sample of the same workload, with a parity check of that sample
+against the GPU output.
+"""
+
+import argparse
+import ctypes
+import json
+import os
+import pathlib
+import sys
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+import __graft_entry__  # noqa: E402
+
+PEAK_FP64 = 78.6e12          # MI355X FP64 vector (and matrix) peak, FLOP/s
+PEAK_HBM = 8.0e12            # bytes/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(mol, pairs, aux, gpu_packed, max_seconds=20.0):
+    """Time the reference C++ (or the oracle's restatement if the reference build is absent) on the
+    first auxiliary shells of every class, all orbital pairs; return throughput and parity."""
+    from tests.helpers import REF_SO, ORACLE_SO, dp, ip
+    kind, path, fn = ("reference", REF_SO, "three_center") if REF_SO.exists() else ("port", ORACLE_SO, "oracle_three_center")
+    cores = os.cpu_count() or 1
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    lib = ctypes.CDLL(str(path))
+    n_ao = mol.n_ao
+    # sample: aux shells whose functions start in [0, n_blk) of each class (rebased block output)
+    n_shell_per_class = 2
+    total_t, total_int = 0.0, 0
+    max_err = 0.0
+    for k_L, pk in aux.items():
+        starts = np.unique(pk.i_ao_start)[:n_shell_per_class]
+        sel = np.isin(pk.i_ao_start, starts)
+        A = np.ascontiguousarray(pk.A_a[sel]); c = np.ascontiguousarray(pk.coefficient[sel])
+        base = {int(s): k * (2 * k_L + 1) for k, s in enumerate(starts)}
+        st = np.array([base[int(s)] for s in pk.i_ao_start[sel]], dtype=np.int32)
+        nblk = len(starts) * (2 * k_L + 1)
+        J = np.zeros((nblk, n_ao, n_ao))
+        t0 = time.perf_counter()
+        for (i_L, j_L), pd in pairs.items():
+            rc = getattr(lib, fn)(i_L, j_L, k_L, dp(pd.P_p), dp(pd.A_a), dp(pd.B_b), dp(pd.coefficient),
+                                  ip(pd.i_ao_start), ip(pd.j_ao_start), pd.P_p.shape[0], dp(A), dp(c), ip(st),
+                                  A.shape[0], dp(J), n_ao, nblk, ctypes.c_bool(True), ctypes.c_double(0.0))
+            assert rc == 0
+        total_t += time.perf_counter() - t0
+        total_int += nblk * n_ao * (n_ao + 1) // 2
+        # parity of the sample against the GPU packed output
+        tri_i, tri_j = np.tril_indices(n_ao)
+        for k, s in enumerate(starts):
+            for m in range(2 * k_L + 1):
+                gpu_col = gpu_packed[:, int(s) + m]
+                ref_col = J[k * (2 * k_L + 1) + m][tri_i, tri_j]
+                max_err = max(max_err, float(np.max(np.abs(gpu_col - ref_col))))
+        if total_t > max_seconds:
+            break
+    sample = f"{n_shell_per_class} auxiliary shells per angular class x all {sum(len(p.P_p) for p in pairs.values())} primitive pairs"
+    return {"value": total_int / total_t, "unit": "3c integrals/s", "cores": int(os.environ["OMP_NUM_THREADS"]),
+            "kind": kind, "sample": sample, "seconds": round(total_t, 3)}, max_err
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the integral path here is a single-GPU build "
+              "(multi-GPU sharding by auxiliary blocks is a design note, not implemented).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    __graft_entry__.load_package()
+    from electron_integral_playground_amd import generators, pair_utility
+    from electron_integral_playground_amd.engine import ThreeCenterPlan, three_center_flops_per_triplet
+
+    torch.cuda.set_device(0)
+    t0 = time.perf_counter()
+    mol = generators.make_molecule("plain")
+    pairs = pair_utility.form_pair_data(mol, 1e-14)
+    aux = pair_utility.form_auxiliary_pair_data(mol)
+    t_host = time.perf_counter() - t0
+    log(f"workload plain: n_ao={mol.n_ao} n_aux={mol.n_aux} pairs={sum(len(p.P_p) for p in pairs.values())} "
+        f"host pair list {t_host:.1f}s")
+    plan = ThreeCenterPlan(mol, pairs, aux)
+    out = plan.new_output()
+    stream = torch.cuda.current_stream()
+
+    for _ in range(args.warmup):
+        plan.run(out)
+    torch.cuda.synchronize()
+    log("warmup done")
+
+    # per-class kernel times with HIP events on the launch stream (one extra pass)
+    ev = []
+    for pi, pc in enumerate(plan.pairs):
+        for ai, ac in enumerate(plan.aux):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            plan.launch_class(pi, ai, out)
+            e1.record(stream)
+            ev.append(((pc.la, pc.lb, ac.lc), pc.n_pp * ac.n_prim, e0, e1))
+    torch.cuda.synchronize()
+    per_class = []
+    for key, ntrip, e0, e1 in ev:
+        ms = e0.elapsed_time(e1)
+        fl = ntrip * three_center_flops_per_triplet(*key)
+        per_class.append((key, ms, fl))
+    per_class.sort(key=lambda r: -r[1])
+    for key, ms, fl in per_class[:8]:
+        log(f"  class {key}: {ms:.3f} ms  {fl / ms / 1e9:.2f} TFLOP/s (model)")
+
+    # timed region
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.run(out)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    n_int = plan.n_integrals()
+    value = n_int / dt
+    log(f"step {dt * 1e3:.2f} ms -> {value:.4e} integrals/s; model flops/step {plan.model_flops():.3e}")
+
+    key, ms, fl = per_class[0]
+    nchunks = plan.lib.eip_three_center_nchunks(*key)
+    roofline = {"bound": "mfma", "pipe": "fp64-valu", "kernel": "j3c_%d%d%d (all %d component chunks)" % (key + (nchunks,)),
+                "achieved": fl / (ms * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                "frac": fl / (ms * 1e-3) / PEAK_FP64, "traffic": None}
+
+    cpu, err = (None, None)
+    if not args.no_cpu:
+        gpu_packed = out.cpu().numpy() if out.numel() * 8 < 8e9 else None
+        if gpu_packed is None:
+            # copy only the columns the sample needs
+            cols = sorted({int(s) + m for k_L, pk in aux.items() for s in np.unique(pk.i_ao_start)[:2]
+                           for m in range(2 * k_L + 1)})
+            sub = out[:, cols].cpu().numpy()
+            gpu_packed = np.zeros((out.shape[0], max(cols) + 1))
+            gpu_packed[:, cols] = sub
+        cpu, err = cpu_baseline(mol, pairs, aux, gpu_packed)
+        log(f"cpu baseline {cpu}; sample max |GPU - reference| = {err:.3e}")
+
+    line = {"metric": "3c integrals/s (plain FP64, device-resident packed output)", "value": value,
+            "unit": "3c integrals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64", "data": "synthetic (seeded generators, plain workload)",
+            "config": {"workload": "plain: 80 atoms, 14 A box, 3s2p1d (1-6 prim), aux s-f 8 exp/l ratio 2.2",
+                       "n_ao": mol.n_ao, "n_aux": mol.n_aux,
+                       "primitive_pairs": int(sum(len(p.P_p) for p in pairs.values())),
+                       "output": "packed (n_ao(n_ao+1)/2, n_aux) float64 on device"},
+            "roofline": roofline, "cpu_baseline": cpu,
+            "max_abs_err_vs_reference_sample": err}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()
