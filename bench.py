@@ -28,6 +28,7 @@ import __graft_entry__  # noqa: E402
 
 PEAK_FP64 = 78.6e12          # MI355X FP64 vector (and matrix) peak, FLOP/s
 PEAK_HBM = 8.0e12            # bytes/s
+SAMPLE_SHELLS = 24           # auxiliary shells per angular class in the CPU-baseline sample
 
 
 def log(*a):
@@ -44,7 +45,7 @@ def cpu_baseline(mol, pairs, aux, gpu_packed, max_seconds=20.0):
     lib = ctypes.CDLL(str(path))
     n_ao = mol.n_ao
     # sample: aux shells whose functions start in [0, n_blk) of each class (rebased block output)
-    n_shell_per_class = 2
+    n_shell_per_class = SAMPLE_SHELLS
     total_t, total_int = 0.0, 0
     max_err = 0.0
     for k_L, pk in aux.items():
@@ -65,7 +66,7 @@ def cpu_baseline(mol, pairs, aux, gpu_packed, max_seconds=20.0):
         total_int += nblk * n_ao * (n_ao + 1) // 2
         # parity of the sample against the GPU packed output
         tri_i, tri_j = np.tril_indices(n_ao)
-        for k, s in enumerate(starts):
+        for k, s in enumerate(starts if gpu_packed is not None else []):
             for m in range(2 * k_L + 1):
                 gpu_col = gpu_packed[:, int(s) + m]
                 ref_col = J[k * (2 * k_L + 1) + m][tri_i, tri_j]
@@ -152,7 +153,7 @@ def main():
         gpu_packed = out.cpu().numpy() if out.numel() * 8 < 8e9 else None
         if gpu_packed is None:
             # copy only the columns the sample needs
-            cols = sorted({int(s) + m for k_L, pk in aux.items() for s in np.unique(pk.i_ao_start)[:2]
+            cols = sorted({int(s) + m for k_L, pk in aux.items() for s in np.unique(pk.i_ao_start)[:SAMPLE_SHELLS]
                            for m in range(2 * k_L + 1)})
             sub = out[:, cols].cpu().numpy()
             gpu_packed = np.zeros((out.shape[0], max(cols) + 1))

@@ -125,3 +125,25 @@ def read_basis(name: str, directory=None) -> BasisSet:
     if not path.is_file():
         raise FileNotFoundError(f"basis {name} not found in {directory}")
     return read_basis_file(path)
+
+
+def read_xyz_content_permissive(xyz_lines: Union[str, List[str]],
+                                unit: LengthUnits = LengthUnits.ANGSTROM) -> Molecule:
+    """Every non-empty line is ``element x y z``; no count or comment line
+    (reference geometry_reader/permissive_xyz_reader.py:9-41)."""
+    elements, coords = [], []
+    for line in _lines(xyz_lines):
+        fields = line.split()
+        if not fields:
+            continue
+        assert len(fields) == 4, f"bad xyz line {line!r}"
+        elements.append(fields[0])
+        coords.append([float(fields[1]), float(fields[2]), float(fields[3])])
+    assert elements
+    geometry = np.array(coords)
+    if unit == LengthUnits.ANGSTROM:
+        geometry *= angstrom_to_bohr
+    elif unit != LengthUnits.BOHR:
+        raise NotImplementedError(f"Unsupported unit {unit}")
+    return Molecule(elements=elements, geometry=geometry, basis_shells=None, n_ao=0,
+                    auxiliary_basis_shells=None, n_aux=0, spherical_basis=True)

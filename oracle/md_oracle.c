@@ -32,11 +32,19 @@
  * The three-centre loop is parallelised over auxiliary shells (one thread owns all rows of an
  * auxiliary shell), so no atomics are needed and results do not depend on the thread count.
  */
-#include <math.h>
 #include <stdbool.h>
+#include <tgmath.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+
+/* Internal arithmetic type: double for the checker, long double for the high-precision build
+ * (libmd_oracle_ld.so) used to decide ill-conditioned cases.  Inputs and outputs stay double. */
+#ifndef REAL_T
+#define REAL_T double
+#endif
+typedef REAL_T real_t;
+#define OR_PI ((real_t) 3.14159265358979323846264338327950288L)
 
 #define OR_MAX_L 6          /* orbital shells (reference angular.h:4) */
 #define OR_MAX_AUX_L 9      /* auxiliary shells (reference angular.h:5) */
@@ -47,8 +55,8 @@
 /* Small combinatorics                                                                          */
 /* ------------------------------------------------------------------------------------------ */
 
-static double fact(int n) { double r = 1.0; for (int i = 2; i <= n; i++) r *= i; return r; }
-static double binom(int n, int k) { return (k < 0 || k > n) ? 0.0 : fact(n) / (fact(k) * fact(n - k)); }
+static real_t fact(int n) { real_t r = 1.0; for (int i = 2; i <= n; i++) r *= i; return r; }
+static real_t binom(int n, int k) { return (k < 0 || k > n) ? 0.0 : fact(n) / (fact(k) * fact(n - k)); }
 static int ncart(int l) { return (l + 1) * (l + 2) / 2; }
 
 /* Cartesian component c of shell l, x-major descending order (reference indexing.hpp:98-112). */
@@ -64,7 +72,7 @@ static void cart_xyz(int l, int c, int *x, int *y, int *z)
 }
 
 /* sqrt(ix! iy! iz! (2L)! / ((2ix)! (2iy)! (2iz)! L!)) -- kernel_cartesian_normalization.hpp:22-26 */
-static double cart_norm(int l, int c)
+static real_t cart_norm(int l, int c)
 {
     int x, y, z;
     cart_xyz(l, c, &x, &y, &z);
@@ -77,10 +85,10 @@ static double cart_norm(int l, int c)
 
 /* raw_sph[l][m][c]: coefficient of monomial c in solid harmonic m (order S_l..S_1, C_0..C_l);
  * p is left as x, y, z (reference cartesian_spherical_transformation.hpp:38-39). */
-static double raw_sph[OR_MAX_AUX_L + 1][2 * OR_MAX_AUX_L + 1][OR_MAX_CART];
+static real_t raw_sph[OR_MAX_AUX_L + 1][2 * OR_MAX_AUX_L + 1][OR_MAX_CART];
 static int tables_ready = 0;
 
-typedef struct { double c[OR_MAX_AUX_L + 1][OR_MAX_AUX_L + 1]; } poly_t;  /* c[a][b], z power = l-a-b */
+typedef struct { real_t c[OR_MAX_AUX_L + 1][OR_MAX_AUX_L + 1]; } poly_t;  /* c[a][b], z power = l-a-b */
 
 static void build_solid_harmonics(void)
 {
@@ -97,16 +105,16 @@ static void build_solid_harmonics(void)
             poly_t P;
             memset(&P, 0, sizeof(P));
             for (int k = 0; 2 * k <= l - m; k++) {
-                const double gamma = ((k % 2) ? -1.0 : 1.0) * pow(2.0, -l) * binom(l, k) * binom(2 * l - 2 * k, l)
+                const real_t gamma = ((k % 2) ? -1.0 : 1.0) * pow(2.0, -l) * binom(l, k) * binom(2 * l - 2 * k, l)
                                      * fact(l - 2 * k) / fact(l - 2 * k - m);
                 /* r^{2k} = sum_{i+j+h=k} k!/(i!j!h!) x^{2i} y^{2j} z^{2h} */
                 for (int i = 0; i <= k; i++)
                     for (int j = 0; i + j <= k; j++) {
-                        const double multinom = fact(k) / (fact(i) * fact(j) * fact(k - i - j));
+                        const real_t multinom = fact(k) / (fact(i) * fact(j) * fact(k - i - j));
                         /* times z^{l-2k-m} times A_m/B_m = sum_p C(m,p) x^p y^{m-p} trig */
                         for (int p = 0; p <= m; p++) {
                             const int q = m - p;            /* power of y from A/B */
-                            double trig;
+                            real_t trig;
                             if (is_sine) trig = (q % 4 == 1) ? 1.0 : (q % 4 == 3) ? -1.0 : 0.0;   /* sin(pi q / 2) */
                             else         trig = (q % 4 == 0) ? 1.0 : (q % 4 == 2) ? -1.0 : 0.0;   /* cos(pi q / 2) */
                             if (trig == 0.0) continue;
@@ -114,7 +122,7 @@ static void build_solid_harmonics(void)
                         }
                     }
             }
-            const double pref = sqrt((is_sine ? 2.0 : (m == 0 ? 1.0 : 2.0)) * fact(l - m) / fact(l + m));
+            const real_t pref = sqrt((is_sine ? 2.0 : (m == 0 ? 1.0 : 2.0)) * fact(l - m) / fact(l + m));
             for (int c = 0; c < ncart(l); c++) {
                 int x, y, z;
                 cart_xyz(l, c, &x, &y, &z);
@@ -144,10 +152,8 @@ double oracle_cart_to_sph_row(int l, int row, int c)
 /* Boys function F_m(x), m = 0..mmax, in long double                                           */
 /* ------------------------------------------------------------------------------------------ */
 
-void oracle_boys(int mmax, double xd, double *out)
+static void boys_internal(int mmax, long double x, long double *F)
 {
-    const long double x = xd;
-    long double F[OR_MAX_LL + 8];
     if (x < 50.0L) {
         /* F_mmax(x) = e^{-x} sum_k (2x)^k / ((2m+1)(2m+3)...(2m+2k+1)), then downward
          * F_m = (2x F_{m+1} + e^{-x}) / (2m+1).  All terms positive: no cancellation. */
@@ -167,6 +173,12 @@ void oracle_boys(int mmax, double xd, double *out)
         F[0] = 0.5L * sqrtl(3.14159265358979323846264338327950288L / x) * erfl(sqrtl(x));
         for (int m = 0; m < mmax; m++) F[m + 1] = ((2 * m + 1) * F[m] - ex) / (2.0L * x);
     }
+}
+
+void oracle_boys(int mmax, double xd, double *out)
+{
+    long double F[OR_MAX_LL + 8];
+    boys_internal(mmax, xd, F);
     for (int m = 0; m <= mmax; m++) out[m] = (double) F[m];
 }
 
@@ -179,16 +191,16 @@ void oracle_boys(int mmax, double xd, double *out)
 /* E[i][j][t] for i <= li, j <= lj on one axis: VRR E^{i+1,0}_t = 1/(2p) E^{i,0}_{t-1} + PA E^{i,0}_t
  * + (t+1) E^{i,0}_{t+1}, then E^{i,j}_t = sum_m C(j, j-m) AB^{j-m} E^{i+m,0}_t
  * (reference mcmurchie_davidson_E_term.hpp:22-45, 166-199). */
-static void hermite_E_pair(int li, int lj, double PA, double AB, double one_over_two_p,
-                           double E[OR_MAX_L + 1][OR_MAX_L + 1][ESTR])
+static void hermite_E_pair(int li, int lj, real_t PA, real_t AB, real_t one_over_two_p,
+                           real_t E[OR_MAX_L + 1][OR_MAX_L + 1][ESTR])
 {
-    double E0[2 * OR_MAX_L + 2][ESTR];
+    real_t E0[2 * OR_MAX_L + 2][ESTR];
     const int L = li + lj;
     memset(E0, 0, sizeof(E0));
     E0[0][0] = 1.0;
     for (int i = 0; i < L; i++)
         for (int t = 0; t <= i + 1; t++) {
-            double v = 0.0;
+            real_t v = 0.0;
             if (t >= 1) v += one_over_two_p * E0[i][t - 1];
             if (t <= i) v += PA * E0[i][t];
             if (t + 1 <= i) v += (t + 1) * E0[i][t + 1];
@@ -197,9 +209,9 @@ static void hermite_E_pair(int li, int lj, double PA, double AB, double one_over
     for (int i = 0; i <= li; i++)
         for (int j = 0; j <= lj; j++)
             for (int t = 0; t < ESTR; t++) {
-                double v = 0.0;
+                real_t v = 0.0;
                 if (t <= i + j) {
-                    double ABpow = 1.0;
+                    real_t ABpow = 1.0;
                     for (int m = j; m >= 0; m--) {
                         if (t <= i + m) v += binom(j, j - m) * ABpow * E0[i + m][t];
                         ABpow *= AB;
@@ -210,13 +222,13 @@ static void hermite_E_pair(int li, int lj, double PA, double AB, double one_over
 }
 
 /* Single-centre coefficients E^{k,0}_s with b = 0 (reference mcmurchie_davidson_E_term.hpp:109-130). */
-static void hermite_E_single(int lk, double one_over_two_q, double E[OR_MAX_AUX_L + 1][OR_MAX_AUX_L + 1])
+static void hermite_E_single(int lk, real_t one_over_two_q, real_t E[OR_MAX_AUX_L + 1][OR_MAX_AUX_L + 1])
 {
-    memset(E, 0, sizeof(double) * (OR_MAX_AUX_L + 1) * (OR_MAX_AUX_L + 1));
+    memset(E, 0, sizeof(real_t) * (OR_MAX_AUX_L + 1) * (OR_MAX_AUX_L + 1));
     E[0][0] = 1.0;
     for (int i = 0; i < lk; i++)
         for (int s = 0; s <= i + 1; s++) {
-            double v = 0.0;
+            real_t v = 0.0;
             if (s >= 1) v += one_over_two_q * E[i][s - 1];
             if (s + 1 <= i) v += (s + 1) * E[i][s + 1];
             E[i + 1][s] = v;
@@ -225,44 +237,44 @@ static void hermite_E_single(int lk, double one_over_two_q, double E[OR_MAX_AUX_
 
 typedef struct {
     int L;
-    double *R;      /* R[(t*(L+1) + u)*(L+1) + v], level 0 */
-    double *work;   /* scratch level */
+    real_t *R;      /* R[(t*(L+1) + u)*(L+1) + v], level 0 */
+    real_t *work;   /* scratch level */
 } herm_R;
 
 static void herm_R_alloc(herm_R *h, int L)
 {
     const size_t n = (size_t) (L + 1) * (L + 1) * (L + 1);
     h->L = L;
-    h->R = (double *) calloc(n, sizeof(double));
-    h->work = (double *) calloc(n, sizeof(double));
+    h->R = (real_t *) calloc(n, sizeof(real_t));
+    h->work = (real_t *) calloc(n, sizeof(real_t));
 }
 
 static void herm_R_free(herm_R *h) { free(h->R); free(h->work); }
 
 /* R^0_{tuv}(zeta, PQ), t+u+v <= L, from R^n_000 = (-2 zeta)^n F_n(zeta |PQ|^2) by the MD recursion. */
-static void herm_R_compute(herm_R *h, double zeta, double X, double Y, double Z)
+static void herm_R_compute(herm_R *h, real_t zeta, real_t X, real_t Y, real_t Z)
 {
     const int L = h->L;
     const int S = L + 1;
-    double F[OR_MAX_LL + 8];
-    oracle_boys(L, zeta * (X * X + Y * Y + Z * Z), F);
+    long double F[OR_MAX_LL + 8];
+    boys_internal(L, (long double) zeta * ((long double) X * X + (long double) Y * Y + (long double) Z * Z), F);
     /* ping-pong between the two buffers; L swaps follow, so start in h->R when L is even and the
      * final level (n = 0) lands in h->R either way */
-    double *cur = h->R, *prev = h->work;
+    real_t *cur = h->R, *prev = h->work;
     if (L % 2 == 1) { cur = h->work; prev = h->R; }
     /* level n = L: only R^L_000 */
-    double m2z = 1.0;
+    real_t m2z = 1.0;
     for (int n = 0; n < L; n++) m2z *= -2.0 * zeta;
     cur[0] = m2z * F[L];
     for (int n = L - 1; n >= 0; n--) {
-        double *tmp = prev; prev = cur; cur = tmp;          /* prev holds level n+1 (|tuv| <= L-n-1) */
+        real_t *tmp = prev; prev = cur; cur = tmp;          /* prev holds level n+1 (|tuv| <= L-n-1) */
         m2z = 1.0;
         for (int k = 0; k < n; k++) m2z *= -2.0 * zeta;
         const int top = L - n;
         for (int t = 0; t <= top; t++)
             for (int u = 0; t + u <= top; u++)
                 for (int v = 0; t + u + v <= top; v++) {
-                    double val;
+                    real_t val;
                     if (t == 0 && u == 0 && v == 0) val = m2z * F[n];
                     else if (t > 0) {
                         val = X * prev[((t - 1) * S + u) * S + v];
@@ -279,7 +291,7 @@ static void herm_R_compute(herm_R *h, double zeta, double X, double Y, double Z)
     }
 }
 
-static inline double herm_R_get(const herm_R *h, int t, int u, int v)
+static inline real_t herm_R_get(const herm_R *h, int t, int u, int v)
 {
     const int S = h->L + 1;
     return h->R[(t * S + u) * S + v];
@@ -291,11 +303,11 @@ static inline double herm_R_get(const herm_R *h, int t, int u, int v)
 
 /* Spherical row m of shell l applied to a Cartesian vector v with stride (after the Cartesian
  * normalisation has been applied to v, the anti-normalised matrix raw/N gives raw coefficients). */
-static double sph_row_dot(int l, int m, const double *v, int stride)
+static real_t sph_row_dot(int l, int m, const real_t *v, int stride)
 {
-    double s = 0.0;
+    real_t s = 0.0;
     for (int c = 0; c < ncart(l); c++) {
-        const double coef = raw_sph[l][m][c];
+        const real_t coef = raw_sph[l][m][c];
         if (coef != 0.0) s += coef / cart_norm(l, c) * v[c * stride];
     }
     return s;
@@ -303,14 +315,14 @@ static double sph_row_dot(int l, int m, const double *v, int stride)
 
 /* block[(a*ni + i)*nj + j] (Cartesian, normalised) -> out[(a*no_i + i')*no_j + j'] spherical or copy.
  * The leading dimension 'a' (nlead entries) is untouched. */
-static void transform_ij(int li, int lj, int nlead, const double *cart, double *out, bool spherical)
+static void transform_ij(int li, int lj, int nlead, const real_t *cart, real_t *out, bool spherical)
 {
     const int ni = ncart(li), nj = ncart(lj);
-    if (!spherical) { memcpy(out, cart, sizeof(double) * nlead * ni * nj); return; }
+    if (!spherical) { memcpy(out, cart, sizeof(real_t) * nlead * ni * nj); return; }
     const int si = 2 * li + 1, sj = 2 * lj + 1;
-    double *tmp = (double *) malloc(sizeof(double) * ni * sj);
+    real_t *tmp = (real_t *) malloc(sizeof(real_t) * ni * sj);
     for (int a = 0; a < nlead; a++) {
-        const double *blk = cart + (size_t) a * ni * nj;
+        const real_t *blk = cart + (size_t) a * ni * nj;
         for (int i = 0; i < ni; i++)
             for (int j = 0; j < sj; j++) tmp[i * sj + j] = sph_row_dot(lj, j, blk + i * nj, 1);
         for (int i = 0; i < si; i++)
@@ -337,18 +349,18 @@ int oracle_overlap(int i_L, int j_L, const double *pair_A_a, const double *pair_
     }
     const int ni = ncart(i_L), nj = ncart(j_L);
     const int oi = spherical ? 2 * i_L + 1 : ni, oj = spherical ? 2 * j_L + 1 : nj;
-    double cart[OR_MAX_CART * OR_MAX_CART], out[OR_MAX_CART * OR_MAX_CART];
-    double Ex[OR_MAX_L + 1][OR_MAX_L + 1][ESTR], Ey[OR_MAX_L + 1][OR_MAX_L + 1][ESTR], Ez[OR_MAX_L + 1][OR_MAX_L + 1][ESTR];
+    real_t cart[OR_MAX_CART * OR_MAX_CART], out[OR_MAX_CART * OR_MAX_CART];
+    real_t Ex[OR_MAX_L + 1][OR_MAX_L + 1][ESTR], Ey[OR_MAX_L + 1][OR_MAX_L + 1][ESTR], Ez[OR_MAX_L + 1][OR_MAX_L + 1][ESTR];
     for (int ip = 0; ip < n_pair; ip++) {
         const double *A = pair_A_a + 4 * ip, *B = pair_B_b + 4 * ip;
         /* P recomputed from A and B, P_p not read (reference overlap.cpp:22-29) */
-        const double p = A[3] + B[3];
-        const double f = -B[3] / p;
-        const double AB[3] = {A[0] - B[0], A[1] - B[1], A[2] - B[2]};
+        const real_t p = A[3] + B[3];
+        const real_t f = -B[3] / p;
+        const real_t AB[3] = {A[0] - B[0], A[1] - B[1], A[2] - B[2]};
         hermite_E_pair(i_L, j_L, f * AB[0], AB[0], 0.5 / p, Ex);
         hermite_E_pair(i_L, j_L, f * AB[1], AB[1], 0.5 / p, Ey);
         hermite_E_pair(i_L, j_L, f * AB[2], AB[2], 0.5 / p, Ez);
-        const double pref = pair_coefficient[ip] * pow(M_PI / p, 1.5);
+        const real_t pref = pair_coefficient[ip] * pow(OR_PI / p, 1.5);
         for (int ci = 0; ci < ni; ci++) {
             int ix, iy, iz; cart_xyz(i_L, ci, &ix, &iy, &iz);
             for (int cj = 0; cj < nj; cj++) {
@@ -387,8 +399,8 @@ int oracle_nuclear_attraction(int i_L, int j_L, const double *pair_P_p, const do
     const int oi = spherical ? 2 * i_L + 1 : ni, oj = spherical ? 2 * j_L + 1 : nj;
 #pragma omp parallel
     {
-        double cart[OR_MAX_CART * OR_MAX_CART], out[OR_MAX_CART * OR_MAX_CART];
-        double Ex[OR_MAX_L + 1][OR_MAX_L + 1][ESTR], Ey[OR_MAX_L + 1][OR_MAX_L + 1][ESTR], Ez[OR_MAX_L + 1][OR_MAX_L + 1][ESTR];
+        real_t cart[OR_MAX_CART * OR_MAX_CART], out[OR_MAX_CART * OR_MAX_CART];
+        real_t Ex[OR_MAX_L + 1][OR_MAX_L + 1][ESTR], Ey[OR_MAX_L + 1][OR_MAX_L + 1][ESTR], Ez[OR_MAX_L + 1][OR_MAX_L + 1][ESTR];
         herm_R R;
         herm_R_alloc(&R, i_L + j_L);
         /* parallel over charges: each thread owns output column c, no two threads touch one element */
@@ -397,20 +409,20 @@ int oracle_nuclear_attraction(int i_L, int j_L, const double *pair_P_p, const do
             const double *C = charge_position + 3 * ic;
             for (int ip = 0; ip < n_pair; ip++) {
                 const double *P = pair_P_p + 4 * ip, *A = pair_A_a + 4 * ip, *B = pair_B_b + 4 * ip;
-                const double p = P[3];
+                const real_t p = P[3];
                 hermite_E_pair(i_L, j_L, P[0] - A[0], A[0] - B[0], 0.5 / p, Ex);
                 hermite_E_pair(i_L, j_L, P[1] - A[1], A[1] - B[1], 0.5 / p, Ey);
                 hermite_E_pair(i_L, j_L, P[2] - A[2], A[2] - B[2], 0.5 / p, Ez);
                 /* reference nuclear_attraction.cpp:44-45, 69 */
-                const double pref = (omega == 0.0) ? pair_coefficient[ip] * 2.0 * M_PI / p
-                                                   : pair_coefficient[ip] * 2.0 * M_PI * omega / (p * sqrt(p + omega * omega));
-                const double zeta = (omega == 0.0) ? p : p * omega * omega / (p + omega * omega);
+                const real_t pref = (omega == 0.0) ? pair_coefficient[ip] * 2.0 * OR_PI / p
+                                                   : pair_coefficient[ip] * 2.0 * OR_PI * omega / (p * sqrt(p + omega * omega));
+                const real_t zeta = (omega == 0.0) ? p : p * omega * omega / (p + omega * omega);
                 herm_R_compute(&R, zeta, P[0] - C[0], P[1] - C[1], P[2] - C[2]);
                 for (int ci = 0; ci < ni; ci++) {
                     int ix, iy, iz; cart_xyz(i_L, ci, &ix, &iy, &iz);
                     for (int cj = 0; cj < nj; cj++) {
                         int jx, jy, jz; cart_xyz(j_L, cj, &jx, &jy, &jz);
-                        double v = 0.0;
+                        real_t v = 0.0;
                         for (int tx = 0; tx <= ix + jx; tx++)
                             for (int ty = 0; ty <= iy + jy; ty++)
                                 for (int tz = 0; tz <= iz + jz; tz++)
@@ -451,9 +463,9 @@ int oracle_two_center(int i_L, int j_L, const double *aux_A_a, const double *aux
     const int oi = spherical ? 2 * i_L + 1 : ni, oj = spherical ? 2 * j_L + 1 : nj;
 #pragma omp parallel
     {
-        double *cart = (double *) malloc(sizeof(double) * ni * nj);
-        double *out = (double *) malloc(sizeof(double) * ni * nj);
-        double Ei[OR_MAX_AUX_L + 1][OR_MAX_AUX_L + 1], Ej[OR_MAX_AUX_L + 1][OR_MAX_AUX_L + 1];
+        real_t *cart = (real_t *) malloc(sizeof(real_t) * ni * nj);
+        real_t *out = (real_t *) malloc(sizeof(real_t) * ni * nj);
+        real_t Ei[OR_MAX_AUX_L + 1][OR_MAX_AUX_L + 1], Ej[OR_MAX_AUX_L + 1][OR_MAX_AUX_L + 1];
         herm_R R;
         herm_R_alloc(&R, i_L + j_L);
         /* parallel over rows i: a thread owns the rows of its i primitive's shell and their mirrors
@@ -466,12 +478,12 @@ int oracle_two_center(int i_L, int j_L, const double *aux_A_a, const double *aux
                     const int is = aux_i_aux_start[ia], js = aux_j_aux_start[ja];
                     if (i_L == j_L && is > js) continue;      /* reference two_center.cpp:138 */
                     const double *A = aux_A_a + 4 * ia, *B = aux_B_b + 4 * ja;
-                    const double p = A[3], q = B[3];
-                    const double coef = aux_i_coefficient[ia] * aux_j_coefficient[ja];
+                    const real_t p = A[3], q = B[3];
+                    const real_t coef = aux_i_coefficient[ia] * aux_j_coefficient[ja];
                     /* reference two_center.cpp:42-45 */
-                    const double pref = (omega == 0.0) ? coef * 2.0 * pow(M_PI, 2.5) / (p * q * sqrt(p + q))
-                                                       : coef * 2.0 * pow(M_PI, 2.5) * omega / (p * q * sqrt(p * q + (p + q) * omega * omega));
-                    const double zeta = (omega == 0.0) ? p * q / (p + q) : 1.0 / (1.0 / p + 1.0 / q + 1.0 / (omega * omega));
+                    const real_t pref = (omega == 0.0) ? coef * 2.0 * pow(OR_PI, 2.5) / (p * q * sqrt(p + q))
+                                                       : coef * 2.0 * pow(OR_PI, 2.5) * omega / (p * q * sqrt(p * q + (p + q) * omega * omega));
+                    const real_t zeta = (omega == 0.0) ? p * q / (p + q) : 1.0 / (1.0 / p + 1.0 / q + 1.0 / (omega * omega));
                     hermite_E_single(i_L, 0.5 / p, Ei);
                     hermite_E_single(j_L, 0.5 / q, Ej);
                     herm_R_compute(&R, zeta, A[0] - B[0], A[1] - B[1], A[2] - B[2]);
@@ -479,7 +491,7 @@ int oracle_two_center(int i_L, int j_L, const double *aux_A_a, const double *aux
                         int ix, iy, iz; cart_xyz(i_L, ci, &ix, &iy, &iz);
                         for (int cj = 0; cj < nj; cj++) {
                             int jx, jy, jz; cart_xyz(j_L, cj, &jx, &jy, &jz);
-                            double v = 0.0;
+                            real_t v = 0.0;
                             for (int tx = ix % 2; tx <= ix; tx += 2)
                                 for (int ty = iy % 2; ty <= iy; ty += 2)
                                     for (int tz = iz % 2; tz <= iz; tz += 2)
@@ -537,11 +549,11 @@ int oracle_three_center(int i_L, int j_L, int k_L, const double *pair_P_p, const
     }
 #pragma omp parallel
     {
-        double *cart = (double *) malloc(sizeof(double) * nk * ni * nj);
-        double *ijsph = (double *) malloc(sizeof(double) * nk * ni * nj);
-        double *blk = (double *) malloc(sizeof(double) * nk * ni * nj);
-        double Ex[OR_MAX_L + 1][OR_MAX_L + 1][ESTR], Ey[OR_MAX_L + 1][OR_MAX_L + 1][ESTR], Ez[OR_MAX_L + 1][OR_MAX_L + 1][ESTR];
-        double Ek[OR_MAX_AUX_L + 1][OR_MAX_AUX_L + 1];
+        real_t *cart = (real_t *) malloc(sizeof(real_t) * nk * ni * nj);
+        real_t *ijsph = (real_t *) malloc(sizeof(real_t) * nk * ni * nj);
+        real_t *blk = (real_t *) malloc(sizeof(real_t) * nk * ni * nj);
+        real_t Ex[OR_MAX_L + 1][OR_MAX_L + 1][ESTR], Ey[OR_MAX_L + 1][OR_MAX_L + 1][ESTR], Ez[OR_MAX_L + 1][OR_MAX_L + 1][ESTR];
+        real_t Ek[OR_MAX_AUX_L + 1][OR_MAX_AUX_L + 1];
         herm_R R;
         herm_R_alloc(&R, i_L + j_L + k_L);
 #pragma omp for schedule(dynamic)
@@ -549,16 +561,16 @@ int oracle_three_center(int i_L, int j_L, int k_L, const double *pair_P_p, const
             for (int ka = 0; ka < n_aux_primitive_k; ka++) {
                 if (aux_i_aux_start[ka] != starts[s]) continue;
                 const double *C = aux_A_a + 4 * ka;
-                const double q = C[3];
+                const real_t q = C[3];
                 hermite_E_single(k_L, 0.5 / q, Ek);
                 for (int ip = 0; ip < n_pair_ij; ip++) {
                     const double *P = pair_P_p + 4 * ip, *A = pair_A_a + 4 * ip, *B = pair_B_b + 4 * ip;
-                    const double p = P[3];
-                    const double coef = pair_coefficient[ip] * aux_coefficient[ka];
+                    const real_t p = P[3];
+                    const real_t coef = pair_coefficient[ip] * aux_coefficient[ka];
                     /* reference three_center.cpp:46-49 */
-                    const double pref = (omega == 0.0) ? coef * 2.0 * pow(M_PI, 2.5) / (p * q * sqrt(p + q))
-                                                       : coef * 2.0 * pow(M_PI, 2.5) * omega / (p * q * sqrt(p * q + (p + q) * omega * omega));
-                    const double zeta = (omega == 0.0) ? p * q / (p + q) : 1.0 / (1.0 / p + 1.0 / q + 1.0 / (omega * omega));
+                    const real_t pref = (omega == 0.0) ? coef * 2.0 * pow(OR_PI, 2.5) / (p * q * sqrt(p + q))
+                                                       : coef * 2.0 * pow(OR_PI, 2.5) * omega / (p * q * sqrt(p * q + (p + q) * omega * omega));
+                    const real_t zeta = (omega == 0.0) ? p * q / (p + q) : 1.0 / (1.0 / p + 1.0 / q + 1.0 / (omega * omega));
                     hermite_E_pair(i_L, j_L, P[0] - A[0], A[0] - B[0], 0.5 / p, Ex);
                     hermite_E_pair(i_L, j_L, P[1] - A[1], A[1] - B[1], 0.5 / p, Ey);
                     hermite_E_pair(i_L, j_L, P[2] - A[2], A[2] - B[2], 0.5 / p, Ez);
@@ -569,12 +581,12 @@ int oracle_three_center(int i_L, int j_L, int k_L, const double *pair_P_p, const
                             int ix, iy, iz; cart_xyz(i_L, ci, &ix, &iy, &iz);
                             for (int cj = 0; cj < nj; cj++) {
                                 int jx, jy, jz; cart_xyz(j_L, cj, &jx, &jy, &jz);
-                                double v = 0.0;
+                                real_t v = 0.0;
                                 for (int tx = 0; tx <= ix + jx; tx++)
                                     for (int ty = 0; ty <= iy + jy; ty++)
                                         for (int tz = 0; tz <= iz + jz; tz++) {
-                                            const double Et = Ex[ix][jx][tx] * Ey[iy][jy][ty] * Ez[iz][jz][tz];
-                                            double w = 0.0;
+                                            const real_t Et = Ex[ix][jx][tx] * Ey[iy][jy][ty] * Ez[iz][jz][tz];
+                                            real_t w = 0.0;
                                             for (int sx = kx % 2; sx <= kx; sx += 2)
                                                 for (int sy = ky % 2; sy <= ky; sy += 2)
                                                     for (int sz = kz % 2; sz <= kz; sz += 2)
@@ -598,7 +610,7 @@ int oracle_three_center(int i_L, int j_L, int k_L, const double *pair_P_p, const
                     for (int k = 0; k < ok; k++)
                         for (int i = 0; i < oi; i++)
                             for (int j = 0; j < oj; j++) {
-                                const double val = blk[((size_t) k * oi + i) * oj + j];
+                                const real_t val = blk[((size_t) k * oi + i) * oj + j];
                                 double *row = J3c_matrix + (size_t) (ks + k) * n_ao * n_ao;
                                 row[(size_t) (is + i) * n_ao + (js + j)] += val;
                                 if (is != js) row[(size_t) (is + i) + (size_t) (js + j) * n_ao] += val;

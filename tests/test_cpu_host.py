@@ -1,0 +1,160 @@
+"""Host layer (readers, normalisation, AO offsets, pair screening and packing) on the CPU.
+
+Expected numbers are the reference's own unit-test values (reference tests/test_basis_utility.py:14-149,
+tests/test_pair_utility.py:11-53).  The vectorised pair builder is checked bit for bit against the
+scalar restatement in oracle/pair_list.py.
+"""
+
+import copy
+import json
+import pathlib
+
+import numpy as np
+import pytest
+
+from electron_integral_playground_amd import generators, pair_utility
+from electron_integral_playground_amd.basis_utility import (AtomicOrbitalOrder, assign_ao_index_to_shell,
+                                                            attach_basis_to_molecule, normalize_shell)
+from electron_integral_playground_amd.data_structure import GaussianShell
+from electron_integral_playground_amd.readers import read_basis_content, read_xyz_content
+from oracle import pair_list as oracle_pairs
+
+GOLDEN = pathlib.Path(__file__).resolve().parent / "golden"
+BASIS = json.loads((GOLDEN / "basis_blocks.json").read_text())
+
+
+def basis(name):
+    return read_basis_content(BASIS[name])
+
+
+def test_normalization_multiple_primitives():
+    shell = GaussianShell(0, -1, -1, np.array([0.3683820000E+02, 0.5481720000E+01, 0.1113270000E+01]),
+                          np.array([0.6966866381E-01, 0.3813463493E+00, 0.6817026244E+00]))
+    expect = {
+        0: [0.7424571167056886, 0.9736824041834441, 0.5265691816179123],
+        1: [9.53466870700217, 4.823483291037769, 1.175548964021699],
+        2: [69.77322364641736, 13.616114130201483, 1.495460534593661],
+        3: [391.7035485493139, 29.48703626431445, 1.4594685003804202],
+    }
+    for l, ref in expect.items():
+        s = copy.deepcopy(shell)
+        s.angular = l
+        normalize_shell(s)
+        np.testing.assert_array_equal(s.primitive_exponents, shell.primitive_exponents)
+        # The reference's fixture allows 1e-14 absolute; at 391.7 that is below one ulp.  numpy 2.2's
+        # vectorised power rounds a^(L/2 + 3/4) one ulp away from libm's pow, and the reference's own
+        # formula gives our value in this environment, so the bar here is 1e-14 or 2 ulp.
+        np.testing.assert_allclose(s.primitive_coefficients, ref, rtol=4.5e-16, atol=1e-14)
+
+
+def test_normalization_one_primitive():
+    expect = {0: 0.1492233559500336, 1: 0.1052328353933318, 2: 0.0428454900225391, 3: 0.0135124649803557}
+    for l, ref in expect.items():
+        s = GaussianShell(l, -1, -1, np.array([0.1243280000E+00]), np.array([1.0]))
+        normalize_shell(s)
+        np.testing.assert_allclose(s.primitive_coefficients, [ref], rtol=0, atol=1e-14)
+
+
+def _hof(geom):
+    return read_xyz_content("3\n\n" + geom)
+
+
+def test_ao_index_low_angular():
+    mol = _hof("O 0 0 0\nH 1 0 0\nF -0.8 0.2 0\n")
+    attach_basis_to_molecule(mol, basis("3-21g"))
+    assert [s.angular for s in mol.basis_shells] == [0, 0, 1, 0, 1, 0, 0, 0, 0, 1, 0, 1]
+    assert [s.i_atom for s in mol.basis_shells] == [0, 0, 0, 0, 0, 1, 1, 2, 2, 2, 2, 2]
+    assert [s.i_ao_start for s in mol.basis_shells] == [0, 1, 2, 5, 6, 9, 10, 11, 12, 13, 16, 17]
+    assert mol.n_ao == 20
+    assign_ao_index_to_shell(mol, AtomicOrbitalOrder.ANGULAR_LEADING)
+    assert [s.angular for s in mol.basis_shells] == [0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1]
+    assert [s.i_atom for s in mol.basis_shells] == [0, 0, 0, 1, 1, 2, 2, 2, 0, 0, 2, 2]
+    assert [s.i_ao_start for s in mol.basis_shells] == [0, 1, 2, 3, 4, 5, 6, 7, 8, 11, 14, 17]
+
+
+def test_ao_index_high_angular():
+    mol = _hof("O 0 0 0\nH 1 0 0\nH -0.8 0.2 0\n")
+    attach_basis_to_molecule(mol, basis("def2-tzvp"))
+    assert [s.i_ao_start for s in mol.basis_shells] == [0, 1, 2, 3, 4, 5, 8, 11, 14, 19, 24, 31, 32, 33, 34,
+                                                        37, 38, 39, 40]
+    assert mol.n_ao == 43
+    mol.spherical_basis = False
+    assign_ao_index_to_shell(mol, AtomicOrbitalOrder.ANGULAR_LEADING)
+    assert [s.i_ao_start for s in mol.basis_shells] == [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 14, 17, 20, 23,
+                                                        26, 32, 38]
+    assert mol.n_ao == 48
+
+
+@pytest.mark.parametrize("geom,counts", [
+    ("O 0 0 0\nH 1 0 0\nF -0.8 0.2 0\n", {(0, 0): 171, (0, 1): 158, (0, 2): 36, (1, 1): 51, (1, 2): 18, (2, 2): 3}),
+    ("O 0 0 0\nH 3 0 0\nF -0.8 2.2 0\n", {(0, 0): 144, (0, 1): 142, (0, 2): 34, (1, 1): 47, (1, 2): 18, (2, 2): 3}),
+])
+def test_pair_counts(geom, counts):
+    mol = _hof(geom)
+    attach_basis_to_molecule(mol, basis("def2-svp"))
+    pl = pair_utility.form_primitive_pair_list(mol, 1e-14)
+    assert {k: len(v) for k, v in pl.items()} == counts
+    for pairs in pl.values():
+        ub = [p.upper_bound for p in pairs]
+        assert all(a >= b for a, b in zip(ub, ub[1:])) and min(ub) >= 1e-14
+
+
+def _assert_pair_lists_identical(mol):
+    got = pair_utility.form_primitive_pair_list(mol, 1e-14)
+    ref = oracle_pairs.form_primitive_pair_list(mol, 1e-14)
+    assert list(got.keys()) == list(ref.keys())
+    for k in ref:
+        a = [(p.i_shell, p.j_shell, p.i_primitive, p.j_primitive, p.upper_bound) for p in got[k]]
+        b = [(p.i_shell, p.j_shell, p.i_primitive, p.j_primitive, p.upper_bound) for p in ref[k]]
+        assert a == b, f"class {k} differs"          # order, membership and bound bits
+    gd = pair_utility.form_primitive_pair_data(mol, got)
+    rd = oracle_pairs.form_primitive_pair_data(mol, ref)
+    for k in rd:
+        for f in ("P_p", "A_a", "B_b", "coefficient", "i_ao_start", "j_ao_start"):
+            np.testing.assert_array_equal(getattr(gd[k], f), rd[k][f], err_msg=f"{k} {f}")
+
+
+@pytest.mark.parametrize("workload", ["small", "hof"])
+def test_pair_list_bit_exact(workload):
+    if workload == "small":
+        mol = generators.make_molecule("small")
+    else:
+        mol = _hof("O 0 0 0\nH 1 0 0\nF -0.8 0.2 0\n")
+        attach_basis_to_molecule(mol, basis("def2-svp"))
+    _assert_pair_lists_identical(mol)
+
+
+def test_pair_list_bit_exact_plain_subset():
+    """20 atoms of the plain workload's generator (1-6 primitives, exponents up to 5e3)."""
+    mol = generators.make_molecule("plain", n_atom=20, box=8.0)
+    _assert_pair_lists_identical(mol)
+
+
+def test_threshold_argument_ignored():
+    """form_pair_data screens at 1e-14 whatever it is given (reference pair_utility.py:185-188)."""
+    mol = generators.make_molecule("small")
+    a = pair_utility.form_pair_data(mol, 1e-3)
+    b = pair_utility.form_pair_data(mol, 1e-14)
+    assert {k: v.P_p.shape for k, v in a.items()} == {k: v.P_p.shape for k, v in b.items()}
+
+
+def test_auxiliary_pair_data_layout():
+    mol = generators.make_molecule("small")
+    aux = pair_utility.form_auxiliary_pair_data(mol)
+    assert list(aux.keys()) == [0, 1, 2]
+    for l, d in aux.items():
+        assert d.P_p is None and d.B_b is None and d.j_ao_start is None
+        assert d.A_a.dtype == np.float64 and d.A_a.flags.c_contiguous and d.A_a.shape[1] == 4
+        assert d.i_ao_start.dtype == np.int32
+    assert sum(d.A_a.shape[0] * (2 * l + 1) for l, d in aux.items()) == mol.n_aux
+
+
+def test_workload_sizes():
+    mol = generators.make_molecule("plain")
+    assert (mol.n_ao, mol.n_aux) == (1120, 10240)
+    mol = generators.make_molecule("small")
+    assert (mol.n_ao, mol.n_aux) == (60, 648)
+    # minimum separation 1 A
+    g = mol.geometry / generators.angstrom_to_bohr
+    d = np.linalg.norm(g[:, None] - g[None], axis=-1) + np.eye(len(g)) * 10
+    assert d.min() >= 1.0

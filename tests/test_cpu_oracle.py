@@ -1,0 +1,209 @@
+"""Pinning the oracle (oracle/md_oracle.c) before it is trusted as the GPU checker.
+
+* the reference's own PySCF fixtures for every case whose data the reference ships, at the reference's
+  tolerances (reference molecular_integral/tests/*);
+* per-class vectors written by the reference's C++ compiled from its sources (tests/golden/make_fixtures.py);
+* the reference's compiled C++ itself on random molecules, where oracle/_ref is built;
+* what the mathematics fixes: Boys function against mpmath, the solid harmonics against the
+  reference generator's d and f equations, unit self-overlap of normalised shells, the closed form of
+  (ss|s), translation invariance, the far-field limit of the nuclear attraction.
+"""
+
+import ctypes
+import math
+
+import mpmath
+import numpy as np
+import pytest
+
+from tests import reference_cases as rc
+from tests.helpers import (ATOL, REF_SO, RTOL, dp, ip, oracle_lib, random_molecule, run_nuclear_oracle,
+                           run_overlap_oracle, run_three_center_oracle, run_two_center_oracle)
+from electron_integral_playground_amd import pair_utility
+
+
+def _oracle_eval(case, b):
+    lib = oracle_lib()
+    return rc.evaluate(
+        case, b,
+        three_center=lambda p, a, m, w: run_three_center_oracle(lib, p, a, m, w),
+        two_center=lambda a, m, w: run_two_center_oracle(lib, a, m, w),
+        overlap=lambda p, m: run_overlap_oracle(lib, p, m),
+        nuclear_attraction=lambda p, m, c, w: run_nuclear_oracle(lib, p, m, c, w))
+
+
+@pytest.mark.parametrize("case", rc.CASES, ids=[c["name"] for c in rc.CASES])
+def test_oracle_vs_reference_fixtures(case):
+    b = rc.build(case)
+    got = _oracle_eval(case, b)
+    ref = rc.expected(case["name"]).reshape(got.shape)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=case["atol"])
+
+
+def _class_vector_cases():
+    with np.load(rc.GOLDEN / "ref_class_vectors.npz") as z:
+        return sorted(k[:-3] for k in z.files if k.endswith("_in"))
+
+
+def _run_class_vector(tag, z, lib, prefix=""):
+    """Re-run one stored class vector through ``lib`` (oracle or HIP library)."""
+    kind, cls, w = tag.split("_")
+    omega = float(w[1:])
+    x = z[tag + "_in"]
+    if kind == "3c":
+        la, lb, lc = int(cls[0]), int(cls[1]), int(cls[2])
+        P, A, B = (np.ascontiguousarray(x[16 * k:16 * k + 16].reshape(4, 4)) for k in range(3))
+        c = np.ascontiguousarray(x[48:52]); C = np.ascontiguousarray(x[52:56].reshape(1, 4)); ck = x[56:57].copy()
+        ni, nj, nk = 2 * la + 1, 2 * lb + 1, 2 * lc + 1
+        si, sj, ks = np.zeros(4, np.int32), np.full(4, ni, np.int32), np.zeros(1, np.int32)
+        J = np.zeros((nk, ni + nj, ni + nj))
+        assert getattr(lib, prefix + "three_center")(la, lb, lc, dp(P), dp(A), dp(B), dp(c), ip(si), ip(sj), 4, dp(C),
+                                                     dp(ck), ip(ks), 1, dp(J), ni + nj, nk, ctypes.c_bool(True),
+                                                     ctypes.c_double(omega)) == 0
+        return {"J": J[:, :ni, ni:]}
+    if kind == "1e":
+        la, lb = int(cls[0]), int(cls[1])
+        P, A, B = (np.ascontiguousarray(x[16 * k:16 * k + 16].reshape(4, 4)) for k in range(3))
+        c = np.ascontiguousarray(x[48:52]); ch = np.ascontiguousarray(x[52:61].reshape(3, 3))
+        ni, nj = 2 * la + 1, 2 * lb + 1
+        si, sj = np.zeros(4, np.int32), np.full(4, ni, np.int32)
+        S = np.zeros((ni + nj, ni + nj)); V = np.zeros((ni + nj, ni + nj, 3))
+        assert getattr(lib, prefix + "overlap")(la, lb, dp(A), dp(B), dp(c), ip(si), ip(sj), 4, dp(S), ni + nj,
+                                                ctypes.c_bool(True)) == 0
+        assert getattr(lib, prefix + "nuclear_attraction")(la, lb, dp(P), dp(A), dp(B), dp(c), ip(si), ip(sj), 4, dp(ch),
+                                                           3, dp(V), ni + nj, ctypes.c_bool(True),
+                                                           ctypes.c_double(omega)) == 0
+        return {"S": S[:ni, ni:], "V": V[:ni, ni:, :]}
+    li, lj = int(cls[0]), int(cls[1])
+    A = np.ascontiguousarray(x[0:8].reshape(2, 4)); ci = x[8:10].copy()
+    B = np.ascontiguousarray(x[10:18].reshape(2, 4)); cj = x[18:20].copy()
+    ni, nj = 2 * li + 1, 2 * lj + 1
+    si, sj = np.zeros(2, np.int32), np.full(2, ni, np.int32)
+    J = np.zeros((ni + nj, ni + nj))
+    assert getattr(lib, prefix + "two_center")(li, lj, dp(A), dp(ci), ip(si), 2, dp(B), dp(cj), ip(sj), 2, dp(J), ni + nj,
+                                               ctypes.c_bool(True), ctypes.c_double(omega)) == 0
+    return {"J": J[:ni, ni:]}
+
+
+def test_oracle_vs_reference_class_vectors():
+    lib = oracle_lib()
+    worst = 0.0
+    with np.load(rc.GOLDEN / "ref_class_vectors.npz") as z:
+        for tag in _class_vector_cases():
+            got = _run_class_vector(tag, z, lib, prefix="oracle_")
+            for key, val in got.items():
+                ref = z[f"{tag}_{key}"]
+                np.testing.assert_allclose(val, ref, rtol=RTOL, atol=ATOL, err_msg=tag + key)
+                worst = max(worst, float(np.max(np.abs(val - ref))))
+    print("worst |oracle - reference| over all classes:", worst)
+
+
+@pytest.mark.skipif(not REF_SO.exists(), reason="reference C++ not built here (oracle/_ref)")
+@pytest.mark.parametrize("seed,orb,aux,sph,omega", [
+    (31, (0, 1, 2), (0, 1, 2, 3), True, 0.0), (32, (0, 1, 3), (1, 4), False, 0.0), (33, (2, 4), (3, 6), True, 0.2)])
+def test_oracle_vs_compiled_reference(seed, orb, aux, sph, omega):
+    mol = random_molecule(n_atom=3, seed=seed, orbital_l=orb, aux_l=aux, spherical=sph)
+    pairs = pair_utility.form_pair_data(mol, 1e-14)
+    auxd = pair_utility.form_auxiliary_pair_data(mol)
+    ref = ctypes.CDLL(str(REF_SO))
+    o = oracle_lib()
+    np.testing.assert_allclose(run_three_center_oracle(o, pairs, auxd, mol, omega),
+                               run_three_center_oracle(ref, pairs, auxd, mol, omega, fn="three_center"),
+                               rtol=RTOL, atol=ATOL)
+    np.testing.assert_allclose(run_two_center_oracle(o, auxd, mol, omega),
+                               run_two_center_oracle(ref, auxd, mol, omega, fn="two_center"), rtol=RTOL, atol=ATOL)
+    ch = np.vstack([mol.geometry, [[5.0, 1.0, -2.0]]])
+    np.testing.assert_allclose(run_nuclear_oracle(o, pairs, mol, ch, omega),
+                               run_nuclear_oracle(ref, pairs, mol, ch, omega, fn="nuclear_attraction"),
+                               rtol=RTOL, atol=ATOL)
+    np.testing.assert_allclose(run_overlap_oracle(o, pairs, mol), run_overlap_oracle(ref, pairs, mol, fn="overlap"),
+                               rtol=RTOL, atol=ATOL)
+
+
+def test_boys_against_mpmath():
+    lib = oracle_lib()
+    mpmath.mp.dps = 40
+    out = (ctypes.c_double * 27)()
+    for x in [0.0, 1e-12, 0.03, 0.1, 0.7, 2.5, 9.9, 17.95, 18.0, 30.0, 49.0, 51.0, 84.9, 85.0, 300.0]:
+        lib.oracle_boys(26, x, out)
+        for m in range(27):
+            ref = mpmath.mpf(1) / (2 * m + 1) if x == 0 else \
+                mpmath.gammainc(m + 0.5, 0, x) / (2 * mpmath.mpf(x) ** (m + 0.5))
+            assert abs(out[m] - float(ref)) <= 2e-16 * abs(float(ref)), (m, x)
+
+
+def test_solid_harmonics_match_generator_equations():
+    """Coefficients on normalised Cartesians equal the reference generator's d and f equations
+    (reference gen/tests/test_cartesian_spherical_transformation.py:7-35)."""
+    lib = oracle_lib()
+
+    def norm(l, x, y, z):
+        f = math.factorial
+        return math.sqrt(f(x) * f(y) * f(z) * f(2 * l) / (f(2 * x) * f(2 * y) * f(2 * z) * f(l)))
+
+    carts = lambda l: [(x, y, l - x - y) for x in range(l, -1, -1) for y in range(l - x, -1, -1)]
+    d = [{(1, 1, 0): 1}, {(0, 1, 1): 1}, {(0, 0, 2): 1, (2, 0, 0): -0.5, (0, 2, 0): -0.5}, {(1, 0, 1): 1},
+         {(2, 0, 0): math.sqrt(3) / 2, (0, 2, 0): -math.sqrt(3) / 2}]
+    f = [{(2, 1, 0): 1.5 / math.sqrt(2), (0, 3, 0): -0.5 * math.sqrt(2.5)}, {(1, 1, 1): 1},
+         {(2, 1, 0): -0.5 * math.sqrt(0.3), (0, 3, 0): -0.5 * math.sqrt(1.5), (0, 1, 2): 2 * math.sqrt(0.3)},
+         {(2, 0, 1): -1.5 / math.sqrt(5), (0, 2, 1): -1.5 / math.sqrt(5), (0, 0, 3): 1},
+         {(3, 0, 0): -0.5 * math.sqrt(1.5), (1, 2, 0): -0.5 * math.sqrt(0.3), (1, 0, 2): 2 * math.sqrt(0.3)},
+         {(2, 0, 1): math.sqrt(3) / 2, (0, 2, 1): -math.sqrt(3) / 2},
+         {(3, 0, 0): 0.5 * math.sqrt(2.5), (1, 2, 0): -1.5 / math.sqrt(2)}]
+    for l, rows in ((2, d), (3, f)):
+        for m, row in enumerate(rows):
+            for c, xyz in enumerate(carts(l)):
+                got = lib.oracle_cart_to_sph_row(l, m, c) / norm(l, *xyz)
+                assert abs(got - row.get(xyz, 0.0)) < 1e-15, (l, m, xyz)
+
+
+def test_normalised_self_overlap_is_one():
+    """Every diagonal element of S is 1 for normalised shells, spherical and Cartesian (reference
+    basis_utility.py:85-140 plus kernel_cartesian_normalization.hpp:22-26)."""
+    for sph in (True, False):
+        mol = random_molecule(n_atom=2, seed=40, orbital_l=(0, 1, 2, 3, 4, 5, 6), aux_l=(0,), spherical=sph)
+        pairs = pair_utility.form_pair_data(mol, 1e-14)
+        S = run_overlap_oracle(oracle_lib(), pairs, mol)
+        np.testing.assert_allclose(np.diag(S), 1.0, rtol=0, atol=1e-13)
+
+
+def test_ss_s_closed_form():
+    """(ss|s) = c_ab c_k 2 pi^(5/2) / (p q sqrt(p+q)) F_0(pq/(p+q) |PQ|^2) for one primitive triplet."""
+    a, b, q = 1.3, 0.7, 0.9
+    A, B, C = np.array([0.1, -0.2, 0.3]), np.array([-0.4, 0.5, 0.2]), np.array([1.0, 0.3, -0.6])
+    p = a + b
+    P = (a * A + b * B) / p
+    cab = 0.8 * math.exp(-a * b / p * float((A - B) @ (A - B)))
+    T = p * q / (p + q) * float((P - C) @ (P - C))
+    F0 = 0.5 * math.sqrt(math.pi / T) * math.erf(math.sqrt(T))
+    expect = cab * 1.7 * 2 * math.pi ** 2.5 / (p * q * math.sqrt(p + q)) * F0
+    PP = np.array([[*P, p]]); AA = np.array([[*A, a]]); BB = np.array([[*B, b]]); CC = np.array([[*C, q]])
+    J = np.zeros((1, 2, 2))
+    lib = oracle_lib()
+    assert lib.oracle_three_center(0, 0, 0, dp(PP), dp(AA), dp(BB), dp(np.array([cab])), ip(np.array([0], np.int32)),
+                                   ip(np.array([1], np.int32)), 1, dp(CC), dp(np.array([1.7])),
+                                   ip(np.array([0], np.int32)), 1, dp(J), 2, 1, ctypes.c_bool(True),
+                                   ctypes.c_double(0.0)) == 0
+    assert abs(J[0, 0, 1] - expect) < 1e-14 * abs(expect) and J[0, 1, 0] == J[0, 0, 1]
+
+
+def test_translation_invariance():
+    mol = random_molecule(n_atom=3, seed=41, orbital_l=(0, 1, 2, 3), aux_l=(0, 2, 4))
+    pairs = pair_utility.form_pair_data(mol, 1e-14)
+    aux = pair_utility.form_auxiliary_pair_data(mol)
+    J0 = run_three_center_oracle(oracle_lib(), pairs, aux, mol)
+    mol.geometry = mol.geometry + np.array([3.1, -1.7, 0.4])
+    pairs = pair_utility.form_pair_data(mol, 1e-14)
+    aux = pair_utility.form_auxiliary_pair_data(mol)
+    J1 = run_three_center_oracle(oracle_lib(), pairs, aux, mol)
+    np.testing.assert_allclose(J1, J0, rtol=1e-10, atol=1e-11)
+
+
+def test_nuclear_far_field():
+    """V(C) -> S / |C - P| for a distant point: V S^-1 |C| tends to 1 on the diagonal."""
+    mol = random_molecule(n_atom=2, seed=42, orbital_l=(0, 1, 2), aux_l=(0,), box=1.0)
+    pairs = pair_utility.form_pair_data(mol, 1e-14)
+    R = np.array([[4000.0, 0.0, 0.0]])
+    V = run_nuclear_oracle(oracle_lib(), pairs, mol, R)[:, :, 0]
+    S = run_overlap_oracle(oracle_lib(), pairs, mol)
+    np.testing.assert_allclose(V * 4000.0, S, rtol=0, atol=2e-3)
