@@ -1,0 +1,94 @@
+/*
+ * molecular_integral.h -- C ABI of libmolecular_integral.so, the MI355X (gfx950) McMurchie-Davidson
+ * integral library.
+ *
+ * Part 1 replaces the reference's four exported functions one for one: same names, argument lists,
+ * accumulate (+=) semantics, mirror rule and return codes (0 ok, 1 unsupported angular class with a
+ * message on stdout; this library adds 2 = HIP runtime error, 3 = kernel launch error).  All pointers
+ * are host pointers to C-contiguous arrays; the library copies inputs to the GPU, computes there and
+ * adds the result into the caller's array.  Thread safe: each calling thread gets its own stream and
+ * buffers.
+ *
+ * Part 2 is additive: device-resident entry points over grouped data already in HBM (no allocation,
+ * no synchronisation; every pointer is a device pointer, `stream` is a hipStream_t).
+ */
+#ifndef ELECTRON_INTEGRAL_PLAYGROUND_AMD_MOLECULAR_INTEGRAL_H
+#define ELECTRON_INTEGRAL_PLAYGROUND_AMD_MOLECULAR_INTEGRAL_H
+
+#include <stdbool.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- Part 1: the reference's C ABI -------------------------------------------------------- */
+
+/* S[i][j] += overlap; P is recomputed from A and B (pair_P_p is not an argument).
+ * Replaces reference c_library/molecular_integral/overlap.cpp:149-159 (`overlap`). */
+int overlap(int i_L, int j_L, const double* pair_A_a, const double* pair_B_b, const double* pair_coefficient,
+            const int* pair_i_ao_start, const int* pair_j_ao_start, int n_pair,
+            double* S_matrix, int n_ao, bool spherical);
+
+/* V[i][j][c] += int i j / |r - C_c| (no charge, positive sign); erf(omega r)/r when omega > 0.
+ * Replaces reference c_library/molecular_integral/nuclear_attraction.cpp:199-213 (`nuclear_attraction`). */
+int nuclear_attraction(int i_L, int j_L, const double* pair_P_p, const double* pair_A_a, const double* pair_B_b,
+                       const double* pair_coefficient, const int* pair_i_ao_start, const int* pair_j_ao_start,
+                       int n_pair, const double* charge_position, int n_charge, double* V_tensor, int n_ao,
+                       bool spherical, double omega);
+
+/* J2c[P][Q] += (P|Q); same-class blocks with i_aux_start > j_aux_start are skipped.
+ * Replaces reference c_library/molecular_integral/two_center.cpp:239-252 (`two_center`). */
+int two_center(int i_L, int j_L, const double* aux_A_a, const double* aux_i_coefficient, const int* aux_i_aux_start,
+               int n_aux_primitive_i, const double* aux_B_b, const double* aux_j_coefficient,
+               const int* aux_j_aux_start, int n_aux_primitive_j, double* J2c_matrix, int n_aux,
+               bool spherical, double omega);
+
+/* J3c[P][i][j] += (P|ij).
+ * Replaces reference c_library/molecular_integral/three_center.cpp:321-339 (`three_center`). */
+int three_center(int i_L, int j_L, int k_L, const double* pair_P_p, const double* pair_A_a, const double* pair_B_b,
+                 const double* pair_coefficient, const int* pair_i_ao_start, const int* pair_j_ao_start,
+                 int n_pair_ij, const double* aux_A_a, const double* aux_coefficient, const int* aux_i_aux_start,
+                 int n_aux_primitive_k, double* J3c_matrix, int n_ao, int n_aux, bool spherical, double omega);
+
+/* ---- Part 2: device-resident entry points (no counterpart in the reference) --------------- */
+
+/* Entries per primitive pair of the packed pair matrix of class (i_L, j_L), or -1 when the class has
+ * no specialised kernels (spherical output, l <= 3). */
+int eip_pair_matrix_nnz(int i_L, int j_L);
+
+/* Number of kernels (auxiliary-component chunks) of class (i_L j_L | k_L), or -1 (aux l <= 4). */
+int eip_three_center_nchunks(int i_L, int j_L, int k_L);
+
+/* Pair matrix (Hermite expansion contracted into real solid harmonics, pair coefficient folded in)
+ * for n_pp primitive pairs: d_E[n_pp][nnz] float64 and, when d_E32 is not null, a float32 copy. */
+int eip_pair_matrix(int i_L, int j_L, const double* d_P_p, const double* d_A_a, const double* d_B_b,
+                    const double* d_coef, int n_pp, double* d_E, float* d_E32, void* stream);
+
+/* (ij|P) of one class over a work-item list.  Item w = (shell pair sp, first auxiliary shell k0)
+ * covers auxiliary shells k0 .. k0+63 (clipped at d_items_ks_end[w], or n_ks when that is null) of
+ * the list d_ks_perm (or 0..n_ks-1).  layout 0 writes d_out[tri(i,j) * ld + col + m] (packed i >= j);
+ * layout 1 writes d_out[(sp * nij + ij) * ld + col + m].  use_f32 selects the float32 kernels. */
+int eip_three_center_items(int i_L, int j_L, int k_L, int use_f32, const int* d_items, const int* d_items_ks_end,
+                           int n_items, const int* d_sp_pp_off, const int* d_sp_i, const int* d_sp_j,
+                           const double* d_pp_P, const double* d_E, const float* d_E32, const int* d_ks_perm,
+                           const int* d_ks_prim_off, const int* d_ks_col, const double* d_ka_C,
+                           const double* d_ka_coef, int n_ks, double* d_out, long long ld, int layout,
+                           double omega, void* stream);
+
+/* (P|Q) blocks of classes (i_L | j_L), aux l <= 4: d_out[(col_i + m_i) * ld + col_j + m_j]; with
+ * same_class, blocks with col_i > col_j are skipped; with mirror, the transposed block is written too. */
+int eip_two_center_device(int i_L, int j_L, const int* d_i_prim_off, const double* d_i_A, const double* d_i_coef,
+                          const int* d_i_col, int n_i, const int* d_j_prim_off, const double* d_j_A,
+                          const double* d_j_coef, const int* d_j_col, int n_j, int same_class, int mirror,
+                          double* d_out, long long ld, double omega, void* stream);
+
+/* Nuclear attraction of shell pairs at n_charge points: d_out[(sp * nij + ij) * ld + c]. */
+int eip_nuclear_device(int i_L, int j_L, const int* d_sp_pp_off, const double* d_pp_P, const double* d_E,
+                       int n_sp, const double* d_charge, int n_charge, double* d_out, long long ld, double omega,
+                       void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
