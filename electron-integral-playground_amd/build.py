@@ -4,6 +4,7 @@ missing.
 """
 
 import concurrent.futures as cf
+import json
 import os
 import pathlib
 import shutil
@@ -33,14 +34,31 @@ def _sources():
 
 
 def _compile(src):
+    """Compile one source; returns (object, {kernel: scratch bytes per lane})."""
     obj = OBJ / (src.stem + ".o")
-    cmd = [HIPCC] + FLAGS + ["-c", str(src), "-o", str(obj)]
+    cmd = [HIPCC] + FLAGS + ["-Rpass-analysis=kernel-resource-usage", "-c", str(src), "-o", str(obj)]
     if src.suffix == ".cpp":
         cmd = [HIPCC, "-x", "hip"] + FLAGS + ["-c", str(src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src.name}:\n{r.stderr[-4000:]}")
-    return obj
+    scratch, name = {}, None
+    for line in r.stderr.splitlines():
+        if "Function Name:" in line:
+            name = line.split("Function Name:")[1].split()[0]
+        elif "ScratchSize [bytes/lane]:" in line and name:
+            scratch[name] = int(line.split("ScratchSize [bytes/lane]:")[1].split()[0])
+    return obj, scratch
+
+
+def _spilling_classes(scratch):
+    """Three-centre classes (la, lb, lc) with a kernel that uses scratch memory."""
+    out = set()
+    for name, nbytes in scratch.items():
+        if nbytes > 0 and name.startswith("j3c_"):
+            cls = name.split("_")[1]
+            out.add((int(cls[0]), int(cls[1]), int(cls[2])))
+    return out
 
 
 def build(jobs=None, verbose=False):
@@ -49,12 +67,35 @@ def build(jobs=None, verbose=False):
         from codegen.generate import generate
     finally:
         sys.path.pop(0)
-    generate(str(GEN))
+    # Kernels must not spill to scratch memory: classes whose kernels still do are regenerated with
+    # smaller auxiliary-component chunks (up to 3 times); the final table is kept for the next build.
+    shrink_file = PKG / "codegen" / "chunk_shrink.json"
+    shrink = {}
+    if shrink_file.exists():
+        shrink = {tuple(json.loads(k)): v for k, v in json.loads(shrink_file.read_text()).items()}
     OBJ.mkdir(parents=True, exist_ok=True)
     jobs = jobs or max(1, min(16, os.cpu_count() or 1))
-    srcs = _sources()
-    with cf.ThreadPoolExecutor(jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+    results = {}
+    for attempt in range(4):
+        before = {p: p.stat().st_mtime_ns for p in GEN.glob("*")}
+        generate(str(GEN), shrink)
+        srcs = [p for p in _sources() if attempt == 0 or before.get(p) != p.stat().st_mtime_ns]
+        with cf.ThreadPoolExecutor(jobs) as ex:
+            for src, res in zip(srcs, ex.map(_compile, srcs)):
+                results[src] = res
+        objs = [o for o, _ in results.values()]
+        scratch = {k: v for _, sc in results.values() for k, v in sc.items()}
+        spilling = {c for c in _spilling_classes(scratch) if shrink.get(c, 0) < 3}
+        if not spilling or attempt == 3:
+            break
+        for c in spilling:
+            shrink[c] = shrink.get(c, 0) + 1
+        if verbose:
+            print("regenerating with smaller chunks:", sorted(spilling))
+    shrink_file.write_text(json.dumps({json.dumps(list(k)): v for k, v in shrink.items()}))
+    still = sorted(k for k, v in scratch.items() if v > 0)
+    if verbose and still:
+        print("kernels still using scratch:", still)
     tmp = LIBDIR / "libmolecular_integral.so.tmp"
     cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", str(tmp)] + [str(o) for o in objs] + ["-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)

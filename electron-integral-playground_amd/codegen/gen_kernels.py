@@ -293,18 +293,19 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64"):
     out.append("    if (!active) return;")
     out.append("    const int col = a.ks_col[ksh];")
     out.append("    const int I = a.sp_i[sp], J = a.sp_j[sp];")
-    out.append("    double* __restrict__ o = a.out;")
+    out.append("    double* __restrict__ o = a.out + (a.sp_out_base ? a.sp_out_base[sp] : 0LL);")
+    out.append("    const long long ld = a.sp_ld ? (long long) a.sp_ld[sp] : a.ld;")
     out.append("    if (a.layout == 0) {")
     for ij in range(nij):
         mi, mj = divmod(ij, nj)
-        out.append(f"      if (I != J || {mi} >= {mj}) {{ const long long row = eip::packed_row(I + {mi}, J + {mj}) * a.ld + col;")
+        out.append(f"      if (I != J || {mi} >= {mj}) {{ const long long row = eip::packed_row(I + {mi}, J + {mj}) * ld + col;")
         for m in ms:
             out.append(f"        o[row + {m}] = (double) acc_{ij}_{m};")
         out.append("      }")
     out.append("    } else {")
     out.append(f"      const long long base = (long long) sp * {nij};")
     for ij in range(nij):
-        out.append(f"      {{ const long long row = (base + {ij}) * a.ld + col;")
+        out.append(f"      {{ const long long row = (base + {ij}) * ld + col;")
         for m in ms:
             out.append(f"        o[row + {m}] = (double) acc_{ij}_{m};")
         out.append("      }")
@@ -460,11 +461,39 @@ def emit_j2c(li, lj):
 # Work split of a class over kernels (register budget)
 # ---------------------------------------------------------------------------------------------
 
-def m_chunks(la, lb, lc, budget=96):
-    """Split the 2lc+1 auxiliary components so acc registers stay near 'budget' values."""
+def _needed_R(la, lb, lc, m0, m1):
+    """Number of Hermite integrals R^0_{t+s} a chunk [m0, m1) of class (la lb|lc) keeps live."""
+    layout = pair_layout(la, lb, True)
+    ts = {t for t, _ in layout}
+    rows = aux_transform(lc, True)
+    need = set()
+    for m in range(m0, m1):
+        for s in rows[m][1]:
+            for t in ts:
+                need.add((t[0] + s[0], t[1] + s[1], t[2] + s[2]))
+    return len(need)
+
+
+# Register budget per lane, in doubles: accumulators + live Hermite integrals + working set must stay
+# well inside the 512 registers (256 VGPR + 256 AGPR) of a wave at occupancy 1, so no kernel spills to
+# scratch memory.  'shrink' lets the build lower the budget of classes whose kernels still spilled.
+REG_BUDGET_DOUBLES = 200
+
+
+def m_chunks(la, lb, lc, shrink=0):
+    """Split the 2lc+1 auxiliary components of class (la lb|lc) over kernels to bound registers."""
     nij = (2 * la + 1) * (2 * lb + 1)
     nk = 2 * lc + 1
-    per = max(1, min(nk, budget // nij))
+    budget = REG_BUDGET_DOUBLES >> shrink
+    per = nk
+    while per > 1:
+        n = -(-nk // per)
+        p = -(-nk // n)
+        worst = max(nij * (min(nk, m + p) - m) + _needed_R(la, lb, lc, m, min(nk, m + p))
+                    for m in range(0, nk, p))
+        if worst <= budget:
+            break
+        per -= 1
     n = -(-nk // per)
     per = -(-nk // n)          # balance
     return [(m, min(nk, m + per)) for m in range(0, nk, per)]

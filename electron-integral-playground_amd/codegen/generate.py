@@ -32,7 +32,9 @@ def _write(path, text):
             f.write(text)
 
 
-def generate(out_dir):
+def generate(out_dir, shrink=None):
+    """shrink: {(la, lb, lc): k} lowers the register budget of a class by 2^k (see m_chunks)."""
+    shrink = shrink or {}
     os.makedirs(out_dir, exist_ok=True)
     boys_path = os.path.join(out_dir, "boys_table.inc")
     if not os.path.exists(boys_path):
@@ -72,7 +74,7 @@ def generate(out_dir):
             src = [HEADER]
             cases = []
             for lc in range(SPEC_MAX_AUX_L + 1):
-                for ci, (m0, m1) in enumerate(m_chunks(la, lb, lc)):
+                for ci, (m0, m1) in enumerate(m_chunks(la, lb, lc, shrink.get((la, lb, lc), 0))):
                     name, code = emit_j3c(la, lb, lc, m0, m1, prec)
                     src.append(code)
                     cases.append(f"        case {lc * 100 + ci}: hipLaunchKernelGGL({name}, dim3(nblocks), dim3(64), 0, s, a); return 0;")

@@ -460,6 +460,24 @@ int eip_three_center_items(int i_L, int j_L, int k_L, int use_f32, const int* d_
     return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
+int eip_three_center_items_batched(int i_L, int j_L, int k_L, int use_f32, const int* d_items,
+                                   const int* d_items_ks_end, int n_items, const int* d_sp_pp_off, const int* d_sp_i,
+                                   const int* d_sp_j, const long long* d_sp_out_base, const int* d_sp_ld,
+                                   const double* d_pp_P, const double* d_E, const float* d_E32, const int* d_ks_perm,
+                                   const int* d_ks_prim_off, const int* d_ks_col, const double* d_ka_C,
+                                   const double* d_ka_coef, int n_ks, double* d_out, double omega, void* stream)
+{
+    const int nchunks = eip_three_center_nchunks(i_L, j_L, k_L);
+    if (nchunks < 0) return 1;
+    if (n_items <= 0) return 0;
+    J3cArgs a{d_items, d_items_ks_end, 0, n_items, d_sp_pp_off, d_sp_i, d_sp_j, d_pp_P, d_E, d_E32,
+              d_ks_perm, d_ks_prim_off, d_ks_col, d_ka_C, d_ka_coef, n_ks, d_out, 0, 0, omega,
+              d_sp_out_base, d_sp_ld};
+    for (int c = 0; c < nchunks; ++c)
+        if (eip_launch_j3c(i_L, j_L, k_L, c, use_f32, a, n_items, (hipStream_t) stream) != 0) return 3;
+    return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
 int eip_two_center_device(int i_L, int j_L, const int* d_i_prim_off, const double* d_i_A, const double* d_i_coef,
                           const int* d_i_col, int n_i, const int* d_j_prim_off, const double* d_j_A,
                           const double* d_j_coef, const int* d_j_col, int n_j, int same_class, int mirror,

@@ -43,6 +43,10 @@ struct J3cArgs {
     long long ld;             // row stride (in doubles)
     int layout;               // 0: rows = packed lower triangle (i >= j); 1: rows = sp * nij + ij
     double omega;
+    // optional, per shell pair (batches of molecules with one output each): offset of the
+    // molecule's output in 'out' and its row stride.  nullptr: 0 and 'ld'.
+    const long long* sp_out_base;
+    const int* sp_ld;
 };
 
 // Nuclear attraction: lanes over point charges, output [(sp * nij + ij) * ld + charge].

@@ -1,6 +1,6 @@
 // Device-side building blocks shared by every generated McMurchie-Davidson kernel (gfx950).
 //
-// * Boys function F_0..F_L: Taylor expansion of F_L about the nearest point of a 1/16 grid on [0, 36]
+// * Boys function F_0..F_L (float32 variant: the large-T upward recursion stays float64): Taylor expansion of F_L about the nearest point of a 1/16 grid on [0, 36]
 //   followed by the stable downward recursion; above the grid F_0 = sqrt(pi/T)/2 and the upward
 //   recursion (stable there since 2T > 2m+1 for every order used).  The float64 version is accurate to
 //   a few ulp; the float32 version to a few float ulp.  (Reference: boys_function.hpp:102-176 uses a
@@ -79,13 +79,16 @@ __device__ __forceinline__ void boys32(const float T, float* __restrict__ F)
             for (int m = L - 1; m >= 0; --m) F[m] = fmaf(twoT, F[m + 1], eT) * (1.0f / (2 * m + 1));
         }
     } else {
-        const float invT = 1.0f / T;
-        F[0] = 0.5f * sqrtf((float) kPi * invT);
+        // large T: the upward recursion runs in float64 and is rounded once per order
+        const double Td = T;
+        const double invT = 1.0 / Td;
+        double f = 0.5 * sqrt(kPi * invT);
+        F[0] = (float) f;
         if constexpr (L > 0) {
-            const float eT = __expf(-T);
-            const float half_invT = 0.5f * invT;
+            const double eT = exp(-Td);
+            const double half_invT = 0.5 * invT;
 #pragma unroll
-            for (int m = 0; m < L; ++m) F[m + 1] = fmaf((float) (2 * m + 1), F[m], -eT) * half_invT;
+            for (int m = 0; m < L; ++m) { f = fma((double) (2 * m + 1), f, -eT) * half_invT; F[m + 1] = (float) f; }
         }
     }
 }

@@ -78,6 +78,15 @@ class AuxClass:
     dev: Dict[str, object] = field(default_factory=dict)
 
 
+def _ranges(counts, first):
+    """For each owner r, counts[r] consecutive items k0 = first[r] + 64 b: returns (owner, k0)."""
+    counts = np.asarray(counts, dtype=np.int64)
+    owner = np.repeat(np.arange(len(counts), dtype=np.int64), counts)
+    start = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    b = np.arange(owner.size, dtype=np.int64) - np.repeat(start, counts)
+    return owner.astype(np.int32), (np.repeat(np.asarray(first, dtype=np.int64), counts) + 64 * b).astype(np.int32)
+
+
 def group_pairs(pd):
     """Stable grouping of one class's primitive pairs by (i_ao_start, j_ao_start)."""
     order = np.lexsort((pd.j_ao_start, pd.i_ao_start))
@@ -113,6 +122,8 @@ class ThreeCenterPlan:
         lib = self.lib
         T = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=self.device)
         self.pairs: List[PairClass] = []
+        self._pair_orders = []
+        self._pair_p = []
         stream = _stream_handle(None)
         for (la, lb), pd in pair_data.items():
             nnz = lib.eip_pair_matrix_nnz(la, lb)
@@ -120,6 +131,8 @@ class ThreeCenterPlan:
                 raise ValueError(f"no specialised kernels for pair class ({la},{lb})")
             order, si, sj, off = group_pairs(pd)
             pc = PairClass(la, lb, len(order), len(si), si, sj, off)
+            self._pair_orders.append(order)
+            self._pair_p.append(pd.P_p[order, 3].copy())
             d = pc.dev
             d["P"] = T(pd.P_p[order], torch.float64)
             d["A"] = T(pd.A_a[order], torch.float64)
@@ -197,6 +210,103 @@ class ThreeCenterPlan:
                 self.launch_class(pi, ai, out, stream)
         return out
 
+    # -- (P|Q) ------------------------------------------------------------------------------------
+    def two_center(self, out=None, stream=None):
+        """Dense (n_aux, n_aux) float64 (P|Q) on the device (always FP64: the metric is factorised)."""
+        torch = self.torch
+        if out is None:
+            out = torch.zeros((self.n_aux, self.n_aux), dtype=torch.float64, device=self.device)
+        for a in self.aux:
+            for b in self.aux:
+                if a.lc > b.lc:
+                    continue
+                rc = self.lib.eip_two_center_device(
+                    a.lc, b.lc, _ptr(a.dev["off"]), _ptr(a.dev["C"]), _ptr(a.dev["c"]), _ptr(a.dev["col"]), a.n_ks,
+                    _ptr(b.dev["off"]), _ptr(b.dev["C"]), _ptr(b.dev["c"]), _ptr(b.dev["col"]), b.n_ks,
+                    int(a.lc == b.lc), 1, _ptr(out), _c_ll(self.n_aux), _c_double(self.omega), _stream_handle(stream))
+                if rc != 0:
+                    raise RuntimeError(f"eip_two_center_device({a.lc},{b.lc}) failed with {rc}")
+        return out
+
+    # -- mixed precision ---------------------------------------------------------------------------
+    def setup_mixed(self, pair_list, q_aux, threshold=1e-6, min_pair_exponent=0.2):
+        """Split every (shell pair, auxiliary shell) triplet between the float64 and float32 kernels.
+
+        Bound: |(ij|P)| <= Q_ij Q_P, with Q_P = sqrt(max_m (P_m|P_m)) from the FP64 metric (``q_aux``:
+        one array per auxiliary class, in shell order) and Q_ij the sum over the shell pair's primitive
+        pairs of the reference's Schwarz-type bound (pair_utility.py:63-71).  Triplets with
+        Q_ij Q_P < threshold run in float32 (Boys, R and accumulation); shell pairs with a primitive
+        exponent p < min_pair_exponent (diffuse pairs) always stay float64.  Auxiliary shells are sorted
+        by Q_P, so each shell pair's float64 set is a prefix of that order.
+        """
+        torch = self.torch
+        T = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=self.device)
+        self.mixed = {}
+        self.mixed_stats = {"f64_triplets": 0, "f32_triplets": 0}
+        for pi, pc in enumerate(self.pairs):
+            key = (pc.la, pc.lb)
+            ub = np.array([p.upper_bound for p in pair_list[key]])
+            # the pair list is in the pair data's order; regroup like the pair data
+            pd_order = self._pair_orders[pi]
+            ub = ub[pd_order]
+            q_sp = np.add.reduceat(ub, pc.sp_pp_off[:-1])
+            pmin = np.minimum.reduceat(self._pair_p[pi], pc.sp_pp_off[:-1])
+            q_sp = np.where(pmin < min_pair_exponent, np.inf, q_sp)
+            for ai, ac in enumerate(self.aux):
+                q = np.asarray(q_aux[ai])
+                perm = np.argsort(-q, kind="stable").astype(np.int32)
+                qs = q[perm]
+                # number of aux shells with q_sp * q >= threshold (q sorted descending)
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    need = np.where(q_sp > 0, threshold / q_sp, np.inf)
+                split = np.searchsorted(-qs, -need, side="right").astype(np.int64)
+                split = np.where(np.isinf(q_sp), ac.n_ks, split)
+                n64 = (split + 63) // 64
+                n32 = (ac.n_ks - split + 63) // 64
+                sp64, k064 = _ranges(n64, np.zeros_like(split))
+                sp32, k032 = _ranges(n32, split)
+                n_prim_sp = np.diff(pc.sp_pp_off).astype(np.int64)
+                kprim = np.diff(ac.ks_prim_off)[perm]
+                cum = np.concatenate([[0], np.cumsum(kprim)])
+                f64 = int(np.sum(n_prim_sp * cum[np.minimum(split, ac.n_ks)]))
+                self.mixed_stats["f64_triplets"] += f64
+                self.mixed_stats["f32_triplets"] += int(np.sum(n_prim_sp) * cum[-1]) - f64
+
+                def items(sp, k0, end):
+                    if len(sp) == 0:
+                        return None, None
+                    return (T(np.stack([sp, k0], axis=1), torch.int32), T(end, torch.int32))
+
+                it64, end64 = items(sp64, k064, split[sp64])
+                it32, end32 = items(sp32, k032, np.full(len(sp32), ac.n_ks))
+                self.mixed[(pi, ai)] = dict(perm=T(perm, torch.int32), it64=it64, end64=end64,
+                                            it32=it32, end32=end32)
+        return self.mixed_stats
+
+    def _launch_items(self, pi, ai, out, items, ends, perm, use_f32, stream=None):
+        if items is None:
+            return
+        pc, ac = self.pairs[pi], self.aux[ai]
+        dp, da = pc.dev, ac.dev
+        rc = self.lib.eip_three_center_items(
+            pc.la, pc.lb, ac.lc, int(use_f32), _ptr(items), _ptr(ends), items.shape[0], _ptr(dp["off"]),
+            _ptr(dp["si"]), _ptr(dp["sj"]), _ptr(dp["P"]), _ptr(dp["E"]), _ptr(dp["E32"]), _ptr(perm),
+            _ptr(da["off"]), _ptr(da["col"]), _ptr(da["C"]), _ptr(da["c"]), ac.n_ks, _ptr(out),
+            _c_ll(self.n_aux), 0, _c_double(self.omega), _stream_handle(stream))
+        if rc != 0:
+            raise RuntimeError(f"eip_three_center_items({pc.la},{pc.lb},{ac.lc}, f32={use_f32}) failed with {rc}")
+
+    def run_mixed(self, out=None, stream=None, zero=True):
+        """Mixed-precision (ij|P) into the packed float64 output (requires ``setup_mixed``)."""
+        if out is None:
+            out = self.new_output()
+        if zero:
+            out.zero_()
+        for (pi, ai), m in self.mixed.items():
+            self._launch_items(pi, ai, out, m["it64"], m["end64"], m["perm"], False, stream)
+            self._launch_items(pi, ai, out, m["it32"], m["end32"], m["perm"], True, stream)
+        return out
+
 
 def packed_to_dense(packed, n_ao):
     """(n_ao(n_ao+1)/2, n_aux) packed -> (n_aux, n_ao, n_ao) dense (reference layout), on the device."""
@@ -207,3 +317,122 @@ def packed_to_dense(packed, n_ao):
     dense[:, idx_i, idx_j] = packed.t()
     dense[:, idx_j, idx_i] = packed.t()
     return dense
+
+
+class BatchPlan:
+    """(ij|P) of many molecules with one launch per class (and auxiliary-component chunk) per batch.
+
+    Shell pairs and auxiliary shells of all molecules are concatenated per angular class; every work
+    item pairs a shell pair with up to 64 auxiliary shells of the *same* molecule, and each shell pair
+    carries the offset and row stride of its molecule's packed output inside one device buffer.  The
+    launch sequence is fixed for a batch, so it can be captured once into a HIP graph and replayed.
+    """
+
+    def __init__(self, molecules, omega=0.0, device="cuda"):
+        """molecules: list of (molecule, pair_data, aux_data)."""
+        import torch
+        self.torch = torch
+        self.lib = library()
+        self.omega = float(omega)
+        self.device = torch.device(device)
+        T = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=self.device)
+        self.sizes = [(m.n_ao, m.n_aux) for m, _, _ in molecules]
+        npk = [n_ao * (n_ao + 1) // 2 * n_aux for n_ao, n_aux in self.sizes]
+        self.out_base = np.concatenate([[0], np.cumsum(npk)]).astype(np.int64)
+        self.n_out = int(self.out_base[-1])
+        pair_keys = sorted({k for _, pd, _ in molecules for k in pd})
+        aux_keys = sorted({k for _, _, ad in molecules for k in ad})
+        stream = _stream_handle(None)
+        self.pairs, self.aux = [], []
+        mol_sp = {}   # (pair key, molecule) -> (sp_lo, sp_hi)
+        for key in pair_keys:
+            la, lb = key
+            nnz = self.lib.eip_pair_matrix_nnz(la, lb)
+            if nnz < 0:
+                raise ValueError(f"no specialised kernels for pair class {key}")
+            P, A, B, c, si, sj, off, base, ld = [], [], [], [], [], [], [0], [], []
+            n_sp = 0
+            for m, (mol, pd, _) in enumerate(molecules):
+                if key not in pd:
+                    continue
+                order, msi, msj, moff = group_pairs(pd[key])
+                P.append(pd[key].P_p[order]); A.append(pd[key].A_a[order]); B.append(pd[key].B_b[order])
+                c.append(pd[key].coefficient[order]); si.append(msi); sj.append(msj)
+                off.extend((moff[1:] + off[-1]).tolist())
+                base.append(np.full(len(msi), self.out_base[m], np.int64)); ld.append(np.full(len(msi), mol.n_aux, np.int32))
+                mol_sp[(key, m)] = (n_sp, n_sp + len(msi))
+                n_sp += len(msi)
+            n_pp = off[-1]
+            d = {"P": T(np.concatenate(P), torch.float64), "A": T(np.concatenate(A), torch.float64),
+                 "B": T(np.concatenate(B), torch.float64), "c": T(np.concatenate(c), torch.float64),
+                 "off": T(np.array(off, np.int32), torch.int32), "si": T(np.concatenate(si), torch.int32),
+                 "sj": T(np.concatenate(sj), torch.int32), "base": T(np.concatenate(base), torch.int64),
+                 "ld": T(np.concatenate(ld), torch.int32)}
+            d["E"] = torch.empty((n_pp, nnz), dtype=torch.float64, device=self.device)
+            d["E32"] = torch.empty((n_pp, nnz), dtype=torch.float32, device=self.device)
+            rc = self.lib.eip_pair_matrix(la, lb, _ptr(d["P"]), _ptr(d["A"]), _ptr(d["B"]), _ptr(d["c"]), n_pp,
+                                          _ptr(d["E"]), _ptr(d["E32"]), stream)
+            if rc != 0:
+                raise RuntimeError(f"eip_pair_matrix{key} failed with {rc}")
+            self.pairs.append((key, n_sp, n_pp, d))
+        mol_ks = {}
+        for lc in aux_keys:
+            C, c, off, col = [], [], [0], []
+            n_ks = 0
+            for m, (mol, _, ad) in enumerate(molecules):
+                if lc not in ad:
+                    continue
+                order, ks, moff = group_aux(ad[lc])
+                C.append(ad[lc].A_a[order]); c.append(ad[lc].coefficient[order]); col.append(ks)
+                off.extend((moff[1:] + off[-1]).tolist())
+                mol_ks[(lc, m)] = (n_ks, n_ks + len(ks))
+                n_ks += len(ks)
+            d = {"C": T(np.concatenate(C), torch.float64), "c": T(np.concatenate(c), torch.float64),
+                 "off": T(np.array(off, np.int32), torch.int32), "col": T(np.concatenate(col), torch.int32)}
+            self.aux.append((lc, n_ks, d))
+        # items per (pair class, aux class), molecule by molecule
+        self.items = {}
+        self.n_triplet_items = 0
+        for pi, (key, _, _, _) in enumerate(self.pairs):
+            for ai, (lc, _, _) in enumerate(self.aux):
+                it_sp, it_k0, it_end = [], [], []
+                for m in range(len(molecules)):
+                    if (key, m) not in mol_sp or (lc, m) not in mol_ks:
+                        continue
+                    s0, s1 = mol_sp[(key, m)]
+                    k0, k1 = mol_ks[(lc, m)]
+                    nb = (k1 - k0 + 63) // 64
+                    it_sp.append(np.repeat(np.arange(s0, s1, dtype=np.int32), nb))
+                    it_k0.append(np.tile(k0 + 64 * np.arange(nb, dtype=np.int32), s1 - s0))
+                    it_end.append(np.full((s1 - s0) * nb, k1, np.int32))
+                if it_sp:
+                    sp = np.concatenate(it_sp)
+                    self.items[(pi, ai)] = (T(np.stack([sp, np.concatenate(it_k0)], axis=1), torch.int32),
+                                            T(np.concatenate(it_end), torch.int32))
+                    self.n_triplet_items += len(sp)
+        torch.cuda.current_stream().synchronize()
+
+    def n_integrals(self):
+        return int(sum(n_aux * n_ao * (n_ao + 1) // 2 for n_ao, n_aux in self.sizes))
+
+    def new_output(self):
+        return self.torch.empty(self.n_out, dtype=self.torch.float64, device=self.device)
+
+    def molecule_output(self, out, m):
+        n_ao, n_aux = self.sizes[m]
+        return out[self.out_base[m]:self.out_base[m + 1]].view(n_ao * (n_ao + 1) // 2, n_aux)
+
+    def run(self, out, stream=None, zero=True):
+        if zero:
+            out.zero_()
+        for (pi, ai), (it, end) in self.items.items():
+            (la, lb), _, _, dp = self.pairs[pi]
+            lc, n_ks, da = self.aux[ai]
+            rc = self.lib.eip_three_center_items_batched(
+                la, lb, lc, 0, _ptr(it), _ptr(end), it.shape[0], _ptr(dp["off"]), _ptr(dp["si"]), _ptr(dp["sj"]),
+                _ptr(dp["base"]), _ptr(dp["ld"]), _ptr(dp["P"]), _ptr(dp["E"]), _ptr(dp["E32"]), _vp(0),
+                _ptr(da["off"]), _ptr(da["col"]), _ptr(da["C"]), _ptr(da["c"]), n_ks, _ptr(out),
+                _c_double(self.omega), _stream_handle(stream))
+            if rc != 0:
+                raise RuntimeError(f"eip_three_center_items_batched({la},{lb},{lc}) failed with {rc}")
+        return out

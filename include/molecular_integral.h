@@ -75,6 +75,16 @@ int eip_three_center_items(int i_L, int j_L, int k_L, int use_f32, const int* d_
                            const double* d_ka_coef, int n_ks, double* d_out, long long ld, int layout,
                            double omega, void* stream);
 
+/* Same over shell pairs of several molecules at once (one launch per class for a whole batch):
+ * shell pair sp writes the packed output of its molecule at d_out + d_sp_out_base[sp] with row stride
+ * d_sp_ld[sp] (that molecule's n_aux); every item pairs a shell pair and auxiliary shells of one molecule. */
+int eip_three_center_items_batched(int i_L, int j_L, int k_L, int use_f32, const int* d_items,
+                                   const int* d_items_ks_end, int n_items, const int* d_sp_pp_off, const int* d_sp_i,
+                                   const int* d_sp_j, const long long* d_sp_out_base, const int* d_sp_ld,
+                                   const double* d_pp_P, const double* d_E, const float* d_E32, const int* d_ks_perm,
+                                   const int* d_ks_prim_off, const int* d_ks_col, const double* d_ka_C,
+                                   const double* d_ka_coef, int n_ks, double* d_out, double omega, void* stream);
+
 /* (P|Q) blocks of classes (i_L | j_L), aux l <= 4: d_out[(col_i + m_i) * ld + col_j + m_j]; with
  * same_class, blocks with col_i > col_j are skipped; with mirror, the transposed block is written too. */
 int eip_two_center_device(int i_L, int j_L, const int* d_i_prim_off, const double* d_i_A, const double* d_i_coef,

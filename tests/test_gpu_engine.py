@@ -36,3 +36,77 @@ def test_plan_matches_oracle(workload):
     out2 = plan.run(out=torch.full_like(out, float("nan")))
     torch.cuda.synchronize()
     assert torch.equal(out, out2)
+
+
+def _q_aux_from_metric(plan, J2c):
+    """sqrt(max diagonal (P|P)) per auxiliary shell, per class, in the plan's shell order."""
+    diag = J2c.diagonal().cpu().numpy()
+    out = []
+    for ac in plan.aux:
+        n = 2 * ac.lc + 1
+        idx = ac.ks_start[:, None] + np.arange(n)[None, :]
+        out.append(np.sqrt(np.abs(diag[idx]).max(axis=1)))
+    return out
+
+
+def test_two_center_device_matches_dropin():
+    import torch
+    from electron_integral_playground_amd.engine import ThreeCenterPlan
+    from electron_integral_playground_amd.molecular_integral import two_center
+    mol = random_molecule(n_atom=4, seed=22, orbital_l=(0, 1), aux_l=(0, 1, 2, 3, 4))
+    pairs = pair_utility.form_pair_data(mol, 1e-14)
+    aux = pair_utility.form_auxiliary_pair_data(mol)
+    plan = ThreeCenterPlan(mol, pairs, aux)
+    J = plan.two_center()
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(J.cpu().numpy(), two_center(aux, mol), rtol=RTOL, atol=ATOL)
+
+
+def test_mixed_precision_error_bound():
+    """Float32 triplets (bound < 1e-6) keep the packed result within 1e-10 of the float64 path."""
+    import torch
+    from electron_integral_playground_amd.engine import ThreeCenterPlan
+    mol = generators.make_molecule("mixed", n_atom=12, box=9.0)
+    pl = pair_utility.form_primitive_pair_list(mol, 1e-14)
+    pairs = pair_utility.form_primitive_pair_data(mol, pl)
+    aux = pair_utility.form_auxiliary_pair_data(mol)
+    plan = ThreeCenterPlan(mol, pairs, aux)
+    ref = plan.run()
+    stats = plan.setup_mixed(pl, _q_aux_from_metric(plan, plan.two_center()), threshold=1e-6)
+    print("mixed split", stats, flush=True)
+    assert stats["f32_triplets"] > 0 and stats["f64_triplets"] > 0
+    got = plan.run_mixed()
+    torch.cuda.synchronize()
+    err = (got - ref).abs().max().item()
+    print("max abs error mixed vs float64:", err)
+    assert err <= 1e-10
+
+
+def test_batch_plan_matches_single_molecule_plans():
+    """A batch of uneven molecules in one launch per class, replayed from a HIP graph, equals the
+    per-molecule plans bit for bit."""
+    import torch
+    from electron_integral_playground_amd.engine import BatchPlan, ThreeCenterPlan
+    mols = []
+    for i, n in enumerate([4, 9, 6]):
+        mol = generators.batched_molecule(i, n)
+        mols.append((mol, pair_utility.form_pair_data(mol, 1e-14), pair_utility.form_auxiliary_pair_data(mol)))
+    batch = BatchPlan(mols)
+    out = batch.new_output()
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        batch.run(out)                        # warm-up outside capture
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g):
+        batch.run(out)
+    for replay in range(3):
+        out.fill_(float("nan"))
+        g.replay()
+        torch.cuda.synchronize()
+        for m, (mol, pd, ad) in enumerate(mols):
+            single = ThreeCenterPlan(mol, pd, ad).run()
+            torch.cuda.synchronize()
+            assert torch.equal(batch.molecule_output(out, m), single), (replay, m)
