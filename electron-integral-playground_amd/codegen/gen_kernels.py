@@ -230,14 +230,18 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64"):
     for ij in range(nij):
         for m in ms:
             out.append(f"    {ty} acc_{ij}_{m} = 0;")
-    out.append("    for (int pp = pp0; pp < pp1; ++pp) {")
-    out.append(f"      const {ty}* __restrict__ E = a.{'E32' if f32 else 'E'} + (size_t) pp * {nnz};")
-    out.append("      const double Px = a.pp_P[4 * pp + 0], Py = a.pp_P[4 * pp + 1], Pz = a.pp_P[4 * pp + 2], p = a.pp_P[4 * pp + 3];")
-    out.append("      for (int kp = kp0; kp < kp1; ++kp) {")
-    out.append("        const double q = a.ka_C[4 * kp + 3];")
-    out.append("        const double ck = a.ka_coef[kp];")
+    # Auxiliary primitives (per lane) outside, primitive pairs (wave-uniform) inside: the pair matrix
+    # E of one primitive pair is then loaded right where it is consumed, as scalar loads, instead of
+    # being hoisted out of an inner loop and spilled from SGPRs into VGPR lanes.
+    out.append("    for (int kp = kp0; kp < kp1; ++kp) {")
+    out.append("      const double Cx = a.ka_C[4 * kp + 0], Cy = a.ka_C[4 * kp + 1], Cz = a.ka_C[4 * kp + 2];")
+    out.append("      const double q = a.ka_C[4 * kp + 3];")
+    out.append("      const double ck = a.ka_coef[kp];")
+    out.append("      for (int pp = pp0; pp < pp1; ++pp) {")
+    out.append(f"        const {ty}* __restrict__ E = a.{'E32' if f32 else 'E'} + (size_t) pp * {nnz};")
+    out.append("        const double Px = a.pp_P[4 * pp + 0], Py = a.pp_P[4 * pp + 1], Pz = a.pp_P[4 * pp + 2], p = a.pp_P[4 * pp + 3];")
     # geometry and prefactor in double always (cheap), then cast
-    out.append("        const double dPQx = Px - a.ka_C[4 * kp + 0], dPQy = Py - a.ka_C[4 * kp + 1], dPQz = Pz - a.ka_C[4 * kp + 2];")
+    out.append("        const double dPQx = Px - Cx, dPQy = Py - Cy, dPQz = Pz - Cz;")
     out.append("        double zeta, pf;")
     out.append("        if (a.omega == 0.0) {")
     out.append("            zeta = p * q / (p + q);")

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <numeric>
@@ -172,6 +173,24 @@ int prepare_pairs(DevPairs& d, int la, int lb, bool sph, const double* P_p, cons
     }
     EIP_CHECK(hipGetLastError());
     return 0;
+}
+
+// Zero whole rows of a row-major matrix: one workgroup per listed row, 16-byte stores.
+__global__ void __launch_bounds__(256) zero_rows_kernel(const int* __restrict__ rows, int n_rows, long long ld,
+                                                        double* __restrict__ out)
+{
+    for (int r = blockIdx.x; r < n_rows; r += gridDim.x) {
+        double* row = out + (long long) rows[r] * ld;
+        const long long n2 = ld / 2;
+        double2* row2 = reinterpret_cast<double2*>(row);
+        const bool aligned = (reinterpret_cast<uintptr_t>(row) & 15) == 0;
+        if (aligned) {
+            for (long long k = threadIdx.x; k < n2; k += blockDim.x) row2[k] = make_double2(0.0, 0.0);
+            if ((ld & 1) && threadIdx.x == 0) row[ld - 1] = 0.0;
+        } else {
+            for (long long k = threadIdx.x; k < ld; k += blockDim.x) row[k] = 0.0;
+        }
+    }
 }
 
 }  // namespace
@@ -475,6 +494,14 @@ int eip_three_center_items_batched(int i_L, int j_L, int k_L, int use_f32, const
               d_sp_out_base, d_sp_ld};
     for (int c = 0; c < nchunks; ++c)
         if (eip_launch_j3c(i_L, j_L, k_L, c, use_f32, a, n_items, (hipStream_t) stream) != 0) return 3;
+    return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int eip_zero_rows(const int* d_rows, int n_rows, long long ld, double* d_out, void* stream)
+{
+    if (n_rows <= 0) return 0;
+    const int grid = n_rows < 65536 ? n_rows : 65536;
+    hipLaunchKernelGGL(zero_rows_kernel, dim3(grid), dim3(256), 0, (hipStream_t) stream, d_rows, n_rows, ld, d_out);
     return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 

@@ -166,6 +166,15 @@ class ThreeCenterPlan:
                 sp = np.repeat(np.arange(pc.n_sp, dtype=np.int32), kb)
                 k0 = np.tile(np.arange(kb, dtype=np.int32) * 64, pc.n_sp)
                 self.items[(pi, ai)] = T(np.stack([sp, k0], axis=1), torch.int32)
+        # packed rows no kernel writes (shell pairs screened out entirely): zeroed by run()
+        written = np.zeros(self.n_packed, dtype=bool)
+        for pc in self.pairs:
+            ni, nj = 2 * pc.la + 1, 2 * pc.lb + 1
+            i = (pc.sp_i[:, None, None] + np.arange(ni)[None, :, None]).astype(np.int64)
+            j = (pc.sp_j[:, None, None] + np.arange(nj)[None, None, :]).astype(np.int64)
+            hi, lo = np.maximum(i, j), np.minimum(i, j)
+            written[(hi * (hi + 1) // 2 + lo).ravel()] = True
+        self.zero_rows = T(np.flatnonzero(~written).astype(np.int32), torch.int32)
         torch.cuda.current_stream().synchronize()
 
     # -- sizes -----------------------------------------------------------------------------------
@@ -199,12 +208,20 @@ class ThreeCenterPlan:
         if rc != 0:
             raise RuntimeError(f"eip_three_center_items({pc.la},{pc.lb},{ac.lc}) failed with {rc}")
 
+    def zero_unwritten(self, out, stream=None):
+        """Zero the packed rows of shell pairs that are screened out (every other element is written
+        exactly once by the class kernels)."""
+        rc = self.lib.eip_zero_rows(_ptr(self.zero_rows), int(self.zero_rows.numel()), _c_ll(self.n_aux), _ptr(out),
+                                    _stream_handle(stream))
+        if rc != 0:
+            raise RuntimeError(f"eip_zero_rows failed with {rc}")
+
     def run(self, out=None, stream=None, zero=True):
         """All classes into ``out`` (packed layout); rows of screened-out shell pairs are zero."""
         if out is None:
             out = self.new_output()
         if zero:
-            out.zero_()
+            self.zero_unwritten(out, stream)
         for pi in range(len(self.pairs)):
             for ai in range(len(self.aux)):
                 self.launch_class(pi, ai, out, stream)
@@ -301,7 +318,7 @@ class ThreeCenterPlan:
         if out is None:
             out = self.new_output()
         if zero:
-            out.zero_()
+            self.zero_unwritten(out, stream)
         for (pi, ai), m in self.mixed.items():
             self._launch_items(pi, ai, out, m["it64"], m["end64"], m["perm"], False, stream)
             self._launch_items(pi, ai, out, m["it32"], m["end32"], m["perm"], True, stream)

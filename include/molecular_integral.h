@@ -85,6 +85,10 @@ int eip_three_center_items_batched(int i_L, int j_L, int k_L, int use_f32, const
                                    const int* d_ks_prim_off, const int* d_ks_col, const double* d_ka_C,
                                    const double* d_ka_coef, int n_ks, double* d_out, double omega, void* stream);
 
+/* Zero rows d_rows[0..n_rows) (each ld doubles long) of d_out: the packed rows of screened-out shell
+ * pairs, which no three-centre kernel writes. */
+int eip_zero_rows(const int* d_rows, int n_rows, long long ld, double* d_out, void* stream);
+
 /* (P|Q) blocks of classes (i_L | j_L), aux l <= 4: d_out[(col_i + m_i) * ld + col_j + m_j]; with
  * same_class, blocks with col_i > col_j are skipped; with mirror, the transposed block is written too. */
 int eip_two_center_device(int i_L, int j_L, const int* d_i_prim_off, const double* d_i_A, const double* d_i_coef,
