@@ -84,6 +84,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune", metavar="JSON", help="autotune chunk-plan variants first and save them to JSON")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run one 4 GiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the integral path here is a single-GPU build "
@@ -93,7 +96,8 @@ def main():
     import torch
     __graft_entry__.load_package()
     from electron_integral_playground_amd import generators, pair_utility
-    from electron_integral_playground_amd.engine import ThreeCenterPlan, three_center_flops_per_triplet
+    from electron_integral_playground_amd.engine import (ThreeCenterPlan, apply_tuning, autotune,
+                                                          three_center_flops_per_triplet)
 
     torch.cuda.set_device(0)
     t0 = time.perf_counter()
@@ -106,6 +110,16 @@ def main():
     plan = ThreeCenterPlan(mol, pairs, aux)
     out = plan.new_output()
     stream = torch.cuda.current_stream()
+    if args.copy_probe:
+        a = torch.empty(1 << 29, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)                                   # 4 GiB read + 4 GiB written
+        torch.cuda.synchronize()
+        del a, b
+    if args.tune:
+        variants = autotune(plan, save=args.tune, log=log)
+    else:
+        variants = apply_tuning().get("variants", {})
 
     for _ in range(args.warmup):
         plan.run(out)
@@ -170,6 +184,7 @@ def main():
                        "primitive_pairs": int(sum(len(p.P_p) for p in pairs.values())),
                        "output": "packed (n_ao(n_ao+1)/2, n_aux) float64 on device"},
             "roofline": roofline, "cpu_baseline": cpu,
+            "three_center_variants": variants,
             "max_abs_err_vs_reference_sample": err}
     print(json.dumps(line), flush=True)
 

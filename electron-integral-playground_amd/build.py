@@ -42,22 +42,35 @@ def _compile(src):
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src.name}:\n{r.stderr[-4000:]}")
-    scratch, name = {}, None
+    res, name = {}, None
+    keys = {"VGPRs:": "vgpr", "AGPRs:": "agpr", "ScratchSize [bytes/lane]:": "scratch",
+            "Occupancy [waves/SIMD]:": "occupancy"}
     for line in r.stderr.splitlines():
         if "Function Name:" in line:
             name = line.split("Function Name:")[1].split()[0]
-        elif "ScratchSize [bytes/lane]:" in line and name:
-            scratch[name] = int(line.split("ScratchSize [bytes/lane]:")[1].split()[0])
-    return obj, scratch
+            res[name] = {}
+            continue
+        for k, v in keys.items():
+            if k in line and name:
+                res[name][v] = int(line.split(k)[1].split()[0])
+    return obj, res
 
 
-def _spilling_classes(scratch):
-    """Three-centre classes (la, lb, lc) with a kernel that uses scratch memory."""
+def _spilling_classes(scratch, shrink):
+    """Three-centre classes (la, lb, lc) whose default chunk plan has a kernel using scratch memory."""
+    sys.path.insert(0, str(PKG))
+    try:
+        from codegen.gen_kernels import m_chunks
+    finally:
+        sys.path.pop(0)
     out = set()
     for name, nbytes in scratch.items():
         if nbytes > 0 and name.startswith("j3c_"):
-            cls = name.split("_")[1]
-            out.add((int(cls[0]), int(cls[1]), int(cls[2])))
+            parts = name.split("_")
+            cls = parts[1]
+            la, lb, lc = int(cls[0]), int(cls[1]), int(cls[2])
+            if (int(parts[2]), int(parts[3])) in m_chunks(la, lb, lc, shrink.get((la, lb, lc), 0)):
+                out.add((la, lb, lc))
     return out
 
 
@@ -84,8 +97,9 @@ def build(jobs=None, verbose=False):
             for src, res in zip(srcs, ex.map(_compile, srcs)):
                 results[src] = res
         objs = [o for o, _ in results.values()]
-        scratch = {k: v for _, sc in results.values() for k, v in sc.items()}
-        spilling = {c for c in _spilling_classes(scratch) if shrink.get(c, 0) < 3}
+        resources = {k: v for _, sc in results.values() for k, v in sc.items()}
+        scratch = {k: v.get("scratch", 0) for k, v in resources.items()}
+        spilling = {c for c in _spilling_classes(scratch, shrink) if shrink.get(c, 0) < 3}
         if not spilling or attempt == 3:
             break
         for c in spilling:
@@ -93,6 +107,9 @@ def build(jobs=None, verbose=False):
         if verbose:
             print("regenerating with smaller chunks:", sorted(spilling))
     shrink_file.write_text(json.dumps({json.dumps(list(k)): v for k, v in shrink.items()}))
+    # per-kernel registers / scratch / occupancy, read by the autotuner (variants that spill are skipped)
+    LIBDIR.mkdir(parents=True, exist_ok=True)
+    (LIBDIR / "kernel_resources.json").write_text(json.dumps(resources, indent=0, sort_keys=True))
     still = sorted(k for k, v in scratch.items() if v > 0)
     if verbose and still:
         print("kernels still using scratch:", still)

@@ -484,11 +484,11 @@ def _needed_R(la, lb, lc, m0, m1):
 REG_BUDGET_DOUBLES = 200
 
 
-def m_chunks(la, lb, lc, shrink=0):
+def m_chunks(la, lb, lc, shrink=0, budget=None):
     """Split the 2lc+1 auxiliary components of class (la lb|lc) over kernels to bound registers."""
     nij = (2 * la + 1) * (2 * lb + 1)
     nk = 2 * lc + 1
-    budget = REG_BUDGET_DOUBLES >> shrink
+    budget = (budget if budget is not None else REG_BUDGET_DOUBLES) >> shrink
     per = nk
     while per > 1:
         n = -(-nk // per)

@@ -15,6 +15,8 @@ from .gen_kernels import (emit_j2c, emit_j3c, emit_nuc, emit_pairE, m_chunks, pa
 
 SPEC_MAX_L = 3
 SPEC_MAX_AUX_L = 4
+# Register budgets (doubles per lane) of the chunk-plan variants; variant 0 is the default.
+VARIANT_BUDGETS = (200, 320, 120)
 
 HEADER = """// Generated by codegen/generate.py -- do not edit.
 #include "../md_device.hpp"
@@ -68,42 +70,58 @@ def generate(out_dir, shrink=None):
     reg.append("int eip_pair_nnz(int la, int lb);")
 
     # ---- three-centre ----
-    chunk_rows = []
+    # Each class gets up to len(VARIANT_BUDGETS) chunk plans (register budgets); variant 0 is the
+    # default, the others are candidates for the autotuner.  Identical chunks are emitted once.
+    plan_rows = []          # (la, lb, lc, variant, chunk, m0, m1)
     for la, lb in pair_classes:
+        plans = {}
+        for lc in range(SPEC_MAX_AUX_L + 1):
+            cls_plans = []
+            for budget in VARIANT_BUDGETS:
+                plan = m_chunks(la, lb, lc, shrink.get((la, lb, lc), 0), budget)
+                if plan not in cls_plans:
+                    cls_plans.append(plan)
+            plans[lc] = cls_plans
+            for v, plan in enumerate(cls_plans):
+                for ci, (m0, m1) in enumerate(plan):
+                    plan_rows.append((la, lb, lc, v, ci, m0, m1))
         for prec in ("f64", "f32"):
             src = [HEADER]
             cases = []
-            for lc in range(SPEC_MAX_AUX_L + 1):
-                for ci, (m0, m1) in enumerate(m_chunks(la, lb, lc, shrink.get((la, lb, lc), 0))):
+            for lc, cls_plans in plans.items():
+                for m0, m1 in sorted({c for plan in cls_plans for c in plan}):
                     name, code = emit_j3c(la, lb, lc, m0, m1, prec)
                     src.append(code)
-                    cases.append(f"        case {lc * 100 + ci}: hipLaunchKernelGGL({name}, dim3(nblocks), dim3(64), 0, s, a); return 0;")
-                    if prec == "f64":
-                        chunk_rows.append((la, lb, lc, ci, m0, m1))
+                    cases.append(f"        case {lc * 10000 + m0 * 100 + m1}: hipLaunchKernelGGL({name}, dim3(nblocks), dim3(64), 0, s, a); return 0;")
             fn = f"eip_launch_j3c_{la}{lb}_{prec}"
-            src.append(f"int {fn}(int lc, int chunk, const J3cArgs& a, int nblocks, hipStream_t s)\n{{\n"
-                       "    switch (lc * 100 + chunk) {\n" + "\n".join(cases) + "\n    }\n    return -1;\n}\n")
+            src.append(f"int {fn}(int lc, int m0, int m1, const J3cArgs& a, int nblocks, hipStream_t s)\n{{\n"
+                       "    switch (lc * 10000 + m0 * 100 + m1) {\n" + "\n".join(cases) + "\n    }\n    return -1;\n}\n")
             _write(os.path.join(out_dir, f"j3c_{la}{lb}_{prec}.hip"), "\n".join(src))
-            reg.append(f"int {fn}(int lc, int chunk, const J3cArgs& a, int nblocks, hipStream_t s);")
-    reg.append("int eip_launch_j3c(int la, int lb, int lc, int chunk, int f32, const J3cArgs& a, int nblocks, hipStream_t s);")
-    reg.append("int eip_j3c_nchunks(int la, int lb, int lc);")
-    reg.append("int eip_j3c_chunk(int la, int lb, int lc, int chunk, int* m0, int* m1);")
-    disp.append("int eip_launch_j3c(int la, int lb, int lc, int chunk, int f32, const J3cArgs& a, int nblocks, hipStream_t s)\n{")
+            reg.append(f"int {fn}(int lc, int m0, int m1, const J3cArgs& a, int nblocks, hipStream_t s);")
+    reg.append("int eip_launch_j3c(int la, int lb, int lc, int variant, int chunk, int f32, const J3cArgs& a, int nblocks, hipStream_t s);")
+    reg.append("int eip_j3c_nvariants(int la, int lb, int lc);")
+    reg.append("int eip_j3c_nchunks(int la, int lb, int lc, int variant);")
+    reg.append("int eip_j3c_chunk(int la, int lb, int lc, int variant, int chunk, int* m0, int* m1);")
+    disp.append("static const short kChunks[][7] = {")
+    for row in plan_rows:
+        disp.append("    {%d, %d, %d, %d, %d, %d, %d}," % row)
+    disp.append("};")
+    disp.append("int eip_j3c_chunk(int la, int lb, int lc, int variant, int chunk, int* m0, int* m1)\n{\n"
+                "    for (const auto& r : kChunks)\n"
+                "        if (r[0] == la && r[1] == lb && r[2] == lc && r[3] == variant && r[4] == chunk) { *m0 = r[5]; *m1 = r[6]; return 0; }\n"
+                "    return -1;\n}\n")
+    disp.append("int eip_j3c_nchunks(int la, int lb, int lc, int variant)\n{\n    int n = 0;\n"
+                "    for (const auto& r : kChunks) n += (r[0] == la && r[1] == lb && r[2] == lc && r[3] == variant);\n    return n;\n}\n")
+    disp.append("int eip_j3c_nvariants(int la, int lb, int lc)\n{\n    int n = 0;\n"
+                "    for (const auto& r : kChunks) if (r[0] == la && r[1] == lb && r[2] == lc && r[3] + 1 > n) n = r[3] + 1;\n    return n;\n}\n")
+    disp.append("int eip_launch_j3c(int la, int lb, int lc, int variant, int chunk, int f32, const J3cArgs& a, int nblocks, hipStream_t s)\n{")
+    disp.append("    int m0, m1;")
+    disp.append("    if (eip_j3c_chunk(la, lb, lc, variant, chunk, &m0, &m1) != 0) return -1;")
     disp.append("    switch (la * 10 + lb) {")
     for la, lb in pair_classes:
-        disp.append(f"        case {la * 10 + lb}: return f32 ? eip_launch_j3c_{la}{lb}_f32(lc, chunk, a, nblocks, s)"
-                    f" : eip_launch_j3c_{la}{lb}_f64(lc, chunk, a, nblocks, s);")
+        disp.append(f"        case {la * 10 + lb}: return f32 ? eip_launch_j3c_{la}{lb}_f32(lc, m0, m1, a, nblocks, s)"
+                    f" : eip_launch_j3c_{la}{lb}_f64(lc, m0, m1, a, nblocks, s);")
     disp.append("    }\n    return -1;\n}\n")
-    disp.append("static const short kChunks[][6] = {")
-    for row in chunk_rows:
-        disp.append("    {%d, %d, %d, %d, %d, %d}," % row)
-    disp.append("};")
-    disp.append("int eip_j3c_nchunks(int la, int lb, int lc)\n{\n    int n = 0;\n"
-                "    for (const auto& r : kChunks) n += (r[0] == la && r[1] == lb && r[2] == lc);\n    return n;\n}\n")
-    disp.append("int eip_j3c_chunk(int la, int lb, int lc, int chunk, int* m0, int* m1)\n{\n"
-                "    for (const auto& r : kChunks)\n"
-                "        if (r[0] == la && r[1] == lb && r[2] == lc && r[3] == chunk) { *m0 = r[4]; *m1 = r[5]; return 0; }\n"
-                "    return -1;\n}\n")
 
     # ---- nuclear attraction ----
     src = [HEADER]

@@ -193,6 +193,11 @@ __global__ void __launch_bounds__(256) zero_rows_kernel(const int* __restrict__ 
     }
 }
 
+// Chunk-plan variant in use per three-centre class (index [f32][la][lb][lc]); set by the autotuner.
+int g_variant[2][EIP_SPEC_MAX_L + 1][EIP_SPEC_MAX_L + 1][EIP_SPEC_MAX_AUX_L + 1] = {};
+
+inline int current_variant(int la, int lb, int lc, int f32) { return g_variant[f32 ? 1 : 0][la][lb][lc]; }
+
 }  // namespace
 
 extern "C" {
@@ -401,9 +406,10 @@ int three_center(int i_L, int j_L, int k_L, const double* pair_P_p, const double
             a.ld = ld;
             a.layout = 1;
             a.omega = omega;
-            const int nchunks = eip_j3c_nchunks(i_L, j_L, k_L);
+            const int v = current_variant(i_L, j_L, k_L, 0);
+            const int nchunks = eip_j3c_nchunks(i_L, j_L, k_L, v);
             for (int c = 0; c < nchunks; ++c)
-                if (eip_launch_j3c(i_L, j_L, k_L, c, 0, a, nb * kblocks, s) != 0) return 3;
+                if (eip_launch_j3c(i_L, j_L, k_L, v, c, 0, a, nb * kblocks, s) != 0) return 3;
             EIP_CHECK(hipGetLastError());
             EIP_CHECK(hipStreamSynchronize(s));        // 'it' is released at scope exit
         } else {
@@ -446,10 +452,32 @@ int eip_pair_matrix_nnz(int i_L, int j_L)
     return eip_pair_nnz(i_L, j_L);
 }
 
+static bool spec_3c(int i_L, int j_L, int k_L)
+{
+    return i_L >= 0 && j_L >= 0 && i_L <= j_L && j_L <= EIP_SPEC_MAX_L && k_L >= 0 && k_L <= EIP_SPEC_MAX_AUX_L;
+}
+
 int eip_three_center_nchunks(int i_L, int j_L, int k_L)
 {
-    if (!(i_L >= 0 && j_L >= 0 && i_L <= j_L && j_L <= EIP_SPEC_MAX_L && k_L >= 0 && k_L <= EIP_SPEC_MAX_AUX_L)) return -1;
-    return eip_j3c_nchunks(i_L, j_L, k_L);
+    if (!spec_3c(i_L, j_L, k_L)) return -1;
+    return eip_j3c_nchunks(i_L, j_L, k_L, current_variant(i_L, j_L, k_L, 0));
+}
+
+int eip_three_center_nvariants(int i_L, int j_L, int k_L)
+{
+    return spec_3c(i_L, j_L, k_L) ? eip_j3c_nvariants(i_L, j_L, k_L) : -1;
+}
+
+int eip_three_center_chunk(int i_L, int j_L, int k_L, int variant, int chunk, int* m0, int* m1)
+{
+    return spec_3c(i_L, j_L, k_L) ? eip_j3c_chunk(i_L, j_L, k_L, variant, chunk, m0, m1) : -1;
+}
+
+int eip_set_three_center_variant(int i_L, int j_L, int k_L, int use_f32, int variant)
+{
+    if (!spec_3c(i_L, j_L, k_L) || variant < 0 || variant >= eip_j3c_nvariants(i_L, j_L, k_L)) return 1;
+    g_variant[use_f32 ? 1 : 0][i_L][j_L][k_L] = variant;
+    return 0;
 }
 
 int eip_pair_matrix(int i_L, int j_L, const double* d_P_p, const double* d_A_a, const double* d_B_b,
@@ -474,8 +502,9 @@ int eip_three_center_items(int i_L, int j_L, int k_L, int use_f32, const int* d_
     if (n_items <= 0) return 0;
     J3cArgs a{d_items, d_items_ks_end, 0, n_items, d_sp_pp_off, d_sp_i, d_sp_j, d_pp_P, d_E, d_E32,
               d_ks_perm, d_ks_prim_off, d_ks_col, d_ka_C, d_ka_coef, n_ks, d_out, ld, layout, omega};
-    for (int c = 0; c < nchunks; ++c)
-        if (eip_launch_j3c(i_L, j_L, k_L, c, use_f32, a, n_items, (hipStream_t) stream) != 0) return 3;
+    const int v = current_variant(i_L, j_L, k_L, use_f32);
+    for (int c = 0; c < eip_j3c_nchunks(i_L, j_L, k_L, v); ++c)
+        if (eip_launch_j3c(i_L, j_L, k_L, v, c, use_f32, a, n_items, (hipStream_t) stream) != 0) return 3;
     return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
@@ -492,8 +521,9 @@ int eip_three_center_items_batched(int i_L, int j_L, int k_L, int use_f32, const
     J3cArgs a{d_items, d_items_ks_end, 0, n_items, d_sp_pp_off, d_sp_i, d_sp_j, d_pp_P, d_E, d_E32,
               d_ks_perm, d_ks_prim_off, d_ks_col, d_ka_C, d_ka_coef, n_ks, d_out, 0, 0, omega,
               d_sp_out_base, d_sp_ld};
-    for (int c = 0; c < nchunks; ++c)
-        if (eip_launch_j3c(i_L, j_L, k_L, c, use_f32, a, n_items, (hipStream_t) stream) != 0) return 3;
+    const int v = current_variant(i_L, j_L, k_L, use_f32);
+    for (int c = 0; c < eip_j3c_nchunks(i_L, j_L, k_L, v); ++c)
+        if (eip_launch_j3c(i_L, j_L, k_L, v, c, use_f32, a, n_items, (hipStream_t) stream) != 0) return 3;
     return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 

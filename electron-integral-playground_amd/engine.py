@@ -453,3 +453,85 @@ class BatchPlan:
             if rc != 0:
                 raise RuntimeError(f"eip_three_center_items_batched({la},{lb},{lc}) failed with {rc}")
         return out
+
+
+# ------------------------------------------------------------------------------------------------
+# Autotuner: per three-centre class, pick the chunk-plan variant (register budget / occupancy trade-off)
+# that runs fastest on the actual workload, skipping variants whose kernels spill to scratch memory.
+# ------------------------------------------------------------------------------------------------
+
+import json as _json
+import pathlib as _pathlib
+
+TUNING_FILE = _pathlib.Path(__file__).resolve().parent / "tuning" / "three_center_variants.json"
+RESOURCES_FILE = _pathlib.Path(__file__).resolve().parent / "lib" / "kernel_resources.json"
+
+
+def _variant_kernels(lib, la, lb, lc, v, f32):
+    m0, m1 = ctypes.c_int(), ctypes.c_int()
+    names, c = [], 0
+    while lib.eip_three_center_chunk(la, lb, lc, v, c, ctypes.byref(m0), ctypes.byref(m1)) == 0:
+        names.append(f"j3c_{la}{lb}{lc}_{m0.value}_{m1.value}" + ("_f32" if f32 else ""))
+        c += 1
+    return names
+
+
+def usable_variants(lib, la, lb, lc, f32=False):
+    """Variants of a class none of whose kernels use scratch memory (variant 0 is always kept)."""
+    res = _json.loads(RESOURCES_FILE.read_text()) if RESOURCES_FILE.exists() else {}
+    out = []
+    for v in range(max(0, lib.eip_three_center_nvariants(la, lb, lc))):
+        names = _variant_kernels(lib, la, lb, lc, v, f32)
+        if v == 0 or all(res.get(n, {}).get("scratch", 1) == 0 for n in names):
+            out.append(v)
+    return out
+
+
+def apply_tuning(path=TUNING_FILE):
+    """Set every class's variant from the tuning file (if it exists); returns the table."""
+    lib = library()
+    if not _pathlib.Path(path).exists():
+        return {}
+    table = _json.loads(_pathlib.Path(path).read_text())
+    for key, v in table.get("variants", {}).items():
+        cls, prec = key.split("_")
+        lib.eip_set_three_center_variant(int(cls[0]), int(cls[1]), int(cls[2]), int(prec == "f32"), int(v))
+    return table
+
+
+def autotune(plan, repeats=3, use_f32=False, save=None, log=None):
+    """Time every usable variant of every class of ``plan`` and keep the fastest."""
+    torch = plan.torch
+    lib = plan.lib
+    out = plan.new_output()
+    chosen = {}
+    stream = torch.cuda.current_stream()
+    for pi, pc in enumerate(plan.pairs):
+        for ai, ac in enumerate(plan.aux):
+            key = f"{pc.la}{pc.lb}{ac.lc}_{'f32' if use_f32 else 'f64'}"
+            best = None
+            for v in usable_variants(lib, pc.la, pc.lb, ac.lc, use_f32):
+                lib.eip_set_three_center_variant(pc.la, pc.lb, ac.lc, int(use_f32), v)
+                plan.launch_class(pi, ai, out, use_f32=use_f32)          # warm-up
+                times = []
+                for _ in range(repeats):
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record(stream)
+                    plan.launch_class(pi, ai, out, use_f32=use_f32)
+                    e1.record(stream)
+                    e1.synchronize()
+                    times.append(e0.elapsed_time(e1))
+                t = min(times)
+                if log:
+                    log(f"  autotune {key} variant {v}: {t:.3f} ms")
+                if best is None or t < best[1]:
+                    best = (v, t)
+            lib.eip_set_three_center_variant(pc.la, pc.lb, ac.lc, int(use_f32), best[0])
+            chosen[key] = best[0]
+    if save:
+        path = _pathlib.Path(save)
+        path.parent.mkdir(parents=True, exist_ok=True)
+        old = _json.loads(path.read_text()) if path.exists() else {"variants": {}}
+        old["variants"].update(chosen)
+        path.write_text(_json.dumps(old, indent=1, sort_keys=True))
+    return chosen

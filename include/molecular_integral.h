@@ -59,6 +59,13 @@ int eip_pair_matrix_nnz(int i_L, int j_L);
 /* Number of kernels (auxiliary-component chunks) of class (i_L j_L | k_L), or -1 (aux l <= 4). */
 int eip_three_center_nchunks(int i_L, int j_L, int k_L);
 
+/* Chunk-plan variants of class (i_L j_L | k_L) (each splits the auxiliary components over kernels with
+ * a different register budget), the [m0, m1) range of one chunk, and the variant the entry points use
+ * for float64 (use_f32 = 0) or float32 kernels.  Defaults to variant 0; chosen by the autotuner. */
+int eip_three_center_nvariants(int i_L, int j_L, int k_L);
+int eip_three_center_chunk(int i_L, int j_L, int k_L, int variant, int chunk, int* m0, int* m1);
+int eip_set_three_center_variant(int i_L, int j_L, int k_L, int use_f32, int variant);
+
 /* Pair matrix (Hermite expansion contracted into real solid harmonics, pair coefficient folded in)
  * for n_pp primitive pairs: d_E[n_pp][nnz] float64 and, when d_E32 is not null, a float32 copy. */
 int eip_pair_matrix(int i_L, int j_L, const double* d_P_p, const double* d_A_a, const double* d_B_b,
