@@ -28,7 +28,8 @@ import __graft_entry__  # noqa: E402
 
 PEAK_FP64 = 78.6e12          # MI355X FP64 vector (and matrix) peak, FLOP/s
 PEAK_HBM = 8.0e12            # bytes/s
-SAMPLE_SHELLS = 24           # auxiliary shells per angular class in the CPU-baseline sample
+SAMPLE_SHELLS = 96           # auxiliary shells per angular class in the CPU-baseline sample
+PMC_SUMMARY = ROOT / "profiles" / "j3c_plain_pmc.md"
 
 
 def log(*a):
@@ -76,6 +77,25 @@ def cpu_baseline(mol, pairs, aux, gpu_packed, max_seconds=20.0):
     sample = f"{n_shell_per_class} auxiliary shells per angular class x all {sum(len(p.P_p) for p in pairs.values())} primitive pairs"
     return {"value": total_int / total_t, "unit": "3c integrals/s", "cores": int(os.environ["OMP_NUM_THREADS"]),
             "kind": kind, "sample": sample, "seconds": round(total_t, 3)}, max_err
+
+
+def measured_traffic(la, lb, lc):
+    """HBM bytes of one pass of class (la lb | lc) (all its component kernels), from the committed
+    rocprofv3 WRITE_SIZE / FETCH_SIZE summary (calibrated against a device copy); None if absent."""
+    if not PMC_SUMMARY.exists():
+        return None
+    total, found = 0.0, False
+    for line in PMC_SUMMARY.read_text().splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        # rows of the traffic table: | kernel | WRITE_SIZE KiB | FETCH_SIZE KiB | calibrated bytes (x.xxxxe+yy) |
+        if (len(cells) == 4 and cells[0].startswith(f"j3c_{la}{lb}{lc}_") and not cells[0].endswith("_f32")
+                and "e+" in cells[3]):
+            try:
+                total += float(cells[3])
+                found = True
+            except ValueError:
+                pass
+    return total if found else None
 
 
 def main():
@@ -160,7 +180,8 @@ def main():
     nchunks = plan.lib.eip_three_center_nchunks(*key)
     roofline = {"bound": "mfma", "pipe": "fp64-valu", "kernel": "j3c_%d%d%d (all %d component chunks)" % (key + (nchunks,)),
                 "achieved": fl / (ms * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-                "frac": fl / (ms * 1e-3) / PEAK_FP64, "traffic": None}
+                "frac": fl / (ms * 1e-3) / PEAK_FP64, "traffic": measured_traffic(*key),
+                "traffic_unit": "bytes per class pass (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)"}
 
     cpu, err = (None, None)
     if not args.no_cpu:
