@@ -1,12 +1,15 @@
-"""Primitive-pair screening and structure-of-arrays packing, vectorised over shell pairs.
+"""Primitive-pair screening and structure-of-arrays packing.
 
 Public functions keep the reference's names, arguments and results (pair_utility.py:6-229):
 ``form_primitive_pair_list``, ``form_primitive_pair_data``, ``form_pair_data``,
-``form_primitive_auxiliary_pair_data``, ``form_auxiliary_pair_data``.
+``form_primitive_auxiliary_pair_data``, ``form_auxiliary_pair_data``; ``form_pair_arrays`` is the same
+list as arrays (no per-pair Python objects), which the device path uses.
 
-Screening decisions must be bit-identical to the reference, so every floating-point value is formed
-with the same numpy operations and the same association order as there; only the loop over shell
-pairs is replaced by array operations over all shell pairs that share a primitive-count signature.
+Screening decisions must be bit-identical to the reference.  The O(N_shell^2 N_prim^2) loop
+(pair_utility.py:78-124) runs natively in the library (``eip_screen_primitive_pairs``): shell-pair
+admission, the cumulative distance rule and a conservative pre-screen of the bound.  The exact bound of
+every surviving candidate is then evaluated here with the same numpy operations, in the same
+association order, as the reference, and the exact keep test and ordering are applied to those values.
 Two reference behaviours are reproduced on purpose:
 
 * the distance used in the bound is shortened *cumulatively* inside the primitive loops and clamped at
@@ -19,13 +22,31 @@ Two reference behaviours are reproduced on purpose:
 (pair_utility.py:185-188).
 """
 
+import ctypes
 from collections import OrderedDict
+from dataclasses import dataclass
 
 import numpy as np
 
+from . import library
 from .data_structure import Molecule, PrimitivePair, PrimitivePairData
 
 REFERENCE_SCREENING_THRESHOLD = 1e-14
+# relative slack of the native pre-screen (its libm exp/pow differ from numpy's by a few ulp at most)
+PRESCREEN_MARGIN = 1e-9
+
+
+@dataclass
+class PairArrays:
+    """One angular class of the screened list, in the reference's order (descending bound)."""
+    i_shell: np.ndarray       # int64
+    j_shell: np.ndarray
+    i_primitive: np.ndarray
+    j_primitive: np.ndarray
+    upper_bound: np.ndarray   # float64
+
+    def __len__(self):
+        return len(self.upper_bound)
 
 
 def _atom_pair_distance(geometry: np.ndarray):
@@ -41,37 +62,86 @@ def _atom_pair_distance(geometry: np.ndarray):
     return norm, sq
 
 
-def _screen_group(li, lj, exp_i, coef_i, exp_j, coef_j, dist, threshold):
-    """Bounds for a group of shell pairs with equal primitive counts.
+def _shell_arrays(shells):
+    l = np.array([s.angular for s in shells], dtype=np.int32)
+    ao = np.array([s.i_ao_start for s in shells], dtype=np.int32)
+    atom = np.array([s.i_atom for s in shells], dtype=np.int32)
+    n_prim = np.array([len(s.primitive_exponents) for s in shells], dtype=np.int64)
+    off = np.concatenate([[0], np.cumsum(n_prim)]).astype(np.int32)
+    exps = np.concatenate([np.asarray(s.primitive_exponents, dtype=np.float64) for s in shells])
+    coefs = np.concatenate([np.asarray(s.primitive_coefficients, dtype=np.float64) for s in shells])
+    return l, ao, atom, off, exps, coefs
 
-    exp_i/coef_i: (n_group, ni); exp_j/coef_j: (n_group, nj); dist: (n_group,).
-    Returns ``bound`` (n_group, ni, nj) and ``keep`` mask.
-    """
-    n_group, ni = exp_i.shape
-    nj = exp_j.shape[1]
-    cA = np.abs(coef_i)
+
+def _native_candidates(shell_arrays, norm, threshold):
+    """(i_shell, j_shell, i_prim, j_prim, clamped distance) of every pre-screened candidate, loop order."""
+    lib = library()
+    l, ao, atom, off, exps, coefs = shell_arrays
+    norm = np.ascontiguousarray(norm, dtype=np.float64)
+    ip_ = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    dp_ = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    fn = lib.eip_screen_primitive_pairs
+    fn.restype = ctypes.c_longlong
+    args = (len(l), ip_(l), ip_(ao), ip_(atom), ip_(off), dp_(exps), dp_(coefs), dp_(norm), norm.shape[0],
+            ctypes.c_double(threshold), ctypes.c_double(PRESCREEN_MARGIN))
+    n = fn(*args, ctypes.c_longlong(0), None, None, None, None, None)
+    if n < 0:
+        raise RuntimeError("eip_screen_primitive_pairs rejected its arguments")
+    out = [np.empty(n, dtype=np.int32) for _ in range(4)] + [np.empty(n, dtype=np.float64)]
+    m = fn(*args, ctypes.c_longlong(n), *(ip_(a) for a in out[:4]), dp_(out[4]))
+    if m != n:
+        raise RuntimeError(f"eip_screen_primitive_pairs returned {m} candidates, expected {n}")
+    return out
+
+
+def _exact_bound(li, lj, a, ca, b, cb, d):
+    """The reference's bound (pair_utility.py:95-114), same numpy operations and association order."""
+    cA = np.abs(ca)
     if li > 0:
-        cA = cA * np.power(li / (2.0 * np.e * exp_i), li / 2.0)
-    cB = np.abs(coef_j)
+        cA = cA * np.power(li / (2.0 * np.e * a), li / 2.0)
+    cB = np.abs(cb)
     if lj > 0:
-        cB = cB * np.power(lj / (2.0 * np.e * exp_j), lj / 2.0)
-    shrink_i = np.sqrt(li / 2.0 / exp_i)
-    shrink_j = np.sqrt(lj / 2.0 / exp_j)
-    d = dist.copy()
-    bound = np.empty((n_group, ni, nj))
-    for ip in range(ni):
-        a = exp_i[:, ip]
-        d = d - shrink_i[:, ip]
-        for jp in range(nj):
-            b = exp_j[:, jp]
-            d = d - shrink_j[:, jp]
-            d = np.where(0.0 > d, 0.0, d)          # Python max(d, 0.0)
-            gaussian = np.exp(-a * b / (a + b) * d * d)
-            c_ab = 2.0 * cA[:, ip] * cB[:, jp]
-            overlap_factor = np.power(np.pi / (a + b), 1.5)
-            schwarz_factor = np.power(np.pi / (a + b), 1.25) * np.power(2, 0.25)
-            bound[:, ip, jp] = c_ab * gaussian * np.maximum(overlap_factor, schwarz_factor)
-    return bound, ~(bound < threshold)
+        cB = cB * np.power(lj / (2.0 * np.e * b), lj / 2.0)
+    gaussian = np.exp(-a * b / (a + b) * d * d)
+    c_ab = 2.0 * cA * cB
+    overlap_factor = np.power(np.pi / (a + b), 1.5)
+    schwarz_factor = np.power(np.pi / (a + b), 1.25) * np.power(2, 0.25)
+    return c_ab * gaussian * np.maximum(overlap_factor, schwarz_factor)
+
+
+def form_pair_arrays(molecule: Molecule, schwarz_upper_bound: float):
+    """Screened primitive pairs per angular class ``(li, lj)`` as ``PairArrays``, in the reference's key
+    order (first appearance in the loop) and element order (stable descending bound)."""
+    shells = molecule.basis_shells
+    assert shells is not None and len(shells) > 0 and molecule.n_ao > 0
+    norm, _ = _atom_pair_distance(molecule.geometry)
+    sa = _shell_arrays(shells)
+    l, _, _, off, exps, coefs = sa
+    ci, cj, cip, cjp, cd = _native_candidates(sa, norm, schwarz_upper_bound)
+    li_all, lj_all = l[ci], l[cj]
+    ka = off[ci].astype(np.int64) + cip
+    kb = off[cj].astype(np.int64) + cjp
+    keep = np.zeros(len(ci), dtype=bool)
+    ub = np.empty(len(ci))
+    code = li_all.astype(np.int64) * 64 + lj_all
+    for c in np.unique(code):
+        li, lj = int(c) // 64, int(c) % 64
+        sel = np.flatnonzero(code == c)
+        v = _exact_bound(li, lj, exps[ka[sel]], coefs[ka[sel]], exps[kb[sel]], coefs[kb[sel]], cd[sel])
+        ub[sel] = v
+        keep[sel] = ~(v < schwarz_upper_bound)
+    kept = np.flatnonzero(keep)
+    # class order = order of each class's first kept pair in the loop
+    cls = code[kept]
+    _, first = np.unique(cls, return_index=True)
+    out = OrderedDict()
+    for f in np.sort(first):
+        key = (int(li_all[kept[f]]), int(lj_all[kept[f]]))
+        rows = kept[cls == key[0] * 64 + key[1]]
+        rows = rows[np.argsort(-ub[rows], kind="stable")]
+        out[key] = PairArrays(ci[rows].astype(np.int64), cj[rows].astype(np.int64), cip[rows].astype(np.int64),
+                              cjp[rows].astype(np.int64), ub[rows])
+    return out
 
 
 def form_primitive_pair_list(molecule: Molecule, schwarz_upper_bound: float):
@@ -79,66 +149,44 @@ def form_primitive_pair_list(molecule: Molecule, schwarz_upper_bound: float):
 
     Returns ``dict[(li, lj)] -> list[PrimitivePair]`` in the reference's key and element order.
     """
-    shells = molecule.basis_shells
-    assert shells is not None and len(shells) > 0 and molecule.n_ao > 0
-    norm, _ = _atom_pair_distance(molecule.geometry)
+    out = OrderedDict()
+    for key, pa in form_pair_arrays(molecule, schwarz_upper_bound).items():
+        out[key] = [PrimitivePair(int(i), int(j), int(ip), int(jp), ub) for i, j, ip, jp, ub in
+                    zip(pa.i_shell, pa.j_shell, pa.i_primitive, pa.j_primitive, pa.upper_bound)]
+    return out
 
-    # Shell pairs admitted by the class rule (pair_utility.py:87-91), bucketed by primitive counts.
-    buckets = {}
-    for i_shell, si in enumerate(shells):
-        for j_shell, sj in enumerate(shells):
-            if si.angular > sj.angular:
-                continue
-            if si.angular == sj.angular and si.i_ao_start > sj.i_ao_start:
-                continue
-            key = (si.angular, sj.angular, len(si.primitive_exponents), len(sj.primitive_exponents))
-            buckets.setdefault(key, []).append((i_shell, j_shell))
 
-    kept = []   # rows: (i_shell, j_shell, ip, jp, bound, li, lj)
-    for (li, lj, ni, nj), members in buckets.items():
-        idx = np.array(members, dtype=np.int64)
-        exp_i = np.stack([shells[s].primitive_exponents for s in idx[:, 0]])
-        coef_i = np.stack([shells[s].primitive_coefficients for s in idx[:, 0]])
-        exp_j = np.stack([shells[s].primitive_exponents for s in idx[:, 1]])
-        coef_j = np.stack([shells[s].primitive_coefficients for s in idx[:, 1]])
-        atoms_i = np.array([shells[s].i_atom for s in idx[:, 0]])
-        atoms_j = np.array([shells[s].i_atom for s in idx[:, 1]])
-        bound, keep = _screen_group(li, lj, exp_i, coef_i, exp_j, coef_j,
-                                    norm[atoms_i, atoms_j], schwarz_upper_bound)
-        g, ip, jp = np.nonzero(keep)
-        for row in zip(idx[g, 0], idx[g, 1], ip, jp, bound[g, ip, jp]):
-            kept.append((int(row[0]), int(row[1]), int(row[2]), int(row[3]), row[4], li, lj))
-
-    # Reference append order is lexicographic in (i_shell, j_shell, i_primitive, j_primitive).
-    kept.sort(key=lambda r: (r[0], r[1], r[2], r[3]))
-    pair_list = OrderedDict()
-    for i_shell, j_shell, ip, jp, ub, li, lj in kept:
-        pair_list.setdefault((li, lj), []).append(PrimitivePair(i_shell, j_shell, ip, jp, ub))
-    for key in pair_list:
-        pair_list[key].sort(key=lambda p: p.upper_bound, reverse=True)   # stable, larger first
-    return pair_list
+def _as_arrays(pairs):
+    if isinstance(pairs, PairArrays):
+        return pairs
+    return PairArrays(np.array([p.i_shell for p in pairs], dtype=np.int64),
+                      np.array([p.j_shell for p in pairs], dtype=np.int64),
+                      np.array([p.i_primitive for p in pairs], dtype=np.int64),
+                      np.array([p.j_primitive for p in pairs], dtype=np.int64),
+                      np.array([p.upper_bound for p in pairs], dtype=np.float64))
 
 
 def form_primitive_pair_data(molecule: Molecule, pair_list):
     """Pack each class's pairs into ``PrimitivePairData`` (reference pair_utility.py:135-183).
 
+    ``pair_list`` holds ``list[PrimitivePair]`` or ``PairArrays`` per class.
     ``P = (aA + bB)/p`` and ``coefficient = c_a c_b exp(-ab/p |AB|^2)``.
     """
     shells = molecule.basis_shells
     assert shells is not None and len(shells) > 0 and molecule.n_ao > 0
     _, sq = _atom_pair_distance(molecule.geometry)
     geometry = molecule.geometry
+    _, ao, atom, off, exps, coefs = _shell_arrays(shells)
     out = OrderedDict()
     for key, pairs in pair_list.items():
-        n = len(pairs)
-        si = [shells[p.i_shell] for p in pairs]
-        sj = [shells[p.j_shell] for p in pairs]
-        atom_i = np.array([s.i_atom for s in si], dtype=np.int64)
-        atom_j = np.array([s.i_atom for s in sj], dtype=np.int64)
-        a = np.array([s.primitive_exponents[p.i_primitive] for s, p in zip(si, pairs)])
-        b = np.array([s.primitive_exponents[p.j_primitive] for s, p in zip(sj, pairs)])
-        ca = np.array([s.primitive_coefficients[p.i_primitive] for s, p in zip(si, pairs)])
-        cb = np.array([s.primitive_coefficients[p.j_primitive] for s, p in zip(sj, pairs)])
+        pa = _as_arrays(pairs)
+        n = len(pa)
+        atom_i = atom[pa.i_shell].astype(np.int64)
+        atom_j = atom[pa.j_shell].astype(np.int64)
+        a = exps[off[pa.i_shell] + pa.i_primitive]
+        b = exps[off[pa.j_shell] + pa.j_primitive]
+        ca = coefs[off[pa.i_shell] + pa.i_primitive]
+        cb = coefs[off[pa.j_shell] + pa.j_primitive]
         A = geometry[atom_i, :]
         B = geometry[atom_j, :]
         p = a + b
@@ -149,8 +197,8 @@ def form_primitive_pair_data(molecule: Molecule, pair_list):
             A_a=np.empty((n, 4), dtype=np.float64, order="C"),
             B_b=np.empty((n, 4), dtype=np.float64, order="C"),
             coefficient=ca * cb * gaussian,
-            i_ao_start=np.array([s.i_ao_start for s in si], dtype=np.int32),
-            j_ao_start=np.array([s.i_ao_start for s in sj], dtype=np.int32),
+            i_ao_start=ao[pa.i_shell].astype(np.int32),
+            j_ao_start=ao[pa.j_shell].astype(np.int32),
             i_atom=atom_i.astype(np.int32),
             j_atom=atom_j.astype(np.int32),
         )
@@ -167,8 +215,7 @@ def form_primitive_pair_data(molecule: Molecule, pair_list):
 def form_pair_data(molecule: Molecule, schwarz_upper_bound: float):
     """Screen at the reference's fixed 1e-14 (the argument is ignored, pair_utility.py:185-188)."""
     del schwarz_upper_bound
-    return form_primitive_pair_data(
-        molecule, form_primitive_pair_list(molecule, REFERENCE_SCREENING_THRESHOLD))
+    return form_primitive_pair_data(molecule, form_pair_arrays(molecule, REFERENCE_SCREENING_THRESHOLD))
 
 
 def form_primitive_auxiliary_pair_data(molecule: Molecule):

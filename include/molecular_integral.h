@@ -52,6 +52,20 @@ int three_center(int i_L, int j_L, int k_L, const double* pair_P_p, const double
 
 /* ---- Part 2: device-resident entry points (no counterpart in the reference) --------------- */
 
+/* Host-side (multithreaded) enumeration and pre-screen of primitive pairs, the native loop of reference
+ * electron_integral_playground/pair_utility.py:78-124 (form_primitive_pair_list): admitted shell pairs
+ * (l_i < l_j, or l_i == l_j and ao_start_i <= ao_start_j) in loop order, the cumulative distance rule and
+ * the bound, keeping every primitive pair with bound >= threshold * (1 - margin).  Shells are given as
+ * arrays (l, AO offset, atom, CSR primitive offsets into prim_exp / prim_coef); atom_dist is the
+ * [n_atom][n_atom] table of |A - B|.  Returns the number of candidates (or -1 on bad arguments); the
+ * outputs (i shell, j shell, i primitive, j primitive, clamped distance) are written only when capacity
+ * is at least that number.  The caller applies the exact keep test and the stable descending order. */
+long long eip_screen_primitive_pairs(int n_shell, const int* shell_l, const int* shell_ao_start,
+                                     const int* shell_atom, const int* shell_prim_off, const double* prim_exp,
+                                     const double* prim_coef, const double* atom_dist, int n_atom,
+                                     double threshold, double margin, long long capacity, int* out_i_shell,
+                                     int* out_j_shell, int* out_i_prim, int* out_j_prim, double* out_dist);
+
 /* Entries per primitive pair of the packed pair matrix of class (i_L, j_L), or -1 when the class has
  * no specialised kernels (spherical output, l <= 3). */
 int eip_pair_matrix_nnz(int i_L, int j_L);

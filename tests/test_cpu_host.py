@@ -130,6 +130,41 @@ def test_pair_list_bit_exact_plain_subset():
     _assert_pair_lists_identical(mol)
 
 
+def test_pair_list_bit_exact_mixed_subset():
+    """15 atoms of the 4s3p2d1f generator: every class up to (ff), cumulative-distance rule with l = 3."""
+    mol = generators.make_molecule("mixed", n_atom=15, box=7.0)
+    _assert_pair_lists_identical(mol)
+
+
+def test_pair_arrays_match_pair_list():
+    mol = generators.make_molecule("small")
+    arrays = pair_utility.form_pair_arrays(mol, 1e-14)
+    ref = oracle_pairs.form_primitive_pair_list(mol, 1e-14)
+    assert list(arrays) == list(ref)
+    for k, pa in arrays.items():
+        got = list(zip(pa.i_shell.tolist(), pa.j_shell.tolist(), pa.i_primitive.tolist(), pa.j_primitive.tolist(),
+                       pa.upper_bound.tolist()))
+        assert got == [(p.i_shell, p.j_shell, p.i_primitive, p.j_primitive, p.upper_bound) for p in ref[k]]
+
+
+def test_native_prescreen_keeps_pairs_at_the_threshold():
+    """The keep test is 'not (bound < threshold)' on the exact bound: a threshold equal to a pair's bound
+    keeps it, the next double above drops it (the native pre-screen must not decide either case)."""
+    mol = generators.make_molecule("plain", n_atom=6, box=5.0)
+    ref = oracle_pairs.form_primitive_pair_list(mol, 1e-14)
+    for key in list(ref)[:3]:
+        pair = ref[key][len(ref[key]) // 2]
+        at = pair_utility.form_pair_arrays(mol, pair.upper_bound)
+        above = pair_utility.form_pair_arrays(mol, np.nextafter(pair.upper_bound, np.inf))
+        ids = lambda pa: set(zip(pa.i_shell.tolist(), pa.j_shell.tolist(), pa.i_primitive.tolist(),
+                                 pa.j_primitive.tolist()))
+        me = (pair.i_shell, pair.j_shell, pair.i_primitive, pair.j_primitive)
+        assert me in ids(at[key])
+        assert key not in above or me not in ids(above[key])
+        oracle_at = oracle_pairs.form_primitive_pair_list(mol, pair.upper_bound)
+        assert {k: len(v) for k, v in at.items()} == {k: len(v) for k, v in oracle_at.items()}
+
+
 def test_threshold_argument_ignored():
     """form_pair_data screens at 1e-14 whatever it is given (reference pair_utility.py:185-188)."""
     mol = generators.make_molecule("small")
