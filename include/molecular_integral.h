@@ -9,8 +9,9 @@
  * adds the result into the caller's array.  Thread safe: each calling thread gets its own stream and
  * buffers.
  *
- * Part 2 is additive: device-resident entry points over grouped data already in HBM (no allocation,
- * no synchronisation; every pointer is a device pointer, `stream` is a hipStream_t).
+ * Part 2 is additive: the native primitive-pair pre-screen (host pointers, host threads) and
+ * device-resident entry points over grouped data already in HBM (no allocation, no synchronisation;
+ * every pointer is a device pointer, `stream` is a hipStream_t).
  */
 #ifndef ELECTRON_INTEGRAL_PLAYGROUND_AMD_MOLECULAR_INTEGRAL_H
 #define ELECTRON_INTEGRAL_PLAYGROUND_AMD_MOLECULAR_INTEGRAL_H
@@ -50,7 +51,7 @@ int three_center(int i_L, int j_L, int k_L, const double* pair_P_p, const double
                  int n_pair_ij, const double* aux_A_a, const double* aux_coefficient, const int* aux_i_aux_start,
                  int n_aux_primitive_k, double* J3c_matrix, int n_ao, int n_aux, bool spherical, double omega);
 
-/* ---- Part 2: device-resident entry points (no counterpart in the reference) --------------- */
+/* ---- Part 2: additive entry points (no counterpart in the reference's C ABI) ------------- */
 
 /* Host-side (multithreaded) enumeration and pre-screen of primitive pairs, the native loop of reference
  * electron_integral_playground/pair_utility.py:78-124 (form_primitive_pair_list): admitted shell pairs
