@@ -98,6 +98,102 @@ def measured_traffic(la, lb, lc):
     return total if found else None
 
 
+def q_aux_from_metric(plan, J2c):
+    """Q_P = sqrt(max_m (P_m|P_m)) per auxiliary shell, per class, in the plan's shell order."""
+    diag = J2c.diagonal().cpu().numpy()
+    out = []
+    for ac in plan.aux:
+        idx = ac.ks_start[:, None] + np.arange(2 * ac.lc + 1)[None, :]
+        out.append(np.sqrt(np.abs(diag[idx]).max(axis=1)))
+    return out
+
+
+def mixed_leg(threshold=1e-6, block_bytes=40e9, n_atom=None):
+    """Mixed-precision workload (200 atoms, 20 A box, 4s3p2d1f, auxiliary s-g, n_ao 6000, n_aux 40000):
+    the (ij|P) tensor (3.9 TB over the significant shell pairs) is streamed through two device buffers
+    in blocks of shell pairs; each block is computed by the plain float64 kernels and by the mixed
+    split, and the two are compared on the device (max abs, RMS).  Times are HIP events on the launch
+    stream around each block's launches."""
+    import torch
+    from electron_integral_playground_amd import generators, pair_utility
+    from electron_integral_playground_amd.engine import ThreeCenterPlan
+    t0 = time.perf_counter()
+    mol = generators.make_molecule("mixed", n_atom=n_atom)
+    arrays = pair_utility.form_pair_arrays(mol, 1e-14)
+    pairs = pair_utility.form_primitive_pair_data(mol, arrays)
+    aux = pair_utility.form_auxiliary_pair_data(mol)
+    t_pairs = time.perf_counter() - t0
+    plan = ThreeCenterPlan(mol, pairs, aux)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    J2c = torch.empty((mol.n_aux, mol.n_aux), dtype=torch.float64, device="cuda")
+    plan.two_center(out=J2c.zero_())                           # warm-up
+    J2c.zero_()
+    e0.record(stream)
+    plan.two_center(out=J2c)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_metric = e0.elapsed_time(e1) * 1e-3
+    q_aux = q_aux_from_metric(plan, J2c)
+    del J2c
+    stats = plan.setup_mixed(arrays, q_aux, threshold=threshold)
+    t_setup = time.perf_counter() - t0
+    log(f"mixed workload: n_ao={mol.n_ao} n_aux={mol.n_aux} primitive pairs={sum(len(p) for p in arrays.values())}"
+        f" host pair list {t_pairs:.1f}s, setup {t_setup:.1f}s; split {stats}")
+    blocks = plan.row_blocks(block_bytes)
+    max_rows = max(plan.block_rows(*b) for b in blocks)
+    A = torch.empty((max_rows, mol.n_aux), dtype=torch.float64, device="cuda")
+    B = torch.empty_like(A)
+    # warm-up: one block of every class in both modes
+    seen = set()
+    for b in blocks:
+        if b[0] not in seen:
+            seen.add(b[0])
+            plan.run_block(*b, A)
+            plan.run_block(*b, B, mixed=True)
+    torch.cuda.synchronize()
+    t_plain = t_mixed = 0.0
+    max_abs = torch.zeros((), dtype=torch.float64, device="cuda")
+    sum_sq = torch.zeros((), dtype=torch.float64, device="cuda")
+    max_ref = torch.zeros((), dtype=torch.float64, device="cuda")
+    n_el = 0
+    ev = []
+    for bi, b in enumerate(blocks):
+        rows = plan.block_rows(*b)
+        a0, a1, a2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        a0.record(stream)
+        plan.run_block(*b, A)
+        a1.record(stream)
+        plan.run_block(*b, B, mixed=True)
+        a2.record(stream)
+        ev.append((a0, a1, a2))
+        for r in range(0, rows, 4096):
+            r1 = min(rows, r + 4096)
+            d = B[r:r1] - A[r:r1]
+            max_abs = torch.maximum(max_abs, d.abs().max())
+            sum_sq += (d * d).sum()
+            max_ref = torch.maximum(max_ref, A[r:r1].abs().max())
+        n_el += rows * mol.n_aux
+        if bi % 10 == 0:
+            log(f"  block {bi}/{len(blocks)}")
+    torch.cuda.synchronize()
+    for a0, a1, a2 in ev:
+        t_plain += a0.elapsed_time(a1) * 1e-3
+        t_mixed += a1.elapsed_time(a2) * 1e-3
+    n_int = mol.n_aux * mol.n_ao * (mol.n_ao + 1) // 2
+    res = {"workload": f"mixed: {len(mol.elements)} atoms, 4s3p2d1f (1-6 prim), aux s-g 8 exp/l",
+           "n_ao": mol.n_ao, "n_aux": mol.n_aux,
+           "primitive_pairs": int(sum(len(p) for p in arrays.values())),
+           "threshold": threshold, "f32_triplet_fraction": stats["f32_triplets"] / (stats["f32_triplets"] + stats["f64_triplets"]),
+           "plain_fp64_integrals_per_s": n_int / t_plain, "mixed_integrals_per_s": n_int / t_mixed,
+           "mixed_over_plain": t_plain / t_mixed, "plain_s": t_plain, "mixed_s": t_mixed,
+           "max_abs_err_vs_fp64": max_abs.item(), "rms_err_vs_fp64": (sum_sq.item() / n_el) ** 0.5,
+           "max_abs_value": max_ref.item(), "metric_fp64_s": t_metric,
+           "output": f"{n_el * 8 / 1e12:.2f} TB of significant-shell-pair rows, streamed in {len(blocks)} blocks"}
+    log(f"mixed leg: {res}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +201,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tune", metavar="JSON", help="autotune chunk-plan variants first and save them to JSON")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision workload")
+    ap.add_argument("--mixed-only", action="store_true", help="run only the mixed-precision workload")
+    ap.add_argument("--mixed-atoms", type=int, default=None, help="atoms of the mixed workload (default 200)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 4 GiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -120,6 +219,9 @@ def main():
                                                           three_center_flops_per_triplet)
 
     torch.cuda.set_device(0)
+    if args.mixed_only:
+        print(json.dumps({"mixed_precision": mixed_leg(n_atom=args.mixed_atoms)}), flush=True)
+        return
     t0 = time.perf_counter()
     mol = generators.make_molecule("plain")
     pairs = pair_utility.form_pair_data(mol, 1e-14)
@@ -196,6 +298,11 @@ def main():
         cpu, err = cpu_baseline(mol, pairs, aux, gpu_packed)
         log(f"cpu baseline {cpu}; sample max |GPU - reference| = {err:.3e}")
 
+    mixed = None
+    if not args.no_mixed:
+        del out, plan
+        torch.cuda.empty_cache()
+        mixed = mixed_leg(n_atom=args.mixed_atoms)
     line = {"metric": "3c integrals/s (plain FP64, device-resident packed output)", "value": value,
             "unit": "3c integrals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -206,6 +313,7 @@ def main():
                        "output": "packed (n_ao(n_ao+1)/2, n_aux) float64 on device"},
             "roofline": roofline, "cpu_baseline": cpu,
             "three_center_variants": variants,
+            "mixed_precision": mixed,
             "max_abs_err_vs_reference_sample": err}
     print(json.dumps(line), flush=True)
 

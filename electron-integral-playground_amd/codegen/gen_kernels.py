@@ -218,15 +218,17 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64"):
     out.append("    const int lb_ = eip::xcd_remap(blockIdx.x, gridDim.x);")
     out.append("    const int item = a.item_begin + lb_;")
     out.append("    if (item >= a.item_end) return;")
+    out.append("    const J3cData d = a.jobs ? a.jobs[a.items_job ? a.items_job[item] : 0] : a.d;")
     out.append("    const int sp = a.items[2 * item];")
     out.append("    const int ks_first = a.items[2 * item + 1];")
     out.append("    const int ks_raw = ks_first + (int) threadIdx.x;")
-    out.append("    const int ks_last = a.items_ks_end ? a.items_ks_end[item] : a.n_ks;")
+    out.append("    const int ks_last = a.items_ks_end ? a.items_ks_end[item] : (int) d.n_ks;")
     out.append("    const bool active = ks_raw < ks_last;")
     out.append("    const int ks = active ? ks_raw : ks_last - 1;")
-    out.append("    const int ksh = a.ks_perm ? a.ks_perm[ks] : ks;")
-    out.append("    const int kp0 = a.ks_prim_off[ksh], kp1 = a.ks_prim_off[ksh + 1];")
-    out.append("    const int pp0 = a.sp_pp_off[sp], pp1 = a.sp_pp_off[sp + 1];")
+    out.append("    const int ksh = d.ks_perm ? d.ks_perm[ks] : ks;")
+    out.append("    const int kp0 = d.ks_prim_off[ksh], kp1 = d.ks_prim_off[ksh + 1];")
+    out.append("    const int pp0 = a.items_pp ? a.items_pp[2 * item] : d.sp_pp_off[sp];")
+    out.append("    const int pp1 = a.items_pp ? a.items_pp[2 * item + 1] : d.sp_pp_off[sp + 1];")
     for ij in range(nij):
         for m in ms:
             out.append(f"    {ty} acc_{ij}_{m} = 0;")
@@ -234,12 +236,12 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64"):
     # E of one primitive pair is then loaded right where it is consumed, as scalar loads, instead of
     # being hoisted out of an inner loop and spilled from SGPRs into VGPR lanes.
     out.append("    for (int kp = kp0; kp < kp1; ++kp) {")
-    out.append("      const double Cx = a.ka_C[4 * kp + 0], Cy = a.ka_C[4 * kp + 1], Cz = a.ka_C[4 * kp + 2];")
-    out.append("      const double q = a.ka_C[4 * kp + 3];")
-    out.append("      const double ck = a.ka_coef[kp];")
+    out.append("      const double Cx = d.ka_C[4 * kp + 0], Cy = d.ka_C[4 * kp + 1], Cz = d.ka_C[4 * kp + 2];")
+    out.append("      const double q = d.ka_C[4 * kp + 3];")
+    out.append("      const double ck = d.ka_coef[kp];")
     out.append("      for (int pp = pp0; pp < pp1; ++pp) {")
-    out.append(f"        const {ty}* __restrict__ E = a.{'E32' if f32 else 'E'} + (size_t) pp * {nnz};")
-    out.append("        const double Px = a.pp_P[4 * pp + 0], Py = a.pp_P[4 * pp + 1], Pz = a.pp_P[4 * pp + 2], p = a.pp_P[4 * pp + 3];")
+    out.append(f"        const {ty}* __restrict__ E = d.{'E32' if f32 else 'E'} + (size_t) pp * {nnz};")
+    out.append("        const double Px = d.pp_P[4 * pp + 0], Py = d.pp_P[4 * pp + 1], Pz = d.pp_P[4 * pp + 2], p = d.pp_P[4 * pp + 3];")
     # geometry and prefactor in double always (cheap), then cast
     out.append("        const double dPQx = Px - Cx, dPQy = Py - Cy, dPQz = Pz - Cz;")
     out.append("        double zeta, pf;")
@@ -295,24 +297,27 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64"):
     out.append("      }")
     out.append("    }")
     out.append("    if (!active) return;")
-    out.append("    const int col = a.ks_col[ksh];")
-    out.append("    const int I = a.sp_i[sp], J = a.sp_j[sp];")
-    out.append("    double* __restrict__ o = a.out + (a.sp_out_base ? a.sp_out_base[sp] : 0LL);")
-    out.append("    const long long ld = a.sp_ld ? (long long) a.sp_ld[sp] : a.ld;")
-    out.append("    if (a.layout == 0) {")
-    for ij in range(nij):
-        mi, mj = divmod(ij, nj)
-        out.append(f"      if (I != J || {mi} >= {mj}) {{ const long long row = eip::packed_row(I + {mi}, J + {mj}) * ld + col;")
-        for m in ms:
-            out.append(f"        o[row + {m}] = (double) acc_{ij}_{m};")
-        out.append("      }")
-    out.append("    } else {")
-    out.append(f"      const long long base = (long long) sp * {nij};")
-    for ij in range(nij):
-        out.append(f"      {{ const long long row = (base + {ij}) * ld + col;")
-        for m in ms:
-            out.append(f"        o[row + {m}] = (double) acc_{ij}_{m};")
-        out.append("      }")
+    out.append("    const int col = d.ks_col[ksh];")
+    out.append("    const int I = d.sp_i[sp], J = d.sp_j[sp];")
+    out.append("    double* __restrict__ o = d.out + (d.sp_out_base ? d.sp_out_base[sp] : 0LL);")
+    out.append("    const long long ld = d.sp_ld ? (long long) d.sp_ld[sp] : d.ld;")
+    for op in ("=", "+="):
+        out.append("    if (a.accumulate == 0) {" if op == "=" else "    } else {")
+        out.append("     if (a.layout == 0) {")
+        for ij in range(nij):
+            mi, mj = divmod(ij, nj)
+            out.append(f"      if (I != J || {mi} >= {mj}) {{ const long long row = eip::packed_row(I + {mi}, J + {mj}) * ld + col;")
+            for m in ms:
+                out.append(f"        o[row + {m}] {op} (double) acc_{ij}_{m};")
+            out.append("      }")
+        out.append("     } else {")
+        out.append(f"      const long long base = (long long) sp * {nij};")
+        for ij in range(nij):
+            out.append(f"      {{ const long long row = (base + {ij}) * ld + col;")
+            for m in ms:
+                out.append(f"        o[row + {m}] {op} (double) acc_{ij}_{m};")
+            out.append("      }")
+        out.append("     }")
     out.append("    }")
     out.append("}")
     return name, "\n".join(out)

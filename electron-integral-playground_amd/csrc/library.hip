@@ -390,20 +390,20 @@ int three_center(int i_L, int j_L, int k_L, const double* pair_P_p, const double
             a.items_ks_end = nullptr;
             a.item_begin = 0;
             a.item_end = nb * kblocks;
-            a.sp_pp_off = d.off.as<int>();
-            a.sp_i = spi.as<int>();
-            a.sp_j = spj.as<int>();
-            a.pp_P = d.P.as<double>();
-            a.E = d.E.as<double>();
-            a.E32 = nullptr;
-            a.ks_perm = nullptr;
-            a.ks_prim_off = ko.as<int>();
-            a.ks_col = kcol.as<int>();
-            a.ka_C = kC.as<double>();
-            a.ka_coef = kc.as<double>();
-            a.n_ks = n_ks;
-            a.out = out.as<double>() - (long long) sp0 * nrow * ld;
-            a.ld = ld;
+            a.d.sp_pp_off = d.off.as<int>();
+            a.d.sp_i = spi.as<int>();
+            a.d.sp_j = spj.as<int>();
+            a.d.pp_P = d.P.as<double>();
+            a.d.E = d.E.as<double>();
+            a.d.E32 = nullptr;
+            a.d.ks_perm = nullptr;
+            a.d.ks_prim_off = ko.as<int>();
+            a.d.ks_col = kcol.as<int>();
+            a.d.ka_C = kC.as<double>();
+            a.d.ka_coef = kc.as<double>();
+            a.d.n_ks = n_ks;
+            a.d.out = out.as<double>() - (long long) sp0 * nrow * ld;
+            a.d.ld = ld;
             a.layout = 1;
             a.omega = omega;
             const int v = current_variant(i_L, j_L, k_L, 0);
@@ -500,8 +500,14 @@ int eip_three_center_items(int i_L, int j_L, int k_L, int use_f32, const int* d_
     const int nchunks = eip_three_center_nchunks(i_L, j_L, k_L);
     if (nchunks < 0) return 1;
     if (n_items <= 0) return 0;
-    J3cArgs a{d_items, d_items_ks_end, 0, n_items, d_sp_pp_off, d_sp_i, d_sp_j, d_pp_P, d_E, d_E32,
-              d_ks_perm, d_ks_prim_off, d_ks_col, d_ka_C, d_ka_coef, n_ks, d_out, ld, layout, omega};
+    J3cArgs a{};
+    a.items = d_items;
+    a.items_ks_end = d_items_ks_end;
+    a.item_end = n_items;
+    a.d = J3cData{d_sp_pp_off, d_sp_i, d_sp_j, d_pp_P, d_E, d_E32, d_ks_perm, d_ks_prim_off, d_ks_col, d_ka_C,
+                  d_ka_coef, n_ks, d_out, ld, nullptr, nullptr};
+    a.layout = layout;
+    a.omega = omega;
     const int v = current_variant(i_L, j_L, k_L, use_f32);
     for (int c = 0; c < eip_j3c_nchunks(i_L, j_L, k_L, v); ++c)
         if (eip_launch_j3c(i_L, j_L, k_L, v, c, use_f32, a, n_items, (hipStream_t) stream) != 0) return 3;
@@ -518,9 +524,38 @@ int eip_three_center_items_batched(int i_L, int j_L, int k_L, int use_f32, const
     const int nchunks = eip_three_center_nchunks(i_L, j_L, k_L);
     if (nchunks < 0) return 1;
     if (n_items <= 0) return 0;
-    J3cArgs a{d_items, d_items_ks_end, 0, n_items, d_sp_pp_off, d_sp_i, d_sp_j, d_pp_P, d_E, d_E32,
-              d_ks_perm, d_ks_prim_off, d_ks_col, d_ka_C, d_ka_coef, n_ks, d_out, 0, 0, omega,
-              d_sp_out_base, d_sp_ld};
+    J3cArgs a{};
+    a.items = d_items;
+    a.items_ks_end = d_items_ks_end;
+    a.item_end = n_items;
+    a.d = J3cData{d_sp_pp_off, d_sp_i, d_sp_j, d_pp_P, d_E, d_E32, d_ks_perm, d_ks_prim_off, d_ks_col, d_ka_C,
+                  d_ka_coef, n_ks, d_out, 0, d_sp_out_base, d_sp_ld};
+    a.layout = 0;
+    a.omega = omega;
+    const int v = current_variant(i_L, j_L, k_L, use_f32);
+    for (int c = 0; c < eip_j3c_nchunks(i_L, j_L, k_L, v); ++c)
+        if (eip_launch_j3c(i_L, j_L, k_L, v, c, use_f32, a, n_items, (hipStream_t) stream) != 0) return 3;
+    return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int eip_three_center_work(int i_L, int j_L, int k_L, int use_f32, const int* d_items, const int* d_items_ks_end,
+                          const int* d_items_pp, const int* d_items_job, int n_items, const void* d_jobs, int layout,
+                          int accumulate, double omega, void* stream)
+{
+    const int nchunks = eip_three_center_nchunks(i_L, j_L, k_L);
+    if (nchunks < 0) return 1;
+    if (n_items <= 0) return 0;
+    if (!d_items || !d_jobs || (layout != 0 && layout != 1) || (accumulate != 0 && accumulate != 1)) return 4;
+    J3cArgs a{};
+    a.items = d_items;
+    a.items_ks_end = d_items_ks_end;
+    a.items_pp = d_items_pp;
+    a.items_job = d_items_job;
+    a.item_end = n_items;
+    a.jobs = static_cast<const J3cData*>(d_jobs);
+    a.layout = layout;
+    a.accumulate = accumulate;
+    a.omega = omega;
     const int v = current_variant(i_L, j_L, k_L, use_f32);
     for (int c = 0; c < eip_j3c_nchunks(i_L, j_L, k_L, v); ++c)
         if (eip_launch_j3c(i_L, j_L, k_L, v, c, use_f32, a, n_items, (hipStream_t) stream) != 0) return 3;

@@ -17,13 +17,9 @@ struct PairPrepArgs {
     int recompute_P;       // 1: P and p from A, B (overlap)
 };
 
-// Three-centre (ij|P) for one (la, lb, lc) class over a list of work items.
-// Work item w = (sp, ks_first): the wave handles shell pair sp and auxiliary shells
-// ks_first .. ks_first + 63 (clipped at items_ks_end[w] or n_ks) of the (optionally permuted) list.
-struct J3cArgs {
-    const int* items;         // [n_items][2]
-    const int* items_ks_end;  // [n_items] or nullptr (then n_ks)
-    int item_begin, item_end;
+// One molecule's (or job's) data for the three-centre kernels.  Every field is 8 bytes wide, so a
+// table of these can be filled from any host language as 16 64-bit words.
+struct J3cData {
     // shell pairs
     const int* sp_pp_off;     // [n_sp + 1] CSR offsets into the primitive-pair arrays
     const int* sp_i;          // [n_sp] AO offset of shell i
@@ -37,16 +33,33 @@ struct J3cArgs {
     const int* ks_col;        // [n_ks] output column of the shell's first function
     const double* ka_C;       // [n_kprim][4]
     const double* ka_coef;    // [n_kprim]
-    int n_ks;
+    long long n_ks;
     // output
     double* out;
     long long ld;             // row stride (in doubles)
-    int layout;               // 0: rows = packed lower triangle (i >= j); 1: rows = sp * nij + ij
-    double omega;
-    // optional, per shell pair (batches of molecules with one output each): offset of the
-    // molecule's output in 'out' and its row stride.  nullptr: 0 and 'ld'.
+    // optional, per shell pair (batches of molecules sharing one buffer): offset of the molecule's
+    // output in 'out' and its row stride.  nullptr: 0 and 'ld'.
     const long long* sp_out_base;
     const int* sp_ld;
+};
+static_assert(sizeof(J3cData) == 16 * 8, "J3cData is a table of 16 64-bit words");
+
+// Three-centre (ij|P) for one (la, lb, lc) class over a list of work items.
+// Work item w = (sp, ks_first): the wave handles shell pair sp and auxiliary shells
+// ks_first .. ks_first + 63 (clipped at items_ks_end[w] or n_ks) of the (optionally permuted) list,
+// over the primitive pairs items_pp[w] = [pp_begin, pp_end) (or all of the shell pair's), of the data
+// jobs[items_job[w]] (or 'd').
+struct J3cArgs {
+    const int* items;         // [n_items][2]
+    const int* items_ks_end;  // [n_items] or nullptr (then n_ks)
+    const int* items_pp;      // [n_items][2] or nullptr (then sp_pp_off[sp] .. sp_pp_off[sp + 1])
+    const int* items_job;     // [n_items] or nullptr (then job 0)
+    int item_begin, item_end;
+    J3cData d;                // used when jobs is nullptr
+    const J3cData* jobs;      // device table of per-job data, or nullptr
+    int layout;               // 0: rows = packed lower triangle (i >= j); 1: rows = sp * nij + ij
+    int accumulate;           // 0: out = integral; 1: out += integral
+    double omega;
 };
 
 // Nuclear attraction: lanes over point charges, output [(sp * nij + ij) * ld + charge].

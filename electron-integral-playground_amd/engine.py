@@ -87,9 +87,15 @@ def _ranges(counts, first):
     return owner.astype(np.int32), (np.repeat(np.asarray(first, dtype=np.int64), counts) + 64 * b).astype(np.int32)
 
 
+# Primitive pairs with p = a + b below this exponent are "diffuse": the mixed-precision split always
+# evaluates them in float64.
+DIFFUSE_PAIR_EXPONENT = 0.2
+
+
 def group_pairs(pd):
-    """Stable grouping of one class's primitive pairs by (i_ao_start, j_ao_start)."""
-    order = np.lexsort((pd.j_ao_start, pd.i_ao_start))
+    """Stable grouping of one class's primitive pairs by (i_ao_start, j_ao_start); within a shell pair
+    the diffuse pairs (p < DIFFUSE_PAIR_EXPONENT) come first, each part in list order (descending bound)."""
+    order = np.lexsort((pd.P_p[:, 3] >= DIFFUSE_PAIR_EXPONENT, pd.j_ao_start, pd.i_ao_start))
     ki, kj = pd.i_ao_start[order], pd.j_ao_start[order]
     new = np.ones(len(order), dtype=bool)
     new[1:] = (ki[1:] != ki[:-1]) | (kj[1:] != kj[:-1])
@@ -105,6 +111,39 @@ def group_aux(pk):
     new[1:] = k[1:] != k[:-1]
     starts = np.flatnonzero(new)
     return order, k[starts].astype(np.int32), np.append(starts, len(order)).astype(np.int32)
+
+
+def mixed_split(sp_pp_off, p, ub, q, ks_prim_off, threshold):
+    """Float64/float32 split of one (pair class, auxiliary class); host logic of setup_mixed.
+
+    sp_pp_off: CSR offsets of the shell pairs' primitive pairs (diffuse pairs first, then descending
+    ub, as group_pairs orders them); p, ub: exponent sum and reference bound per primitive pair;
+    q: Q_P per auxiliary shell; ks_prim_off: CSR primitive offsets of the auxiliary shells.
+    Returns (s, perm, gprim): s[sp, g] = number of leading primitive pairs of shell pair sp evaluated in
+    float64 for auxiliary group g (shells perm[64 g : 64 g + 64], perm = descending Q_P), and gprim[g] =
+    auxiliary primitives in group g.  A primitive pair is float64 for group g when it is diffuse or
+    ub * max_{P in g} Q_P >= threshold.
+    """
+    sp_pp_off = np.asarray(sp_pp_off, dtype=np.int64)
+    n_sp = len(sp_pp_off) - 1
+    q = np.asarray(q, dtype=np.float64)
+    perm = np.argsort(-q, kind="stable").astype(np.int32)
+    G = (len(q) + 63) // 64
+    qmax = q[perm[np.arange(G) * 64]]
+    kprim = np.diff(np.asarray(ks_prim_off, dtype=np.int64))[perm]
+    gprim = np.add.reduceat(kprim, np.arange(G) * 64)
+    n_pp_sp = np.diff(sp_pp_off)
+    sp_of_pp = np.repeat(np.arange(n_sp, dtype=np.int64), n_pp_sp)
+    diffuse = np.asarray(p) < DIFFUSE_PAIR_EXPONENT
+    n_diff = np.bincount(sp_of_pp[diffuse], minlength=n_sp)
+    with np.errstate(divide="ignore"):
+        need = threshold / np.asarray(ub, dtype=np.float64)
+    # g_min: the pair is float64 for the groups g < g_min (qmax descends, so qmax_g >= need there)
+    g_min = np.searchsorted(-qmax, -need, side="right")
+    nd = ~diffuse
+    hist = np.bincount(sp_of_pp[nd] * (G + 1) + g_min[nd], minlength=n_sp * (G + 1)).reshape(n_sp, G + 1)
+    at_least = np.cumsum(hist[:, ::-1], axis=1)[:, ::-1]        # at_least[:, h] = #(g_min >= h)
+    return n_diff[:, None] + at_least[:, 1:], perm, gprim
 
 
 class ThreeCenterPlan:
@@ -175,6 +214,7 @@ class ThreeCenterPlan:
             hi, lo = np.maximum(i, j), np.minimum(i, j)
             written[(hi * (hi + 1) // 2 + lo).ravel()] = True
         self.zero_rows = T(np.flatnonzero(~written).astype(np.int32), torch.int32)
+        self._job_tables = {}
         torch.cuda.current_stream().synchronize()
 
     # -- sizes -----------------------------------------------------------------------------------
@@ -245,73 +285,81 @@ class ThreeCenterPlan:
                     raise RuntimeError(f"eip_two_center_device({a.lc},{b.lc}) failed with {rc}")
         return out
 
-    # -- mixed precision ---------------------------------------------------------------------------
-    def setup_mixed(self, pair_list, q_aux, threshold=1e-6, min_pair_exponent=0.2):
-        """Split every (shell pair, auxiliary shell) triplet between the float64 and float32 kernels.
+    # -- generic work launches (job table) ---------------------------------------------------------
+    def job_words(self, pi, ai, out_ptr, ld, perm=None):
+        """The 16 words of one eip_j3c_job entry: pair class pi, auxiliary class ai, output."""
+        dp, da = self.pairs[pi].dev, self.aux[ai].dev
+        return [dp["off"].data_ptr(), dp["si"].data_ptr(), dp["sj"].data_ptr(), dp["P"].data_ptr(),
+                dp["E"].data_ptr(), dp["E32"].data_ptr(), perm.data_ptr() if perm is not None else 0,
+                da["off"].data_ptr(), da["col"].data_ptr(), da["C"].data_ptr(), da["c"].data_ptr(),
+                self.aux[ai].n_ks, int(out_ptr), int(ld), 0, 0]
 
-        Bound: |(ij|P)| <= Q_ij Q_P, with Q_P = sqrt(max_m (P_m|P_m)) from the FP64 metric (``q_aux``:
-        one array per auxiliary class, in shell order) and Q_ij the sum over the shell pair's primitive
-        pairs of the reference's Schwarz-type bound (pair_utility.py:63-71).  Triplets with
-        Q_ij Q_P < threshold run in float32 (Boys, R and accumulation); shell pairs with a primitive
-        exponent p < min_pair_exponent (diffuse pairs) always stay float64.  Auxiliary shells are sorted
-        by Q_P, so each shell pair's float64 set is a prefix of that order.
+    def job_table(self, entries):
+        """Device table (int64 [n][16]) of eip_j3c_job entries; kept (and reused) for the plan's lifetime,
+        so a table is never freed while launches that read it may be in flight."""
+        key = tuple(int(w) for e in entries for w in e)
+        t = self._job_tables.get(key)
+        if t is None:
+            t = self.torch.tensor(np.array(key, dtype=np.int64).reshape(-1, 16), dtype=self.torch.int64,
+                                  device=self.device)
+            self._job_tables[key] = t
+        return t
+
+    def launch_work(self, pi, ai, items, jobs, job=None, pp=None, ks_end=None, layout=0, accumulate=0,
+                    use_f32=False, stream=None):
+        if items is None or items.shape[0] == 0:
+            return
+        pc, ac = self.pairs[pi], self.aux[ai]
+        rc = self.lib.eip_three_center_work(
+            pc.la, pc.lb, ac.lc, int(use_f32), _ptr(items), _ptr(ks_end), _ptr(pp), _ptr(job), int(items.shape[0]),
+            _ptr(jobs), int(layout), int(accumulate), _c_double(self.omega), _stream_handle(stream))
+        if rc != 0:
+            raise RuntimeError(f"eip_three_center_work({pc.la},{pc.lb},{ac.lc}, f32={use_f32}) failed with {rc}")
+
+    # -- mixed precision ---------------------------------------------------------------------------
+    def setup_mixed(self, pair_list, q_aux, threshold=1e-6):
+        """Split every primitive triplet between the float64 and float32 kernels.
+
+        Bound of a triplet (primitive pair pp, auxiliary shell P): ub_pp Q_P, with ub_pp the reference's
+        Schwarz-type bound of the primitive pair (pair_utility.py:95-114) and Q_P = sqrt(max_m (P_m|P_m))
+        from the float64 metric (``q_aux``: one array per auxiliary class, in shell order).  Triplets
+        with ub_pp Q_P < threshold run in float32 (Boys, R and accumulation); pairs with p <
+        DIFFUSE_PAIR_EXPONENT always run in float64.  Auxiliary shells are ordered by descending Q_P and
+        taken 64 at a time; for a (shell pair, 64 auxiliary shells) item the bound uses the largest Q_P
+        of the 64.  Within a shell pair the diffuse pairs come first and the rest by descending ub, so
+        the float64 pairs of an item are a prefix [pp0, pp0 + s) and the float32 pairs the suffix: the
+        float64 launch stores its prefix sums (zeros when s = 0), the float32 launch adds the rest.
+        ``pair_list``: per class ``PairArrays`` or ``list[PrimitivePair]`` (the list the pair data was
+        packed from).  Returns triplet counts.
         """
+        from .pair_utility import _as_arrays
         torch = self.torch
         T = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=self.device)
         self.mixed = {}
-        self.mixed_stats = {"f64_triplets": 0, "f32_triplets": 0}
+        self.mixed_stats = {"f64_triplets": 0, "f32_triplets": 0, "threshold": threshold}
         for pi, pc in enumerate(self.pairs):
-            key = (pc.la, pc.lb)
-            ub = np.array([p.upper_bound for p in pair_list[key]])
-            # the pair list is in the pair data's order; regroup like the pair data
-            pd_order = self._pair_orders[pi]
-            ub = ub[pd_order]
-            q_sp = np.add.reduceat(ub, pc.sp_pp_off[:-1])
-            pmin = np.minimum.reduceat(self._pair_p[pi], pc.sp_pp_off[:-1])
-            q_sp = np.where(pmin < min_pair_exponent, np.inf, q_sp)
+            ub = _as_arrays(pair_list[(pc.la, pc.lb)]).upper_bound[self._pair_orders[pi]]
             for ai, ac in enumerate(self.aux):
-                q = np.asarray(q_aux[ai])
-                perm = np.argsort(-q, kind="stable").astype(np.int32)
-                qs = q[perm]
-                # number of aux shells with q_sp * q >= threshold (q sorted descending)
-                with np.errstate(divide="ignore", invalid="ignore"):
-                    need = np.where(q_sp > 0, threshold / q_sp, np.inf)
-                split = np.searchsorted(-qs, -need, side="right").astype(np.int64)
-                split = np.where(np.isinf(q_sp), ac.n_ks, split)
-                n64 = (split + 63) // 64
-                n32 = (ac.n_ks - split + 63) // 64
-                sp64, k064 = _ranges(n64, np.zeros_like(split))
-                sp32, k032 = _ranges(n32, split)
-                n_prim_sp = np.diff(pc.sp_pp_off).astype(np.int64)
-                kprim = np.diff(ac.ks_prim_off)[perm]
-                cum = np.concatenate([[0], np.cumsum(kprim)])
-                f64 = int(np.sum(n_prim_sp * cum[np.minimum(split, ac.n_ks)]))
-                self.mixed_stats["f64_triplets"] += f64
-                self.mixed_stats["f32_triplets"] += int(np.sum(n_prim_sp) * cum[-1]) - f64
-
-                def items(sp, k0, end):
-                    if len(sp) == 0:
-                        return None, None
-                    return (T(np.stack([sp, k0], axis=1), torch.int32), T(end, torch.int32))
-
-                it64, end64 = items(sp64, k064, split[sp64])
-                it32, end32 = items(sp32, k032, np.full(len(sp32), ac.n_ks))
-                self.mixed[(pi, ai)] = dict(perm=T(perm, torch.int32), it64=it64, end64=end64,
-                                            it32=it32, end32=end32)
+                split_rows, perm, gprim = mixed_split(pc.sp_pp_off, self._pair_p[pi], ub, q_aux[ai],
+                                                      ac.ks_prim_off, threshold)
+                G = len(gprim)
+                n_pp_sp = np.diff(pc.sp_pp_off).astype(np.int64)
+                sp = np.repeat(np.arange(pc.n_sp, dtype=np.int32), G)
+                k0 = np.tile(np.arange(G, dtype=np.int32) * 64, pc.n_sp)
+                pp0 = np.repeat(pc.sp_pp_off[:-1].astype(np.int64), G)
+                pp1 = np.repeat(pc.sp_pp_off[1:].astype(np.int64), G)
+                split = pp0 + split_rows.ravel()
+                f32 = split < pp1
+                f64_trip = int(np.sum(split_rows * gprim[None, :]))
+                self.mixed_stats["f64_triplets"] += f64_trip
+                self.mixed_stats["f32_triplets"] += int(np.sum(n_pp_sp) * np.sum(gprim)) - f64_trip
+                items = np.stack([sp, k0], axis=1)
+                self.mixed[(pi, ai)] = dict(
+                    perm=T(perm, torch.int32),
+                    it64=T(items, torch.int32), pp64=T(np.stack([pp0, split], axis=1), torch.int32),
+                    it32=T(items[f32], torch.int32), pp32=T(np.stack([split[f32], pp1[f32]], axis=1), torch.int32),
+                    sp32=sp[f32], G=G)
         return self.mixed_stats
-
-    def _launch_items(self, pi, ai, out, items, ends, perm, use_f32, stream=None):
-        if items is None:
-            return
-        pc, ac = self.pairs[pi], self.aux[ai]
-        dp, da = pc.dev, ac.dev
-        rc = self.lib.eip_three_center_items(
-            pc.la, pc.lb, ac.lc, int(use_f32), _ptr(items), _ptr(ends), items.shape[0], _ptr(dp["off"]),
-            _ptr(dp["si"]), _ptr(dp["sj"]), _ptr(dp["P"]), _ptr(dp["E"]), _ptr(dp["E32"]), _ptr(perm),
-            _ptr(da["off"]), _ptr(da["col"]), _ptr(da["C"]), _ptr(da["c"]), ac.n_ks, _ptr(out),
-            _c_ll(self.n_aux), 0, _c_double(self.omega), _stream_handle(stream))
-        if rc != 0:
-            raise RuntimeError(f"eip_three_center_items({pc.la},{pc.lb},{ac.lc}, f32={use_f32}) failed with {rc}")
 
     def run_mixed(self, out=None, stream=None, zero=True):
         """Mixed-precision (ij|P) into the packed float64 output (requires ``setup_mixed``)."""
@@ -319,9 +367,53 @@ class ThreeCenterPlan:
             out = self.new_output()
         if zero:
             self.zero_unwritten(out, stream)
-        for (pi, ai), m in self.mixed.items():
-            self._launch_items(pi, ai, out, m["it64"], m["end64"], m["perm"], False, stream)
-            self._launch_items(pi, ai, out, m["it32"], m["end32"], m["perm"], True, stream)
+        keys = list(self.mixed)
+        jobs = self.job_table([self.job_words(pi, ai, out.data_ptr(), self.n_aux, self.mixed[(pi, ai)]["perm"])
+                               for pi, ai in keys])
+        for j, (pi, ai) in enumerate(keys):
+            m = self.mixed[(pi, ai)]
+            self.launch_work(pi, ai, m["it64"], jobs[j], pp=m["pp64"], layout=0, accumulate=0, stream=stream)
+            self.launch_work(pi, ai, m["it32"], jobs[j], pp=m["pp32"], layout=0, accumulate=1, use_f32=True,
+                             stream=stream)
+        return out
+
+    # -- streaming by shell-pair blocks (outputs larger than device memory) ---------------------------
+    def row_blocks(self, max_bytes):
+        """Blocks (pi, sp0, sp1) of consecutive shell pairs of one class whose compact output
+        ((sp1 - sp0) * nij rows of n_aux float64) fits in max_bytes."""
+        blocks = []
+        for pi, pc in enumerate(self.pairs):
+            nij = (2 * pc.la + 1) * (2 * pc.lb + 1)
+            per = max(1, int(max_bytes // (nij * self.n_aux * 8)))
+            for sp0 in range(0, pc.n_sp, per):
+                blocks.append((pi, sp0, min(pc.n_sp, sp0 + per)))
+        return blocks
+
+    def block_rows(self, pi, sp0, sp1):
+        pc = self.pairs[pi]
+        return (sp1 - sp0) * (2 * pc.la + 1) * (2 * pc.lb + 1)
+
+    def run_block(self, pi, sp0, sp1, out, mixed=False, stream=None):
+        """(ij|P) of shell pairs [sp0, sp1) of class pi, all auxiliary shells, into the compact block
+        out[(sp - sp0) * nij + ij, P] (every row written; no zeroing needed)."""
+        pc = self.pairs[pi]
+        nij = (2 * pc.la + 1) * (2 * pc.lb + 1)
+        base = out.data_ptr() - 8 * sp0 * nij * self.n_aux
+        entries = [self.job_words(pi, ai, base, self.n_aux, self.mixed[(pi, ai)]["perm"] if mixed else None)
+                   for ai in range(len(self.aux))]
+        jobs = self.job_table(entries)
+        for ai, ac in enumerate(self.aux):
+            if not mixed:
+                G = (ac.n_ks + 63) // 64
+                self.launch_work(pi, ai, self.items[(pi, ai)][sp0 * G:sp1 * G], jobs[ai], layout=1, stream=stream)
+                continue
+            m = self.mixed[(pi, ai)]
+            G = m["G"]
+            self.launch_work(pi, ai, m["it64"][sp0 * G:sp1 * G], jobs[ai], pp=m["pp64"][sp0 * G:sp1 * G], layout=1,
+                             accumulate=0, stream=stream)
+            lo, hi = np.searchsorted(m["sp32"], [sp0, sp1])
+            self.launch_work(pi, ai, m["it32"][lo:hi], jobs[ai], pp=m["pp32"][lo:hi], layout=1, accumulate=1,
+                             use_f32=True, stream=stream)
         return out
 
 

@@ -107,6 +107,40 @@ int eip_three_center_items_batched(int i_L, int j_L, int k_L, int use_f32, const
                                    const int* d_ks_prim_off, const int* d_ks_col, const double* d_ka_C,
                                    const double* d_ka_coef, int n_ks, double* d_out, double omega, void* stream);
 
+/* One molecule's (or job's) three-centre data, as read by eip_three_center_work: 16 64-bit words.
+ * Shell pairs: CSR primitive-pair offsets, AO offsets of shells i and j, primitive pairs P_p (n,4),
+ * pair matrices (float64 and optional float32); auxiliary shells: optional position->shell permutation,
+ * CSR primitive offsets, output column, primitives C_q (n,4) and coefficients; n_ks; output pointer and
+ * row stride; optional per-shell-pair output offset and row stride (batches sharing one buffer). */
+typedef struct {
+    const int* sp_pp_off;
+    const int* sp_i;
+    const int* sp_j;
+    const double* pp_P;
+    const double* E;
+    const float* E32;
+    const int* ks_perm;
+    const int* ks_prim_off;
+    const int* ks_col;
+    const double* ka_C;
+    const double* ka_coef;
+    long long n_ks;
+    double* out;
+    long long ld;
+    const long long* sp_out_base;
+    const int* sp_ld;
+} eip_j3c_job;
+
+/* (ij|P) of one class over work items that may belong to different molecules (jobs): item w =
+ * (shell pair sp, first auxiliary shell k0) of job d_items_job[w] (or job 0 when null) in the device table
+ * d_jobs (eip_j3c_job[]), over auxiliary shells k0 .. min(k0 + 64, d_items_ks_end[w] or n_ks) and primitive
+ * pairs d_items_pp[w] = [begin, end) (or all of the shell pair's when null).  accumulate = 0 stores,
+ * accumulate = 1 adds into the output (the float32 part of a mixed-precision split is added on top of the
+ * float64 part).  Layouts as eip_three_center_items.  Returns 4 on bad arguments. */
+int eip_three_center_work(int i_L, int j_L, int k_L, int use_f32, const int* d_items, const int* d_items_ks_end,
+                          const int* d_items_pp, const int* d_items_job, int n_items, const void* d_jobs, int layout,
+                          int accumulate, double omega, void* stream);
+
 /* Zero rows d_rows[0..n_rows) (each ld doubles long) of d_out: the packed rows of screened-out shell
  * pairs, which no three-centre kernel writes. */
 int eip_zero_rows(const int* d_rows, int n_rows, long long ld, double* d_out, void* stream);

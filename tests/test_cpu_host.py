@@ -193,3 +193,37 @@ def test_workload_sizes():
     g = mol.geometry / generators.angstrom_to_bohr
     d = np.linalg.norm(g[:, None] - g[None], axis=-1) + np.eye(len(g)) * 10
     assert d.min() >= 1.0
+
+
+def test_mixed_split_matches_brute_force():
+    """Host logic of the mixed-precision split: for every (shell pair, group of 64 auxiliary shells) the
+    float64 primitive pairs are exactly a prefix, and that prefix is the set a direct test selects."""
+    from electron_integral_playground_amd.engine import DIFFUSE_PAIR_EXPONENT, mixed_split
+    rng = np.random.default_rng(7)
+    n_sp = 40
+    counts = rng.integers(1, 9, n_sp)
+    off = np.concatenate([[0], np.cumsum(counts)])
+    p_parts, ub_parts = [], []
+    for c in counts:
+        p = np.exp(rng.uniform(np.log(0.05), np.log(50.0), c))
+        ub = 10.0 ** rng.uniform(-13, 1, c)
+        order = np.lexsort((-ub, p >= DIFFUSE_PAIR_EXPONENT))    # diffuse first, then descending ub
+        p_parts.append(p[order]); ub_parts.append(ub[order])
+    p, ub = np.concatenate(p_parts), np.concatenate(ub_parts)
+    n_ks = 150
+    q = 10.0 ** rng.uniform(-3, 1, n_ks)
+    ks_prim_off = np.concatenate([[0], np.cumsum(rng.integers(1, 3, n_ks))])
+    for thr in (1e-6, 1e-9):
+        s, perm, gprim = mixed_split(off, p, ub, q, ks_prim_off, thr)
+        assert sorted(perm.tolist()) == list(range(n_ks)) and np.all(np.diff(q[perm]) <= 0)
+        G = (n_ks + 63) // 64
+        assert s.shape == (n_sp, G)
+        for g in range(G):
+            shells = perm[64 * g:64 * g + 64]
+            qmax = q[shells].max()
+            assert gprim[g] == np.sum(np.diff(ks_prim_off)[shells])
+            for sp in range(n_sp):
+                sel = slice(off[sp], off[sp + 1])
+                f64 = (p[sel] < DIFFUSE_PAIR_EXPONENT) | (ub[sel] * qmax >= thr)
+                k = s[sp, g]
+                assert f64[:k].all() and not f64[k:].any(), (thr, g, sp)

@@ -110,3 +110,70 @@ def test_batch_plan_matches_single_molecule_plans():
             single = ThreeCenterPlan(mol, pd, ad).run()
             torch.cuda.synchronize()
             assert torch.equal(batch.molecule_output(out, m), single), (replay, m)
+
+
+def _block_plan(n_atom=10, box=8.0):
+    from electron_integral_playground_amd.engine import ThreeCenterPlan
+    mol = generators.make_molecule("mixed", n_atom=n_atom, box=box)
+    arrays = pair_utility.form_pair_arrays(mol, 1e-14)
+    pairs = pair_utility.form_primitive_pair_data(mol, arrays)
+    aux = pair_utility.form_auxiliary_pair_data(mol)
+    return mol, arrays, ThreeCenterPlan(mol, pairs, aux)
+
+
+def test_row_blocks_match_packed_output():
+    """Streaming by shell-pair blocks (compact rows, job-table launches) gives the packed path's values
+    bit for bit (up to the i <-> j symmetry inside diagonal shell pairs), for blocks small enough that
+    every class is split."""
+    import torch
+    mol, _, plan = _block_plan()
+    ref = plan.run()
+    blocks = plan.row_blocks(3 * 25 * mol.n_aux * 8)
+    assert len(blocks) > len(plan.pairs)
+    buf = torch.full((max(plan.block_rows(*b) for b in blocks), mol.n_aux), float("nan"), dtype=torch.float64,
+                     device="cuda")
+    for pi, sp0, sp1 in blocks:
+        print(f"case block pi={pi} sp=[{sp0},{sp1})", flush=True)
+        pc = plan.pairs[pi]
+        plan.run_block(pi, sp0, sp1, buf)
+        torch.cuda.synchronize()
+        ni, nj = 2 * pc.la + 1, 2 * pc.lb + 1
+        for sp in range(sp0, sp1):
+            I, J = int(pc.sp_i[sp]), int(pc.sp_j[sp])
+            for a in range(ni):
+                for b in range(nj):
+                    hi, lo = max(I + a, J + b), min(I + a, J + b)
+                    row = (sp - sp0) * ni * nj + a * nj + b
+                    want = ref[hi * (hi + 1) // 2 + lo]
+                    if I == J and a < b:
+                        # the packed path keeps the (b, a) element of a diagonal shell pair; (a, b) is
+                        # the same integral through the transposed pair matrix entry
+                        assert torch.allclose(buf[row], want, rtol=RTOL, atol=ATOL), (pi, sp, a, b)
+                    else:
+                        assert torch.equal(buf[row], want), (pi, sp, a, b)
+
+
+def test_row_blocks_mixed_within_bound():
+    """The mixed split (float64 prefix stored, float32 suffix added) per block stays within 1e-10 of
+    float64, and a threshold of 0 (everything float64) reproduces float64 exactly."""
+    import torch
+    mol, arrays, plan = _block_plan()
+    q = _q_aux_from_metric(plan, plan.two_center())
+    blocks = plan.row_blocks(4e9)
+    rows = max(plan.block_rows(*b) for b in blocks)
+    A = torch.empty((rows, mol.n_aux), dtype=torch.float64, device="cuda")
+    B = torch.empty_like(A)
+    for thr, bound in ((1e-6, 1e-10), (0.0, 0.0)):
+        stats = plan.setup_mixed(arrays, q, threshold=thr)
+        print(f"case threshold={thr} split={stats}", flush=True)
+        if thr > 0:
+            assert stats["f32_triplets"] > 0
+        else:
+            assert stats["f32_triplets"] == 0
+        for b in blocks:
+            n = plan.block_rows(*b)
+            plan.run_block(*b, A)
+            plan.run_block(*b, B, mixed=True)
+            torch.cuda.synchronize()
+            err = (B[:n] - A[:n]).abs().max().item()
+            assert err <= bound, (thr, b, err)
