@@ -45,7 +45,7 @@ def _compile(src):
         raise RuntimeError(f"hipcc failed on {src.name}:\n{r.stderr[-4000:]}")
     res, name = {}, None
     keys = {"VGPRs:": "vgpr", "AGPRs:": "agpr", "ScratchSize [bytes/lane]:": "scratch",
-            "Occupancy [waves/SIMD]:": "occupancy"}
+            "Occupancy [waves/SIMD]:": "occupancy", "VGPRs Spill:": "vgpr_spill", "SGPRs Spill:": "sgpr_spill"}
     for line in r.stderr.splitlines():
         if "Function Name:" in line:
             name = line.split("Function Name:")[1].split()[0]
@@ -57,8 +57,14 @@ def _compile(src):
     return obj, res
 
 
+def spills(res):
+    """A kernel spills when it moves VGPRs to scratch memory.  (SGPR spills go to VGPR lanes, and a few
+    dozen bytes of private segment hold a few SGPR spills or the backend's reserved scavenging slots.)"""
+    return res.get("vgpr_spill", 0) > 0 or res.get("scratch", 0) > 128
+
+
 def _spilling_classes(scratch, shrink):
-    """Three-centre classes (la, lb, lc) whose default chunk plan has a kernel using scratch memory."""
+    """Three-centre classes (la, lb, lc) whose default chunk plan has a kernel spilling to scratch memory."""
     sys.path.insert(0, str(PKG))
     try:
         from codegen.gen_kernels import m_chunks
@@ -66,7 +72,8 @@ def _spilling_classes(scratch, shrink):
         sys.path.pop(0)
     out = set()
     for name, nbytes in scratch.items():
-        if nbytes > 0 and name.startswith("j3c_"):
+        # the job-table twins ("_jobs") share their body with the kernarg kernels; the kernarg kernels decide
+        if nbytes > 0 and name.startswith("j3c_") and not name.endswith("_jobs"):
             parts = name.split("_")
             cls = parts[1]
             la, lb, lc = int(cls[0]), int(cls[1]), int(cls[2])
@@ -99,7 +106,7 @@ def build(jobs=None, verbose=False):
                 results[src] = res
         objs = [o for o, _ in results.values()]
         resources = {k: v for _, sc in results.values() for k, v in sc.items()}
-        scratch = {k: v.get("scratch", 0) for k, v in resources.items()}
+        scratch = {k: (v.get("scratch", 0) if spills(v) else 0) for k, v in resources.items()}
         spilling = {c for c in _spilling_classes(scratch, shrink) if shrink.get(c, 0) < 3}
         if not spilling or attempt == 3:
             break

@@ -213,22 +213,26 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64"):
             for s in aux_rows[m][1]:
                 needed.add((t[0] + s[0], t[1] + s[1], t[2] + s[2]))
     out = [f"// ({la}{lb}|{lc}) aux components [{m0},{m1}) {precision}: L = {L}, {nnz} pair entries"]
-    out.append(f'extern "C" __global__ void __launch_bounds__(64) {name}(const J3cArgs a)')
+    # One body, two kernels: '{name}' reads the molecule's data from its kernel arguments (rematerialised
+    # scalar loads from the kernarg segment), '{name}_jobs' from the per-item job table.
+    out.append("template <bool kJobs>")
+    out.append(f"__device__ __forceinline__ void {name}_body(const J3cArgs a)")
     out.append("{")
     out.append("    const int lb_ = eip::xcd_remap(blockIdx.x, gridDim.x);")
     out.append("    const int item = a.item_begin + lb_;")
     out.append("    if (item >= a.item_end) return;")
-    out.append("    const J3cData d = a.jobs ? a.jobs[a.items_job ? a.items_job[item] : 0] : a.d;")
-    out.append("    const int sp = a.items[2 * item];")
-    out.append("    const int ks_first = a.items[2 * item + 1];")
+    out.append("    const eip::cptr<J3cData> jt = kJobs ? j3c_item_job(a, item) : (eip::cptr<J3cData>) nullptr;")
+    out.append("#define JD(f) (kJobs ? (decltype(a.d.f)) jt->f : a.d.f)")
+    out.append("    const int sp = eip::ro(a.items)[2 * item];")
+    out.append("    const int ks_first = eip::ro(a.items)[2 * item + 1];")
     out.append("    const int ks_raw = ks_first + (int) threadIdx.x;")
-    out.append("    const int ks_last = a.items_ks_end ? a.items_ks_end[item] : (int) d.n_ks;")
+    out.append("    const int ks_last = a.items_ks_end ? eip::ro(a.items_ks_end)[item] : (int) JD(n_ks);")
     out.append("    const bool active = ks_raw < ks_last;")
     out.append("    const int ks = active ? ks_raw : ks_last - 1;")
-    out.append("    const int ksh = d.ks_perm ? d.ks_perm[ks] : ks;")
-    out.append("    const int kp0 = d.ks_prim_off[ksh], kp1 = d.ks_prim_off[ksh + 1];")
-    out.append("    const int pp0 = a.items_pp ? a.items_pp[2 * item] : d.sp_pp_off[sp];")
-    out.append("    const int pp1 = a.items_pp ? a.items_pp[2 * item + 1] : d.sp_pp_off[sp + 1];")
+    out.append("    const int ksh = JD(ks_perm) ? eip::ro(JD(ks_perm))[ks] : ks;")
+    out.append("    const int kp0 = eip::ro(JD(ks_prim_off))[ksh], kp1 = eip::ro(JD(ks_prim_off))[ksh + 1];")
+    out.append("    const int pp0 = a.items_pp ? eip::ro(a.items_pp)[2 * item] : eip::ro(JD(sp_pp_off))[sp];")
+    out.append("    const int pp1 = a.items_pp ? eip::ro(a.items_pp)[2 * item + 1] : eip::ro(JD(sp_pp_off))[sp + 1];")
     for ij in range(nij):
         for m in ms:
             out.append(f"    {ty} acc_{ij}_{m} = 0;")
@@ -236,12 +240,12 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64"):
     # E of one primitive pair is then loaded right where it is consumed, as scalar loads, instead of
     # being hoisted out of an inner loop and spilled from SGPRs into VGPR lanes.
     out.append("    for (int kp = kp0; kp < kp1; ++kp) {")
-    out.append("      const double Cx = d.ka_C[4 * kp + 0], Cy = d.ka_C[4 * kp + 1], Cz = d.ka_C[4 * kp + 2];")
-    out.append("      const double q = d.ka_C[4 * kp + 3];")
-    out.append("      const double ck = d.ka_coef[kp];")
+    out.append("      const double Cx = eip::ro(JD(ka_C))[4 * kp + 0], Cy = eip::ro(JD(ka_C))[4 * kp + 1], Cz = eip::ro(JD(ka_C))[4 * kp + 2];")
+    out.append("      const double q = eip::ro(JD(ka_C))[4 * kp + 3];")
+    out.append("      const double ck = eip::ro(JD(ka_coef))[kp];")
     out.append("      for (int pp = pp0; pp < pp1; ++pp) {")
-    out.append(f"        const {ty}* __restrict__ E = d.{'E32' if f32 else 'E'} + (size_t) pp * {nnz};")
-    out.append("        const double Px = d.pp_P[4 * pp + 0], Py = d.pp_P[4 * pp + 1], Pz = d.pp_P[4 * pp + 2], p = d.pp_P[4 * pp + 3];")
+    out.append(f"        const auto E = eip::ro(JD({'E32' if f32 else 'E'})) + (size_t) pp * {nnz};")
+    out.append("        const auto Pp = eip::ro(JD(pp_P)) + 4 * (size_t) pp;\n        const double Px = Pp[0], Py = Pp[1], Pz = Pp[2], p = Pp[3];")
     # geometry and prefactor in double always (cheap), then cast
     out.append("        const double dPQx = Px - Cx, dPQy = Py - Cy, dPQz = Pz - Cz;")
     out.append("        double zeta, pf;")
@@ -297,10 +301,10 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64"):
     out.append("      }")
     out.append("    }")
     out.append("    if (!active) return;")
-    out.append("    const int col = d.ks_col[ksh];")
-    out.append("    const int I = d.sp_i[sp], J = d.sp_j[sp];")
-    out.append("    double* __restrict__ o = d.out + (d.sp_out_base ? d.sp_out_base[sp] : 0LL);")
-    out.append("    const long long ld = d.sp_ld ? (long long) d.sp_ld[sp] : d.ld;")
+    out.append("    const int col = eip::ro(JD(ks_col))[ksh];")
+    out.append("    const int I = eip::ro(JD(sp_i))[sp], J = eip::ro(JD(sp_j))[sp];")
+    out.append("    __attribute__((address_space(1))) double* __restrict__ o = (__attribute__((address_space(1))) double*) JD(out) + (JD(sp_out_base) ? eip::ro(JD(sp_out_base))[sp] : 0LL);")
+    out.append("    const long long ld = JD(sp_ld) ? (long long) eip::ro(JD(sp_ld))[sp] : JD(ld);")
     for op in ("=", "+="):
         out.append("    if (a.accumulate == 0) {" if op == "=" else "    } else {")
         out.append("     if (a.layout == 0) {")
@@ -319,7 +323,10 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64"):
             out.append("      }")
         out.append("     }")
     out.append("    }")
+    out.append("#undef JD")
     out.append("}")
+    out.append(f'extern "C" __global__ void __launch_bounds__(64) {name}(const J3cArgs a) {{ {name}_body<false>(a); }}')
+    out.append(f'extern "C" __global__ void __launch_bounds__(64) {name}_jobs(const J3cArgs a) {{ {name}_body<true>(a); }}')
     return name, "\n".join(out)
 
 

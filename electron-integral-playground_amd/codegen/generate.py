@@ -92,7 +92,8 @@ def generate(out_dir, shrink=None):
                 for m0, m1 in sorted({c for plan in cls_plans for c in plan}):
                     name, code = emit_j3c(la, lb, lc, m0, m1, prec)
                     src.append(code)
-                    cases.append(f"        case {lc * 10000 + m0 * 100 + m1}: hipLaunchKernelGGL({name}, dim3(nblocks), dim3(64), 0, s, a); return 0;")
+                    cases.append(f"        case {lc * 10000 + m0 * 100 + m1}: if (a.jobs) hipLaunchKernelGGL({name}_jobs, dim3(nblocks), dim3(64), 0, s, a);"
+                                 f" else hipLaunchKernelGGL({name}, dim3(nblocks), dim3(64), 0, s, a); return 0;")
             fn = f"eip_launch_j3c_{la}{lb}_{prec}"
             src.append(f"int {fn}(int lc, int m0, int m1, const J3cArgs& a, int nblocks, hipStream_t s)\n{{\n"
                        "    switch (lc * 10000 + m0 * 100 + m1) {\n" + "\n".join(cases) + "\n    }\n    return -1;\n}\n")

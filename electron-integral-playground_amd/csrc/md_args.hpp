@@ -91,3 +91,12 @@ struct J2cArgs {
     long long ld;
     double omega;
 };
+
+// Job-table entry of work item 'item' (a.jobs, indexed by a.items_job or 0), viewed through the constant
+// address space: its fields are wave-uniform invariant loads, re-read where they are used.
+__device__ __forceinline__ const __attribute__((address_space(4))) J3cData* j3c_item_job(const J3cArgs a, int item)
+{
+    typedef const __attribute__((address_space(4))) J3cData* job_ptr;
+    typedef const __attribute__((address_space(4))) int* int_ptr;
+    return (job_ptr) a.jobs + (a.items_job ? ((int_ptr) a.items_job)[item] : 0);
+}

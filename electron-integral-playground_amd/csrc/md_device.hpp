@@ -94,6 +94,19 @@ __device__ __forceinline__ void boys32(const float T, float* __restrict__ F)
 }
 
 // Row of the i >= j packed lower triangle holding element (i, j) of a symmetric matrix.
+// Address-space views of read-only kernel inputs.  Data the kernels only read (pair matrices, primitive
+// data, offsets, the job table) are viewed through the constant address space, so wave-uniform loads
+// become scalar loads (SGPR operands) and lane-varying loads global (not flat) vector loads; nothing
+// writes these arrays while a kernel runs.
+template <class T>
+using cptr = const __attribute__((address_space(4))) T*;
+
+template <class T>
+__device__ __forceinline__ cptr<T> ro(const T* p)
+{
+    return (cptr<T>) p;
+}
+
 __device__ __forceinline__ long long packed_row(int i, int j)
 {
     return i >= j ? (long long) i * (i + 1) / 2 + j : (long long) j * (j + 1) / 2 + i;

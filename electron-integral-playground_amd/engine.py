@@ -569,12 +569,13 @@ def _variant_kernels(lib, la, lb, lc, v, f32):
 
 
 def usable_variants(lib, la, lb, lc, f32=False):
-    """Variants of a class none of whose kernels use scratch memory (variant 0 is always kept)."""
+    """Variants of a class none of whose kernels spill to scratch memory (variant 0 is always kept)."""
+    from .build import spills
     res = _json.loads(RESOURCES_FILE.read_text()) if RESOURCES_FILE.exists() else {}
     out = []
     for v in range(max(0, lib.eip_three_center_nvariants(la, lb, lc))):
         names = _variant_kernels(lib, la, lb, lc, v, f32)
-        if v == 0 or all(res.get(n, {}).get("scratch", 1) == 0 for n in names):
+        if v == 0 or all(n in res and not spills(res[n]) for n in names):
             out.append(v)
     return out
 
