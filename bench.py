@@ -194,6 +194,38 @@ def mixed_leg(threshold=1e-6, block_bytes=40e9, n_atom=None):
     return res
 
 
+def serving_leg(n_jobs, rate, budget=200e9, mixed=True):
+    """Poisson stream of integral jobs (atom counts lognormal, median 40, clipped 10-250; 4s3p2d1f
+    orbital and s-g auxiliary generators) through the continuous-batching server."""
+    import torch
+    from electron_integral_playground_amd import generators
+    from electron_integral_playground_amd.serving import IntegralServer, run_poisson
+    jobs = generators.serving_jobs(n_jobs=n_jobs, rate=rate)
+    # warm-up: one small job through a throw-away server (kernel loading, allocator)
+    warm = IntegralServer(budget_bytes=budget, mixed=mixed)
+    warm.submit(0, generators.serving_molecule(10, 1))
+    warm.drain(timeout=300)
+    warm.close()
+    del warm
+    torch.cuda.empty_cache()
+    server = IntegralServer(budget_bytes=budget, mixed=mixed)
+    span = run_poisson(server, jobs, generators.serving_molecule, log=log)
+    server.close()
+    done = [j for j in server.completed if j.error is None]
+    lat = np.array([j.latency for j in done])
+    n_int = sum(j.n_aux * j.n_ao * (j.n_ao + 1) // 2 for j in done)
+    res = {"jobs": n_jobs, "completed": len(done), "errors": len(server.completed) - len(done),
+           "rate_jobs_per_s": rate, "budget_GB": budget / 1e9, "mixed_precision": mixed,
+           "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+           "mean_latency_s": float(lat.mean()), "max_latency_s": float(lat.max()),
+           "span_s": span, "jobs_per_s": len(done) / span, "integrals_per_s": n_int / span,
+           "steps": server.n_steps, "launches": server.n_launches,
+           "median_atoms": float(np.median([n for _, n, _ in jobs])),
+           "atoms_range": [int(min(n for _, n, _ in jobs)), int(max(n for _, n, _ in jobs))]}
+    log(f"serving: {res}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -204,6 +236,10 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision workload")
     ap.add_argument("--mixed-only", action="store_true", help="run only the mixed-precision workload")
     ap.add_argument("--mixed-atoms", type=int, default=None, help="atoms of the mixed workload (default 200)")
+    ap.add_argument("--serving", action="store_true", help="run only the serving workload")
+    ap.add_argument("--serving-jobs", type=int, default=2000)
+    ap.add_argument("--serving-rate", type=float, default=1.0, help="Poisson arrival rate, jobs/s")
+    ap.add_argument("--serving-plain", action="store_true", help="serve in plain float64 instead of mixed")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 4 GiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -219,6 +255,10 @@ def main():
                                                           three_center_flops_per_triplet)
 
     torch.cuda.set_device(0)
+    if args.serving:
+        print(json.dumps({"serving": serving_leg(args.serving_jobs, args.serving_rate,
+                                                 mixed=not args.serving_plain)}), flush=True)
+        return
     if args.mixed_only:
         print(json.dumps({"mixed_precision": mixed_leg(n_atom=args.mixed_atoms)}), flush=True)
         return

@@ -232,6 +232,20 @@ class ThreeCenterPlan:
     def model_flops(self):
         return sum(n * three_center_flops_per_triplet(*k) for k, n in self.primitive_triplets().items())
 
+    def device_bytes(self):
+        """Device memory held by the plan (pair and auxiliary data, pair matrices, work items)."""
+        def size(v):
+            if hasattr(v, "numel"):
+                return v.numel() * v.element_size()
+            if isinstance(v, dict):
+                return sum(size(x) for x in v.values())
+            if isinstance(v, (list, tuple)):
+                return sum(size(x) for x in v)
+            return 0
+        total = sum(size(pc.dev) for pc in self.pairs) + sum(size(ac.dev) for ac in self.aux)
+        total += size(self.items) + size(getattr(self, "mixed", {})) + size(self.zero_rows)
+        return int(total)
+
     # -- execution -------------------------------------------------------------------------------
     def new_output(self):
         return self.torch.empty((self.n_packed, self.n_aux), dtype=self.torch.float64, device=self.device)

@@ -1,0 +1,355 @@
+"""Continuous-batching integral server: jobs (geometry + basis) arrive over time; their pair tables,
+metric and three-centre outputs stay resident in HBM; admission is by the projected footprint of each
+job's (ij|P) tensor against a device budget; all admitted work of one step runs as ONE launch per
+angular class (work items of every job in the step, each item pointing at its job through a job table),
+and a new step is formed while the previous one is still on the GPU, so arrivals join as memory and
+slots free up.
+
+Unit of work: a whole job when its compact (ij|P) tensor (significant shell pairs x n_aux, float64)
+fits the per-unit cap, otherwise blocks of shell pairs of one class (a job larger than device memory is
+streamed block by block; every block is consumed -- reduced to a checksum -- and freed when it is done).
+A job is complete when all its units are; latency = completion (observed on the host) - arrival.
+
+Determinism: every output element is written by exactly one lane of one float64 launch (and, in the
+mixed mode, added to by one lane of one float32 launch) with a fixed summation order, so a job's result
+does not depend on which other jobs share its launches.
+"""
+
+import concurrent.futures as cf
+import time
+from collections import deque
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import pair_utility
+from .engine import ThreeCenterPlan
+
+UNIT_CAP_FRACTION = 0.25      # a unit's output may use at most this share of the budget
+INPUT_FRACTION = 0.25         # admitted jobs' resident inputs (pair tables, metric) use at most this share
+
+
+def _q_aux(plan, J2c):
+    diag = J2c.diagonal().cpu().numpy()
+    out = []
+    for ac in plan.aux:
+        idx = ac.ks_start[:, None] + np.arange(2 * ac.lc + 1)[None, :]
+        out.append(np.sqrt(np.abs(diag[idx]).max(axis=1)))
+    return out
+
+
+@dataclass
+class Unit:
+    job: "Job"
+    blocks: List[tuple]           # (pi, sp0, sp1)
+    rows: int                     # compact output rows
+    bytes: int
+    row_off: List[int]            # first output row of each block
+    out: object = None            # device tensor (rows, n_aux) while in flight / kept
+    checksum: object = None
+
+
+@dataclass
+class Job:
+    job_id: int
+    arrival: float
+    molecule: object
+    n_ao: int = 0
+    n_aux: int = 0
+    plan: Optional[ThreeCenterPlan] = None
+    J2c: object = None
+    units: List[Unit] = field(default_factory=list)
+    units_done: int = 0
+    completion: Optional[float] = None
+    input_bytes: int = 0
+    error: Optional[str] = None
+
+    @property
+    def latency(self):
+        return None if self.completion is None else self.completion - self.arrival
+
+
+class IntegralServer:
+    """Single-GPU continuous-batching server for (ij|P) (mixed precision or plain float64) plus the
+    float64 metric (P|Q) of every job."""
+
+    def __init__(self, budget_bytes=200e9, mixed=True, threshold=1e-6, keep_outputs=False, prep_workers=8,
+                 max_steps_in_flight=2, unit_cap_bytes=None, device="cuda", log=None):
+        import torch
+        self.torch = torch
+        self.budget = float(budget_bytes)
+        self.unit_cap = min(UNIT_CAP_FRACTION * self.budget, unit_cap_bytes or float("inf"))
+        self.mixed = mixed
+        self.threshold = threshold
+        self.keep_outputs = keep_outputs
+        self.max_steps_in_flight = max_steps_in_flight
+        self.device = torch.device(device)
+        self.compute_stream = torch.cuda.Stream(device=self.device)
+        self.setup_stream = torch.cuda.Stream(device=self.device)
+        self.pool = cf.ThreadPoolExecutor(prep_workers)
+        self.log = log
+        self.prepared = deque()       # host-prepared jobs awaiting device setup and admission
+        self.pending_units = deque()  # units of admitted jobs not yet launched
+        self.in_flight = deque()      # (event, [units])
+        self.inputs_used = 0.0        # bytes of resident job inputs (pair tables, metric)
+        self.outputs_used = 0.0       # bytes of unit outputs in flight (or kept)
+        self.jobs: Dict[int, Job] = {}
+        self.completed: List[Job] = []
+        self.n_launches = 0
+        self.n_steps = 0
+
+    # -- host preparation (thread pool) --------------------------------------------------------
+    def submit(self, job_id, molecule, arrival=None):
+        """Accept a job (a Molecule, or a callable returning one); its geometry, basis and pair tables
+        are built on a host thread."""
+        job = Job(job_id, time.perf_counter() if arrival is None else arrival, molecule)
+        self.jobs[job_id] = job
+        fut = self.pool.submit(self._prepare_host, job)
+        self.prepared.append((job, fut))
+        return job
+
+    @staticmethod
+    def _prepare_host(job):
+        if callable(job.molecule):
+            job.molecule = job.molecule()
+        mol = job.molecule
+        arrays = pair_utility.form_pair_arrays(mol, pair_utility.REFERENCE_SCREENING_THRESHOLD)
+        pairs = pair_utility.form_primitive_pair_data(mol, arrays)
+        aux = pair_utility.form_auxiliary_pair_data(mol)
+        return arrays, pairs, aux
+
+    # -- footprint --------------------------------------------------------------------------------
+    @staticmethod
+    def projected_bytes(molecule, pairs):
+        """Compact (ij|P) bytes: significant shell pairs' rows x n_aux float64."""
+        from .engine import group_pairs
+        rows = 0
+        for (la, lb), pd in pairs.items():
+            _, si, _, _ = group_pairs(pd)
+            rows += len(si) * (2 * la + 1) * (2 * lb + 1)
+        return rows * molecule.n_aux * 8
+
+    # -- device setup and admission -----------------------------------------------------------------
+    def _setup_device(self, job, host):
+        torch = self.torch
+        arrays, pairs, aux = host
+        mol = job.molecule
+        job.n_ao, job.n_aux = mol.n_ao, mol.n_aux
+        with torch.cuda.stream(self.setup_stream):
+            plan = ThreeCenterPlan(mol, pairs, aux, device=self.device)
+            job.J2c = plan.two_center()
+            q = _q_aux(plan, job.J2c)                     # synchronises the setup stream only
+            if self.mixed:
+                plan.setup_mixed(arrays, q, threshold=self.threshold)
+            self.setup_stream.synchronize()
+        job.plan = plan
+        job.input_bytes = plan.device_bytes() + job.J2c.numel() * 8
+        # units: the whole job, or blocks of shell pairs
+        total_rows = sum(plan.block_rows(pi, 0, pc.n_sp) for pi, pc in enumerate(plan.pairs))
+        if total_rows * mol.n_aux * 8 <= self.unit_cap:
+            blocks = [(pi, 0, pc.n_sp) for pi, pc in enumerate(plan.pairs) if pc.n_sp > 0]
+            groups = [blocks]
+        else:
+            groups = [[b] for b in plan.row_blocks(self.unit_cap)]
+        for g in groups:
+            offs, r = [], 0
+            for b in g:
+                offs.append(r)
+                r += plan.block_rows(*b)
+            job.units.append(Unit(job, g, r, r * mol.n_aux * 8, offs))
+
+    @staticmethod
+    def estimate_input_bytes(molecule, host):
+        """Resident inputs of a job before it is set up: pair matrices (float64 + float32 copies, at most
+        2143 entries per primitive pair for (ff)), primitive data, and the float64 metric."""
+        _, pairs, aux = host
+        n_pp = sum(len(pd.P_p) for pd in pairs.values())
+        return n_pp * (120 + 12 * 400) + molecule.n_aux ** 2 * 8
+
+    def _admit(self):
+        """Device setup for prepared jobs, in arrival order, while their resident inputs stay within
+        INPUT_FRACTION of the budget (a job is always admitted when none is resident).  Output memory is
+        taken per unit when a step is formed; since inputs never exceed INPUT_FRACTION and a unit never
+        exceeds UNIT_CAP_FRACTION, the head unit always fits once the steps in flight have retired."""
+        while self.prepared:
+            job, fut = self.prepared[0]
+            if not fut.done():
+                break
+            host = fut.result()
+            est = self.estimate_input_bytes(job.molecule, host)
+            if self.inputs_used > 0 and self.inputs_used + est > INPUT_FRACTION * self.budget:
+                break
+            self.prepared.popleft()
+            try:
+                self._setup_device(job, host)
+            except Exception as exc:                      # a job that cannot run is reported, not fatal
+                job.error = repr(exc)
+                job.completion = time.perf_counter()
+                self.completed.append(job)
+                continue
+            self.inputs_used += job.input_bytes
+            self.pending_units.extend(job.units)
+
+    # -- steps ----------------------------------------------------------------------------------------
+    def _form_step(self):
+        """Pending units (FIFO) whose outputs fit the free budget."""
+        units, free = [], self.budget - self.inputs_used - self.outputs_used
+        if not self.in_flight and self.outputs_used == 0 and self.pending_units and self.pending_units[0].bytes > free:
+            # progress guarantee for a single job whose inputs alone crowd the budget
+            free = self.pending_units[0].bytes
+        while self.pending_units and self.pending_units[0].bytes <= free:
+            u = self.pending_units.popleft()
+            free -= u.bytes
+            units.append(u)
+        return units
+
+    def _launch_step(self, units):
+        torch = self.torch
+        step_bytes = 0
+        with torch.cuda.stream(self.compute_stream):
+            for u in units:
+                u.out = torch.empty((u.rows, u.job.n_aux), dtype=torch.float64, device=self.device)
+                step_bytes += u.bytes
+            self.outputs_used += step_bytes
+            # per angular class: items of every unit, one job-table entry per (unit, pair class, aux class)
+            work = {}
+            entries = []
+            for u in units:
+                plan = u.job.plan
+                for (pi, sp0, sp1), r0 in zip(u.blocks, u.row_off):
+                    pc = plan.pairs[pi]
+                    nij = (2 * pc.la + 1) * (2 * pc.lb + 1)
+                    base = u.out.data_ptr() + 8 * (r0 - sp0 * nij) * u.job.n_aux
+                    for ai, ac in enumerate(plan.aux):
+                        perm = plan.mixed[(pi, ai)]["perm"] if self.mixed else None
+                        entries.append(plan.job_words(pi, ai, base, u.job.n_aux, perm))
+                        j = len(entries) - 1
+                        key = (pc.la, pc.lb, ac.lc)
+                        w = work.setdefault(key, {"f64": [], "f32": [], "pp64": [], "pp32": [], "j64": [], "j32": [],
+                                                  "ref": (plan, pi, ai)})
+                        if self.mixed:
+                            m = plan.mixed[(pi, ai)]
+                            G = m["G"]
+                            it, pp = m["it64"][sp0 * G:sp1 * G], m["pp64"][sp0 * G:sp1 * G]
+                            lo, hi = np.searchsorted(m["sp32"], [sp0, sp1])
+                            it32, pp32 = m["it32"][lo:hi], m["pp32"][lo:hi]
+                        else:
+                            G = (ac.n_ks + 63) // 64
+                            it, pp = plan.items[(pi, ai)][sp0 * G:sp1 * G], None
+                            it32 = pp32 = None
+                        if it.shape[0]:
+                            w["f64"].append(it); w["pp64"].append(pp); w["j64"].append((j, it.shape[0]))
+                        if it32 is not None and it32.shape[0]:
+                            w["f32"].append(it32); w["pp32"].append(pp32); w["j32"].append((j, it32.shape[0]))
+            keep = []
+
+            def upload(a):
+                """Host array -> device without synchronising: pinned staging, asynchronous copy on the
+                compute stream; the staging buffer is kept until the step retires."""
+                h = torch.from_numpy(np.ascontiguousarray(a)).pin_memory()
+                keep.append(h)
+                return h.to(self.device, non_blocking=True)
+
+            jobs = upload(np.array(entries, dtype=np.int64).reshape(-1, 16))
+            keep.append(jobs)
+            for key, w in work.items():
+                plan, pi, ai = w["ref"]
+                for prec in ("f64", "f32"):
+                    if not w[prec]:
+                        continue
+                    items = torch.cat(w[prec])
+                    pp = torch.cat(w["pp" + prec[1:]]) if self.mixed else None
+                    jid = upload(np.repeat(np.array([j for j, _ in w["j" + prec[1:]]], dtype=np.int32),
+                                           [n for _, n in w["j" + prec[1:]]]))
+                    keep += [items, jid] + ([pp] if pp is not None else [])
+                    plan.launch_work(pi, ai, items, jobs, job=jid, pp=pp, layout=1,
+                                     accumulate=int(prec == "f32"), use_f32=(prec == "f32"),
+                                     stream=self.compute_stream)
+                    self.n_launches += 1
+            ev = torch.cuda.Event()
+            ev.record(self.compute_stream)
+        self.in_flight.append((ev, units, keep))
+        self.n_steps += 1
+
+    def _retire(self):
+        """Consume the units of finished steps (checksum on the device) and free their memory."""
+        torch = self.torch
+        while self.in_flight and self.in_flight[0][0].query():
+            now = time.perf_counter()
+            _, units, _ = self.in_flight.popleft()
+            with torch.cuda.stream(self.compute_stream):
+                for u in units:
+                    u.checksum = torch.stack([u.out.sum(), (u.out * u.out).sum()])
+                    if not self.keep_outputs:
+                        u.out = None
+                    if not self.keep_outputs:
+                        self.outputs_used -= u.bytes
+                    job = u.job
+                    job.units_done += 1
+                    if job.units_done == len(job.units):
+                        job.completion = now
+                        self.completed.append(job)
+                        if not self.keep_outputs:
+                            job.plan = None
+                            job.J2c = None
+                            self.inputs_used -= job.input_bytes
+
+    def poll(self):
+        """One scheduling iteration: retire finished steps, admit prepared jobs, launch a step."""
+        self._retire()
+        self._admit()
+        if len(self.in_flight) < self.max_steps_in_flight and self.pending_units:
+            units = self._form_step()
+            if units:
+                self._launch_step(units)
+                return True
+        return False
+
+    def drain(self, timeout=None):
+        t0 = time.perf_counter()
+        while self.prepared or self.pending_units or self.in_flight:
+            if not self.poll():
+                time.sleep(2e-4)
+            if timeout is not None and time.perf_counter() - t0 > timeout:
+                raise TimeoutError("server did not drain")
+        self.torch.cuda.synchronize()
+
+    def close(self):
+        self.pool.shutdown(wait=True)
+
+    # -- results ------------------------------------------------------------------------------------
+    def job_output(self, job_id):
+        """(ij|P) of a job kept on the device (keep_outputs=True, whole-job units): the compact tensor
+        rows (pair class pi, shell pair sp, ij) -> {pi: tensor (n_sp * nij, n_aux)}."""
+        job = self.jobs[job_id]
+        out = {}
+        for u in job.units:
+            for (pi, sp0, sp1), r0 in zip(u.blocks, u.row_off):
+                pc = job.plan.pairs[pi]
+                nij = (2 * pc.la + 1) * (2 * pc.lb + 1)
+                out.setdefault(pi, []).append(u.out[r0:r0 + (sp1 - sp0) * nij])
+        return {pi: self.torch.cat(v) for pi, v in out.items()}
+
+
+def run_poisson(server, jobs, make_molecule, log=None):
+    """Feed (arrival offset s, n_atom, seed) jobs to the server at their arrival times (wall clock);
+    returns per-job latencies and the span."""
+    t0 = time.perf_counter()
+    i = logged = 0
+    n = len(jobs)
+    while i < n or server.prepared or server.pending_units or server.in_flight:
+        now = time.perf_counter() - t0
+        while i < n and jobs[i][0] <= now:
+            t_arr, n_atom, seed = jobs[i]
+            server.submit(i, (lambda n=n_atom, sd=seed: make_molecule(n, sd)), arrival=t0 + t_arr)
+            i += 1
+        if not server.poll():
+            time.sleep(2e-4)
+        if log and len(server.completed) >= logged + 50:
+            logged = len(server.completed)
+            log(f"  {logged}/{n} jobs done at {time.perf_counter() - t0:.1f}s, steps in flight "
+                f"{len(server.in_flight)}, pending units {len(server.pending_units)}, inputs "
+                f"{server.inputs_used / 1e9:.1f} GB, outputs {server.outputs_used / 1e9:.1f} GB")
+    server.torch.cuda.synchronize()
+    return time.perf_counter() - t0
