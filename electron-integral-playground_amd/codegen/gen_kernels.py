@@ -51,6 +51,19 @@ def pair_nnz(la, lb, spherical):
     return len(pair_layout(la, lb, spherical))
 
 
+def pair_stride(la, lb, spherical):
+    """Row stride of the pair matrix: the entries rounded up to 4, so every row starts on a 16-byte
+    boundary in both the float64 and float32 copies (rows are staged into LDS in 16-byte pieces)."""
+    return (pair_nnz(la, lb, spherical) + 3) // 4 * 4
+
+
+# Three-centre kernels stage each primitive pair's matrix row (and P) into LDS with asynchronous
+# global->LDS loads, double-buffered: the row of the next primitive pair is in flight while the current
+# one is contracted, and the contraction reads wave-uniform LDS broadcasts instead of waiting on scalar
+# loads.
+STAGE_LDS = True
+
+
 # ---------------------------------------------------------------------------------------------
 # Hermite E (pair) code: VRR + HRR per axis
 # ---------------------------------------------------------------------------------------------
@@ -117,8 +130,9 @@ def emit_pairE(la, lb, spherical):
     L.append("    (void) PAx; (void) PAy; (void) PAz; (void) ABx; (void) ABy; (void) ABz; (void) o2p;")
     for ax in "xyz":
         _emit_E_axis(L, ax, la, lb)
-    L.append("    double* __restrict__ out = a.E + (size_t) pp * %d;" % nnz)
-    L.append("    float* __restrict__ out32 = a.E32 ? a.E32 + (size_t) pp * %d : nullptr;" % nnz)
+    stride = pair_stride(la, lb, spherical) if spherical else nnz
+    L.append("    double* __restrict__ out = a.E + (size_t) pp * %d;" % stride)
+    L.append("    float* __restrict__ out32 = a.E32 ? a.E32 + (size_t) pp * %d : nullptr;" % stride)
     for k, (t, row) in enumerate(layout):
         r, terms = rows[row]
         pieces = []
@@ -133,7 +147,7 @@ def emit_pairE(la, lb, spherical):
         expr = " + ".join(pieces) if pieces else "0.0"
         L.append(f"    {{ const double v = c * ({expr}); out[{k}] = v; if (out32) out32[{k}] = (float) v; }}")
     L.append("}")
-    return name, nnz, "\n".join(L)
+    return name, stride, "\n".join(L)
 
 
 # ---------------------------------------------------------------------------------------------
@@ -239,13 +253,54 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64"):
     # Auxiliary primitives (per lane) outside, primitive pairs (wave-uniform) inside: the pair matrix
     # E of one primitive pair is then loaded right where it is consumed, as scalar loads, instead of
     # being hoisted out of an inner loop and spilled from SGPRs into VGPR lanes.
-    out.append("    for (int kp = kp0; kp < kp1; ++kp) {")
-    out.append("      const double Cx = eip::ro(JD(ka_C))[4 * kp + 0], Cy = eip::ro(JD(ka_C))[4 * kp + 1], Cz = eip::ro(JD(ka_C))[4 * kp + 2];")
-    out.append("      const double q = eip::ro(JD(ka_C))[4 * kp + 3];")
-    out.append("      const double ck = eip::ro(JD(ka_coef))[kp];")
-    out.append("      for (int pp = pp0; pp < pp1; ++pp) {")
-    out.append(f"        const auto E = eip::ro(JD({'E32' if f32 else 'E'})) + (size_t) pp * {nnz};")
-    out.append("        const auto Pp = eip::ro(JD(pp_P)) + 4 * (size_t) pp;\n        const double Px = Pp[0], Py = Pp[1], Pz = Pp[2], p = Pp[3];")
+    stride = pair_stride(la, lb, spherical)
+    esize = 4 if f32 else 8
+    row_bytes = stride * esize
+    n_pieces = (row_bytes + 1023) // 1024                  # 64 lanes x 16 bytes per global->LDS load
+    if STAGE_LDS:
+        out.append(f"    __shared__ __attribute__((aligned(16))) {ty} sE[2][{n_pieces * 1024 // esize}];")
+        out.append("    __shared__ __attribute__((aligned(16))) double sP[2][4];")
+        out.append("    const int lane = (int) threadIdx.x;")
+        out.append(f"    const char* __restrict__ Ebase = (const char*) JD({'E32' if f32 else 'E'});")
+        out.append("    const char* __restrict__ Pbase = (const char*) JD(pp_P);")
+        # stage(pp, buf): issue the asynchronous loads of row pp into buffer buf
+        out.append("#define EIP_STAGE(PP, BUF) do { \\")
+        out.append(f"        const char* src_ = Ebase + (size_t) (PP) * {row_bytes} + 16 * lane; \\")
+        for r in range(n_pieces):
+            cond = "" if (r + 1) * 1024 <= row_bytes else f"if (16 * lane < {row_bytes - r * 1024}) "
+            out.append(f"        {cond}__builtin_amdgcn_global_load_lds((const void*) (src_ + {r * 1024}), "
+                       f"(__attribute__((address_space(3))) void*) ((char*) &sE[BUF][0] + {r * 1024}), 16, 0, 0); \\")
+        out.append("        if (lane < 2) __builtin_amdgcn_global_load_lds((const void*) (Pbase + (size_t) (PP) * 32 + 16 * lane), "
+                   "(__attribute__((address_space(3))) void*) &sP[BUF][0], 16, 0, 0); \\")
+        out.append("    } while (0)")
+    if STAGE_LDS:
+        # Every lane takes part in staging, so the loop over auxiliary primitives runs to the wave's
+        # largest primitive count; lanes past their own count contribute with coefficient 0.
+        out.append("    const int nkp_wave = eip::wave_max(kp1 - kp0);")
+        out.append("    for (int t_kp = 0; t_kp < nkp_wave; ++t_kp) {")
+        out.append("      const bool kp_on = kp0 + t_kp < kp1;")
+        out.append("      const int kp = kp_on ? kp0 + t_kp : kp1 - 1;")
+        out.append("      const double Cx = eip::ro(JD(ka_C))[4 * kp + 0], Cy = eip::ro(JD(ka_C))[4 * kp + 1], Cz = eip::ro(JD(ka_C))[4 * kp + 2];")
+        out.append("      const double q = eip::ro(JD(ka_C))[4 * kp + 3];")
+        out.append("      const double ck = kp_on ? eip::ro(JD(ka_coef))[kp] : 0.0;")
+    else:
+        out.append("    for (int kp = kp0; kp < kp1; ++kp) {")
+        out.append("      const double Cx = eip::ro(JD(ka_C))[4 * kp + 0], Cy = eip::ro(JD(ka_C))[4 * kp + 1], Cz = eip::ro(JD(ka_C))[4 * kp + 2];")
+        out.append("      const double q = eip::ro(JD(ka_C))[4 * kp + 3];")
+        out.append("      const double ck = eip::ro(JD(ka_coef))[kp];")
+    if STAGE_LDS:
+        out.append("      if (pp0 < pp1) EIP_STAGE(pp0, 0);")
+        out.append("      for (int pp = pp0, buf = 0; pp < pp1; ++pp, buf ^= 1) {")
+        # vmcnt(0): the asynchronous global->LDS loads of row pp have landed (the compiler does not
+        # track them across the loop back-edge); the barrier orders the LDS accesses.
+        out.append("        __builtin_amdgcn_s_waitcnt(0x0F70);")
+        out.append("        __syncthreads();")
+        out.append(f"        const {ty}* __restrict__ E = &sE[buf][0];")
+        out.append("        const double Px = sP[buf][0], Py = sP[buf][1], Pz = sP[buf][2], p = sP[buf][3];")
+    else:
+        out.append("      for (int pp = pp0; pp < pp1; ++pp) {")
+        out.append(f"        const auto E = eip::ro(JD({'E32' if f32 else 'E'})) + (size_t) pp * {stride};")
+        out.append("        const auto Pp = eip::ro(JD(pp_P)) + 4 * (size_t) pp;\n        const double Px = Pp[0], Py = Pp[1], Pz = Pp[2], p = Pp[3];")
     # geometry and prefactor in double always (cheap), then cast
     out.append("        const double dPQx = Px - Cx, dPQy = Py - Cy, dPQz = Pz - Cz;")
     out.append("        double zeta, pf;")
@@ -270,6 +325,8 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64"):
     out.append(f"        const {ty} T = ({ty}) (zeta * (dPQx * dPQx + dPQy * dPQy + dPQz * dPQz));")
     out.append(f"        {ty} F[{L + 1}];")
     out.append(f"        eip::boys{'32' if f32 else '64'}<{L}>(T, F);")
+    if STAGE_LDS:
+        out.append("        if (pp + 1 < pp1) EIP_STAGE(pp + 1, buf ^ 1);   // after the Boys table loads: their wait excludes it")
     out.append(f"        const {ty} m2z = ({ty}) (-2.0 * zeta);")
     out.append(f"        const {ty} R000_0 = ({ty}) pf * F[0];")
     for n in range(1, L + 1):
@@ -324,6 +381,8 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64"):
         out.append("     }")
     out.append("    }")
     out.append("#undef JD")
+    if STAGE_LDS:
+        out.append("#undef EIP_STAGE")
     out.append("}")
     out.append(f'extern "C" __global__ void __launch_bounds__(64) {name}(const J3cArgs a) {{ {name}_body<false>(a); }}')
     out.append(f'extern "C" __global__ void __launch_bounds__(64) {name}_jobs(const J3cArgs a) {{ {name}_body<true>(a); }}')
@@ -360,7 +419,7 @@ def emit_nuc(la, lb):
     for ij in range(nij):
         out.append(f"    double acc_{ij} = 0;")
     out.append("    for (int pp = pp0; pp < pp1; ++pp) {")
-    out.append(f"        const double* __restrict__ E = a.E + (size_t) pp * {nnz};")
+    out.append(f"        const double* __restrict__ E = a.E + (size_t) pp * {pair_stride(la, lb, spherical)};")
     out.append("        const double p = a.pp_P[4 * pp + 3];")
     out.append("        const double PQx = a.pp_P[4 * pp + 0] - Cx, PQy = a.pp_P[4 * pp + 1] - Cy, PQz = a.pp_P[4 * pp + 2] - Cz;")
     out.append("        double zeta, pf;")

@@ -93,7 +93,14 @@ __device__ __forceinline__ void boys32(const float T, float* __restrict__ F)
     }
 }
 
-// Row of the i >= j packed lower triangle holding element (i, j) of a symmetric matrix.
+// Largest value of v over the 64 lanes of the wave (butterfly over lane-index bits).
+__device__ __forceinline__ int wave_max(int v)
+{
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) v = max(v, __shfl_xor(v, d, 64));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
 // Address-space views of read-only kernel inputs.  Data the kernels only read (pair matrices, primitive
 // data, offsets, the job table) are viewed through the constant address space, so wave-uniform loads
 // become scalar loads (SGPR operands) and lane-varying loads global (not flat) vector loads; nothing
@@ -107,6 +114,7 @@ __device__ __forceinline__ cptr<T> ro(const T* p)
     return (cptr<T>) p;
 }
 
+// Row of the i >= j packed lower triangle holding element (i, j) of a symmetric matrix.
 __device__ __forceinline__ long long packed_row(int i, int j)
 {
     return i >= j ? (long long) i * (i + 1) / 2 + j : (long long) j * (j + 1) / 2 + i;
