@@ -108,7 +108,7 @@ def q_aux_from_metric(plan, J2c):
     return out
 
 
-def mixed_leg(threshold=1e-6, block_bytes=40e9, n_atom=None):
+def mixed_leg(threshold=1e-6, block_bytes=40e9, n_atom=None, tune=None):
     """Mixed-precision workload (200 atoms, 20 A box, 4s3p2d1f, auxiliary s-g, n_ao 6000, n_aux 40000):
     the (ij|P) tensor (3.9 TB over the significant shell pairs) is streamed through two device buffers
     in blocks of shell pairs; each block is computed by the plain float64 kernels and by the mixed
@@ -136,6 +136,10 @@ def mixed_leg(threshold=1e-6, block_bytes=40e9, n_atom=None):
     t_metric = e0.elapsed_time(e1) * 1e-3
     q_aux = q_aux_from_metric(plan, J2c)
     del J2c
+    if tune:
+        from electron_integral_playground_amd.engine import autotune
+        autotune(plan, save=tune, log=log, block_bytes=4e9)
+        autotune(plan, save=tune, log=log, block_bytes=4e9, use_f32=True)
     stats = plan.setup_mixed(arrays, q_aux, threshold=threshold)
     t_setup = time.perf_counter() - t0
     log(f"mixed workload: n_ao={mol.n_ao} n_aux={mol.n_aux} primitive pairs={sum(len(p) for p in arrays.values())}"
@@ -260,7 +264,7 @@ def main():
                                                  mixed=not args.serving_plain)}), flush=True)
         return
     if args.mixed_only:
-        print(json.dumps({"mixed_precision": mixed_leg(n_atom=args.mixed_atoms)}), flush=True)
+        print(json.dumps({"mixed_precision": mixed_leg(n_atom=args.mixed_atoms, tune=args.tune)}), flush=True)
         return
     t0 = time.perf_counter()
     mol = generators.make_molecule("plain")

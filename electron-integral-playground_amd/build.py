@@ -58,9 +58,10 @@ def _compile(src):
 
 
 def spills(res):
-    """A kernel spills when it moves VGPRs to scratch memory.  (SGPR spills go to VGPR lanes, and a few
-    dozen bytes of private segment hold a few SGPR spills or the backend's reserved scavenging slots.)"""
-    return res.get("vgpr_spill", 0) > 0 or res.get("scratch", 0) > 128
+    """A kernel spills when it moves registers to scratch memory.  (VGPRs spilled to AGPRs and SGPRs
+    spilled to VGPR lanes stay in registers; a few dozen bytes of private segment hold a few SGPR spills
+    or the backend's reserved scavenging slots.)"""
+    return res.get("scratch", 0) > 128
 
 
 def _spilling_classes(scratch, shrink):

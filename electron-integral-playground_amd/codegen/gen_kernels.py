@@ -362,23 +362,36 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64"):
     out.append("    const int I = eip::ro(JD(sp_i))[sp], J = eip::ro(JD(sp_j))[sp];")
     out.append("    __attribute__((address_space(1))) double* __restrict__ o = (__attribute__((address_space(1))) double*) JD(out) + (JD(sp_out_base) ? eip::ro(JD(sp_out_base))[sp] : 0LL);")
     out.append("    const long long ld = JD(sp_ld) ? (long long) eip::ro(JD(sp_ld))[sp] : JD(ld);")
-    for op in ("=", "+="):
-        out.append("    if (a.accumulate == 0) {" if op == "=" else "    } else {")
-        out.append("     if (a.layout == 0) {")
-        for ij in range(nij):
-            mi, mj = divmod(ij, nj)
-            out.append(f"      if (I != J || {mi} >= {mj}) {{ const long long row = eip::packed_row(I + {mi}, J + {mj}) * ld + col;")
-            for m in ms:
-                out.append(f"        o[row + {m}] {op} (double) acc_{ij}_{m};")
-            out.append("      }")
-        out.append("     } else {")
-        out.append(f"      const long long base = (long long) sp * {nij};")
-        for ij in range(nij):
-            out.append(f"      {{ const long long row = (base + {ij}) * ld + col;")
-            for m in ms:
-                out.append(f"        o[row + {m}] {op} (double) acc_{ij}_{m};")
-            out.append("      }")
-        out.append("     }")
+    out.append("    if (a.accumulate == 0) {")
+    out.append("     if (a.layout == 0) {")
+    for ij in range(nij):
+        mi, mj = divmod(ij, nj)
+        out.append(f"      if (I != J || {mi} >= {mj}) {{ const long long row = eip::packed_row(I + {mi}, J + {mj}) * ld + col;")
+        for m in ms:
+            out.append(f"        o[row + {m}] = (double) acc_{ij}_{m};")
+        out.append("      }")
+    out.append("     } else {")
+    out.append(f"      const long long base = (long long) sp * {nij};")
+    for ij in range(nij):
+        out.append(f"      {{ const long long row = (base + {ij}) * ld + col;")
+        for m in ms:
+            out.append(f"        o[row + {m}] = (double) acc_{ij}_{m};")
+        out.append("      }")
+    out.append("     }")
+    out.append("    } else {")
+    # accumulate: loads of a group of outputs are issued together, then the sums are stored
+    out.append(f"      const long long base = (long long) sp * {nij};")
+    pairs_ = [(ij, m) for ij in range(nij) for m in ms]
+    for ij in range(nij):
+        mi, mj = divmod(ij, nj)
+        out.append(f"      const bool ok_{ij} = a.layout != 0 || I != J || {mi} >= {mj};")
+        out.append(f"      const long long row_{ij} = (a.layout == 0 ? eip::packed_row(I + {mi}, J + {mj}) : base + {ij}) * ld + col;")
+    for g0 in range(0, len(pairs_), 16):
+        grp = pairs_[g0:g0 + 16]
+        for ij, m in grp:
+            out.append(f"      const double o_{ij}_{m} = ok_{ij} ? o[row_{ij} + {m}] : 0.0;")
+        for ij, m in grp:
+            out.append(f"      if (ok_{ij}) o[row_{ij} + {m}] = o_{ij}_{m} + (double) acc_{ij}_{m};")
     out.append("    }")
     out.append("#undef JD")
     if STAGE_LDS:

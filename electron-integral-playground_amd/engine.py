@@ -146,6 +146,17 @@ def mixed_split(sp_pp_off, p, ub, q, ks_prim_off, threshold):
     return n_diff[:, None] + at_least[:, 1:], perm, gprim
 
 
+_TUNING_APPLIED = False
+
+
+def _ensure_tuning():
+    """Apply the shipped chunk-plan variant table once per process."""
+    global _TUNING_APPLIED
+    if not _TUNING_APPLIED:
+        _TUNING_APPLIED = True
+        apply_tuning()
+
+
 class ThreeCenterPlan:
     """All classes of one molecule's (ij|P), resident on one GPU."""
 
@@ -153,6 +164,7 @@ class ThreeCenterPlan:
         import torch
         self.torch = torch
         self.lib = library()
+        _ensure_tuning()
         self.n_ao, self.n_aux = molecule.n_ao, molecule.n_aux
         self.omega = float(omega)
         self.device = torch.device(device)
@@ -331,7 +343,7 @@ class ThreeCenterPlan:
             raise RuntimeError(f"eip_three_center_work({pc.la},{pc.lb},{ac.lc}, f32={use_f32}) failed with {rc}")
 
     # -- mixed precision ---------------------------------------------------------------------------
-    def setup_mixed(self, pair_list, q_aux, threshold=1e-6):
+    def setup_mixed(self, pair_list, q_aux, threshold=1e-6, granularity="item"):
         """Split every primitive triplet between the float64 and float32 kernels.
 
         Bound of a triplet (primitive pair pp, auxiliary shell P): ub_pp Q_P, with ub_pp the reference's
@@ -345,7 +357,13 @@ class ThreeCenterPlan:
         float64 launch stores its prefix sums (zeros when s = 0), the float32 launch adds the rest.
         ``pair_list``: per class ``PairArrays`` or ``list[PrimitivePair]`` (the list the pair data was
         packed from).  Returns triplet counts.
+
+        granularity "pair" splits inside an item as above (two launches over the item, the second adding
+        into the output); "item" (default) evaluates an item in float32 only when all its primitive pairs
+        qualify (s = 0) and in float64 otherwise, so every item is one wave that stores its result once.
         """
+        if granularity not in ("item", "pair"):
+            raise ValueError("granularity is 'item' or 'pair'")
         from .pair_utility import _as_arrays
         torch = self.torch
         T = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=self.device)
@@ -363,16 +381,23 @@ class ThreeCenterPlan:
                 pp0 = np.repeat(pc.sp_pp_off[:-1].astype(np.int64), G)
                 pp1 = np.repeat(pc.sp_pp_off[1:].astype(np.int64), G)
                 split = pp0 + split_rows.ravel()
-                f32 = split < pp1
-                f64_trip = int(np.sum(split_rows * gprim[None, :]))
-                self.mixed_stats["f64_triplets"] += f64_trip
-                self.mixed_stats["f32_triplets"] += int(np.sum(n_pp_sp) * np.sum(gprim)) - f64_trip
                 items = np.stack([sp, k0], axis=1)
-                self.mixed[(pi, ai)] = dict(
-                    perm=T(perm, torch.int32),
-                    it64=T(items, torch.int32), pp64=T(np.stack([pp0, split], axis=1), torch.int32),
-                    it32=T(items[f32], torch.int32), pp32=T(np.stack([split[f32], pp1[f32]], axis=1), torch.int32),
-                    sp32=sp[f32], G=G)
+                trip = (pp1 - pp0) * np.tile(gprim, pc.n_sp)                 # triplets per item
+                if granularity == "pair":
+                    f32 = split < pp1
+                    f64_trip = int(np.sum(split_rows * gprim[None, :]))
+                    m = dict(it64=T(items, torch.int32), pp64=T(np.stack([pp0, split], axis=1), torch.int32),
+                             sp64=sp, it32=T(items[f32], torch.int32),
+                             pp32=T(np.stack([split[f32], pp1[f32]], axis=1), torch.int32), sp32=sp[f32],
+                             acc32=1)
+                else:
+                    f32 = split == pp0
+                    f64_trip = int(np.sum(trip[~f32]))
+                    m = dict(it64=T(items[~f32], torch.int32), pp64=None, sp64=sp[~f32],
+                             it32=T(items[f32], torch.int32), pp32=None, sp32=sp[f32], acc32=0)
+                self.mixed_stats["f64_triplets"] += f64_trip
+                self.mixed_stats["f32_triplets"] += int(np.sum(trip)) - f64_trip
+                self.mixed[(pi, ai)] = dict(perm=T(perm, torch.int32), G=G, **m)
         return self.mixed_stats
 
     def run_mixed(self, out=None, stream=None, zero=True):
@@ -387,9 +412,18 @@ class ThreeCenterPlan:
         for j, (pi, ai) in enumerate(keys):
             m = self.mixed[(pi, ai)]
             self.launch_work(pi, ai, m["it64"], jobs[j], pp=m["pp64"], layout=0, accumulate=0, stream=stream)
-            self.launch_work(pi, ai, m["it32"], jobs[j], pp=m["pp32"], layout=0, accumulate=1, use_f32=True,
-                             stream=stream)
+            self.launch_work(pi, ai, m["it32"], jobs[j], pp=m["pp32"], layout=0, accumulate=m["acc32"],
+                             use_f32=True, stream=stream)
         return out
+
+    def mixed_items(self, pi, ai, sp0, sp1):
+        """(items64, pp64, items32, pp32, accumulate32) of shell pairs [sp0, sp1) (views; pp may be None)."""
+        m = self.mixed[(pi, ai)]
+        a, b = np.searchsorted(m["sp64"], [sp0, sp1])
+        c, d = np.searchsorted(m["sp32"], [sp0, sp1])
+        pp64 = m["pp64"][a:b] if m["pp64"] is not None else None
+        pp32 = m["pp32"][c:d] if m["pp32"] is not None else None
+        return m["it64"][a:b], pp64, m["it32"][c:d], pp32, m["acc32"]
 
     # -- streaming by shell-pair blocks (outputs larger than device memory) ---------------------------
     def row_blocks(self, max_bytes):
@@ -421,13 +455,10 @@ class ThreeCenterPlan:
                 G = (ac.n_ks + 63) // 64
                 self.launch_work(pi, ai, self.items[(pi, ai)][sp0 * G:sp1 * G], jobs[ai], layout=1, stream=stream)
                 continue
-            m = self.mixed[(pi, ai)]
-            G = m["G"]
-            self.launch_work(pi, ai, m["it64"][sp0 * G:sp1 * G], jobs[ai], pp=m["pp64"][sp0 * G:sp1 * G], layout=1,
-                             accumulate=0, stream=stream)
-            lo, hi = np.searchsorted(m["sp32"], [sp0, sp1])
-            self.launch_work(pi, ai, m["it32"][lo:hi], jobs[ai], pp=m["pp32"][lo:hi], layout=1, accumulate=1,
-                             use_f32=True, stream=stream)
+            it64, pp64, it32, pp32, acc32 = self.mixed_items(pi, ai, sp0, sp1)
+            self.launch_work(pi, ai, it64, jobs[ai], pp=pp64, layout=1, accumulate=0, stream=stream)
+            self.launch_work(pi, ai, it32, jobs[ai], pp=pp32, layout=1, accumulate=acc32, use_f32=True,
+                             stream=stream)
         return out
 
 
@@ -456,6 +487,7 @@ class BatchPlan:
         import torch
         self.torch = torch
         self.lib = library()
+        _ensure_tuning()
         self.omega = float(omega)
         self.device = torch.device(device)
         T = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=self.device)
@@ -606,25 +638,44 @@ def apply_tuning(path=TUNING_FILE):
     return table
 
 
-def autotune(plan, repeats=3, use_f32=False, save=None, log=None):
-    """Time every usable variant of every class of ``plan`` and keep the fastest."""
+def autotune(plan, repeats=3, use_f32=False, save=None, log=None, block_bytes=None):
+    """Time every usable variant of every class of ``plan`` and keep the fastest.
+
+    Classes are timed on the packed output, or -- with ``block_bytes`` (molecules whose output exceeds
+    device memory) -- on the first block of shell pairs of each pair class (compact rows)."""
     torch = plan.torch
     lib = plan.lib
-    out = plan.new_output()
     chosen = {}
     stream = torch.cuda.current_stream()
+    if block_bytes is None:
+        out = plan.new_output()
+        launch = lambda pi, ai: plan.launch_class(pi, ai, out, use_f32=use_f32)
+    else:
+        firsts = {}
+        for b in plan.row_blocks(block_bytes):
+            firsts.setdefault(b[0], b)
+        rows = max(plan.block_rows(*b) for b in firsts.values())
+        buf = torch.empty((rows, plan.n_aux), dtype=torch.float64, device=plan.device)
+
+        def launch(pi, ai):
+            _, sp0, sp1 = firsts[pi]
+            pc = plan.pairs[pi]
+            nij = (2 * pc.la + 1) * (2 * pc.lb + 1)
+            jobs = plan.job_table([plan.job_words(pi, ai, buf.data_ptr() - 8 * sp0 * nij * plan.n_aux, plan.n_aux)])
+            G = (plan.aux[ai].n_ks + 63) // 64
+            plan.launch_work(pi, ai, plan.items[(pi, ai)][sp0 * G:sp1 * G], jobs, layout=1, use_f32=use_f32)
     for pi, pc in enumerate(plan.pairs):
         for ai, ac in enumerate(plan.aux):
             key = f"{pc.la}{pc.lb}{ac.lc}_{'f32' if use_f32 else 'f64'}"
             best = None
             for v in usable_variants(lib, pc.la, pc.lb, ac.lc, use_f32):
                 lib.eip_set_three_center_variant(pc.la, pc.lb, ac.lc, int(use_f32), v)
-                plan.launch_class(pi, ai, out, use_f32=use_f32)          # warm-up
+                launch(pi, ai)                                        # warm-up
                 times = []
                 for _ in range(repeats):
                     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                     e0.record(stream)
-                    plan.launch_class(pi, ai, out, use_f32=use_f32)
+                    launch(pi, ai)
                     e1.record(stream)
                     e1.synchronize()
                     times.append(e0.elapsed_time(e1))

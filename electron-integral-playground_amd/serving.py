@@ -75,13 +75,14 @@ class IntegralServer:
     float64 metric (P|Q) of every job."""
 
     def __init__(self, budget_bytes=200e9, mixed=True, threshold=1e-6, keep_outputs=False, prep_workers=8,
-                 max_steps_in_flight=2, unit_cap_bytes=None, device="cuda", log=None):
+                 max_steps_in_flight=2, unit_cap_bytes=None, granularity="item", device="cuda", log=None):
         import torch
         self.torch = torch
         self.budget = float(budget_bytes)
         self.unit_cap = min(UNIT_CAP_FRACTION * self.budget, unit_cap_bytes or float("inf"))
         self.mixed = mixed
         self.threshold = threshold
+        self.granularity = granularity
         self.keep_outputs = keep_outputs
         self.max_steps_in_flight = max_steps_in_flight
         self.device = torch.device(device)
@@ -141,7 +142,7 @@ class IntegralServer:
             job.J2c = plan.two_center()
             q = _q_aux(plan, job.J2c)                     # synchronises the setup stream only
             if self.mixed:
-                plan.setup_mixed(arrays, q, threshold=self.threshold)
+                plan.setup_mixed(arrays, q, threshold=self.threshold, granularity=self.granularity)
             self.setup_stream.synchronize()
         job.plan = plan
         job.input_bytes = plan.device_bytes() + job.J2c.numel() * 8
@@ -227,13 +228,9 @@ class IntegralServer:
                         j = len(entries) - 1
                         key = (pc.la, pc.lb, ac.lc)
                         w = work.setdefault(key, {"f64": [], "f32": [], "pp64": [], "pp32": [], "j64": [], "j32": [],
-                                                  "ref": (plan, pi, ai)})
+                                                  "ref": (plan, pi, ai), "acc32": 0})
                         if self.mixed:
-                            m = plan.mixed[(pi, ai)]
-                            G = m["G"]
-                            it, pp = m["it64"][sp0 * G:sp1 * G], m["pp64"][sp0 * G:sp1 * G]
-                            lo, hi = np.searchsorted(m["sp32"], [sp0, sp1])
-                            it32, pp32 = m["it32"][lo:hi], m["pp32"][lo:hi]
+                            it, pp, it32, pp32, w["acc32"] = plan.mixed_items(pi, ai, sp0, sp1)
                         else:
                             G = (ac.n_ks + 63) // 64
                             it, pp = plan.items[(pi, ai)][sp0 * G:sp1 * G], None
@@ -259,12 +256,13 @@ class IntegralServer:
                     if not w[prec]:
                         continue
                     items = torch.cat(w[prec])
-                    pp = torch.cat(w["pp" + prec[1:]]) if self.mixed else None
+                    pps = w["pp" + prec[1:]]
+                    pp = torch.cat(pps) if pps and pps[0] is not None else None
                     jid = upload(np.repeat(np.array([j for j, _ in w["j" + prec[1:]]], dtype=np.int32),
                                            [n for _, n in w["j" + prec[1:]]]))
                     keep += [items, jid] + ([pp] if pp is not None else [])
                     plan.launch_work(pi, ai, items, jobs, job=jid, pp=pp, layout=1,
-                                     accumulate=int(prec == "f32"), use_f32=(prec == "f32"),
+                                     accumulate=w["acc32"] if prec == "f32" else 0, use_f32=(prec == "f32"),
                                      stream=self.compute_stream)
                     self.n_launches += 1
             ev = torch.cuda.Event()

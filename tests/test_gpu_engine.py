@@ -72,7 +72,7 @@ def test_mixed_precision_error_bound():
     aux = pair_utility.form_auxiliary_pair_data(mol)
     plan = ThreeCenterPlan(mol, pairs, aux)
     ref = plan.run()
-    stats = plan.setup_mixed(pl, _q_aux_from_metric(plan, plan.two_center()), threshold=1e-6)
+    stats = plan.setup_mixed(pl, _q_aux_from_metric(plan, plan.two_center()), threshold=1e-6, granularity="pair")
     print("mixed split", stats, flush=True)
     assert stats["f32_triplets"] > 0 and stats["f64_triplets"] > 0
     got = plan.run_mixed()
@@ -163,12 +163,12 @@ def test_row_blocks_mixed_within_bound():
     rows = max(plan.block_rows(*b) for b in blocks)
     A = torch.empty((rows, mol.n_aux), dtype=torch.float64, device="cuda")
     B = torch.empty_like(A)
-    for thr, bound in ((1e-6, 1e-10), (0.0, 0.0)):
-        stats = plan.setup_mixed(arrays, q, threshold=thr)
-        print(f"case threshold={thr} split={stats}", flush=True)
-        if thr > 0:
+    for thr, bound, gran in ((1e-6, 1e-10, "pair"), (1e-6, 1e-10, "item"), (0.0, 0.0, "pair"), (0.0, 0.0, "item")):
+        stats = plan.setup_mixed(arrays, q, threshold=thr, granularity=gran)
+        print(f"case threshold={thr} granularity={gran} split={stats}", flush=True)
+        if thr > 0 and gran == "pair":
             assert stats["f32_triplets"] > 0
-        else:
+        if thr == 0:
             assert stats["f32_triplets"] == 0
         for b in blocks:
             n = plan.block_rows(*b)
