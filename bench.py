@@ -230,6 +230,72 @@ def serving_leg(n_jobs, rate, budget=200e9, mixed=True):
     return res
 
 
+def batched_leg(budget=200e9, n_mol=256):
+    """Batched workload: 256 molecules, atom counts lognormal (median 20, sigma 0.5, clipped 4-120), plain
+    generators (orbital l <= 2, auxiliary l <= 3), box scaled with the atom count.  Molecules are grouped
+    in arrival order into waves whose outputs fit the device budget; every wave is one BatchPlan (one
+    launch per class and component chunk for all its molecules, plus their (P|Q)), captured once in a
+    HIP graph and replayed; each wave's outputs are reduced to a checksum on the device, then freed."""
+    import concurrent.futures as cf
+    import torch
+    from electron_integral_playground_amd import generators, pair_utility
+    from electron_integral_playground_amd.engine import BatchPlan
+    t0 = time.perf_counter()
+    sizes = generators.batched_sizes(n_mol=n_mol)
+
+    def prep(i):
+        mol = generators.batched_molecule(i, int(sizes[i]))
+        return mol, pair_utility.form_pair_data(mol, 1e-14), pair_utility.form_auxiliary_pair_data(mol)
+
+    with cf.ThreadPoolExecutor(8) as ex:
+        mols = list(ex.map(prep, range(n_mol)))
+    t_host = time.perf_counter() - t0
+    out_bytes = [8 * (m.n_ao * (m.n_ao + 1) // 2 * m.n_aux + m.n_aux ** 2) for m, _, _ in mols]
+    waves, cur, acc = [], [], 0
+    for i, b in enumerate(out_bytes):
+        if cur and acc + b > budget:
+            waves.append(cur)
+            cur, acc = [], 0
+        cur.append(i)
+        acc += b
+    waves.append(cur)
+    stream = torch.cuda.Stream()
+    t_dev = 0.0
+    checksum = 0.0
+    for w in waves:
+        bp = BatchPlan([mols[i] for i in w])
+        out = bp.new_output()
+        out2 = bp.new_two_center_output()
+        stream.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(stream):
+            bp.run(out)                                # warm-up outside capture
+            bp.two_center(out2)
+        torch.cuda.current_stream().wait_stream(stream)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            bp.run(out)
+            bp.two_center(out2)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        g.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        t_dev += e0.elapsed_time(e1) * 1e-3
+        checksum += float(out.sum().item()) + float(out2.sum().item())
+        del g, out, out2, bp
+        torch.cuda.empty_cache()
+    n_int = sum(m.n_aux * m.n_ao * (m.n_ao + 1) // 2 for m, _, _ in mols)
+    res = {"workload": f"batched: {n_mol} molecules, atoms lognormal(median 20, sigma 0.5) in [4, 120], "
+                       "3s2p1d / aux s-f", "molecules": n_mol, "atoms_min_median_max": [int(sizes.min()),
+                       float(np.median(sizes)), int(sizes.max())],
+           "waves": len(waves), "three_center_integrals": n_int,
+           "integrals_per_s_device": n_int / t_dev, "device_s": t_dev, "wall_s_incl_setup": time.perf_counter() - t0,
+           "host_pair_lists_s": t_host, "checksum": checksum}
+    log(f"batched leg: {res}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -241,6 +307,8 @@ def main():
     ap.add_argument("--mixed-only", action="store_true", help="run only the mixed-precision workload")
     ap.add_argument("--mixed-atoms", type=int, default=None, help="atoms of the mixed workload (default 200)")
     ap.add_argument("--serving", action="store_true", help="run only the serving workload")
+    ap.add_argument("--batched-only", action="store_true", help="run only the batched workload")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched workload")
     ap.add_argument("--serving-jobs", type=int, default=2000)
     ap.add_argument("--serving-rate", type=float, default=1.0, help="Poisson arrival rate, jobs/s")
     ap.add_argument("--serving-plain", action="store_true", help="serve in plain float64 instead of mixed")
@@ -262,6 +330,9 @@ def main():
     if args.serving:
         print(json.dumps({"serving": serving_leg(args.serving_jobs, args.serving_rate,
                                                  mixed=not args.serving_plain)}), flush=True)
+        return
+    if args.batched_only:
+        print(json.dumps({"batched": batched_leg()}), flush=True)
         return
     if args.mixed_only:
         print(json.dumps({"mixed_precision": mixed_leg(n_atom=args.mixed_atoms, tune=args.tune)}), flush=True)
@@ -342,11 +413,13 @@ def main():
         cpu, err = cpu_baseline(mol, pairs, aux, gpu_packed)
         log(f"cpu baseline {cpu}; sample max |GPU - reference| = {err:.3e}")
 
-    mixed = None
+    mixed = batched = None
+    del out, plan
+    torch.cuda.empty_cache()
     if not args.no_mixed:
-        del out, plan
-        torch.cuda.empty_cache()
         mixed = mixed_leg(n_atom=args.mixed_atoms)
+    if not args.no_batched:
+        batched = batched_leg()
     line = {"metric": "3c integrals/s (plain FP64, device-resident packed output)", "value": value,
             "unit": "3c integrals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -358,6 +431,7 @@ def main():
             "roofline": roofline, "cpu_baseline": cpu,
             "three_center_variants": variants,
             "mixed_precision": mixed,
+            "batched": batched,
             "max_abs_err_vs_reference_sample": err}
     print(json.dumps(line), flush=True)
 

@@ -565,6 +565,24 @@ class BatchPlan:
                     self.items[(pi, ai)] = (T(np.stack([sp, np.concatenate(it_k0)], axis=1), torch.int32),
                                             T(np.concatenate(it_end), torch.int32))
                     self.n_triplet_items += len(sp)
+        self.mol_ks = mol_ks
+        # per molecule: packed rows that no kernel writes (shell pairs screened out entirely)
+        self.zero_rows = {}
+        for m, (mol, pd, _) in enumerate(molecules):
+            n_ao = mol.n_ao
+            written = np.zeros(n_ao * (n_ao + 1) // 2, dtype=bool)
+            for (la, lb), p in pd.items():
+                _, si, sj, _ = group_pairs(p)
+                i = (si[:, None, None] + np.arange(2 * la + 1)[None, :, None]).astype(np.int64)
+                j = (sj[:, None, None] + np.arange(2 * lb + 1)[None, None, :]).astype(np.int64)
+                hi, lo = np.maximum(i, j), np.minimum(i, j)
+                written[(hi * (hi + 1) // 2 + lo).ravel()] = True
+            rows = np.flatnonzero(~written).astype(np.int32)
+            if len(rows):
+                self.zero_rows[m] = T(rows, torch.int32)
+        # (P|Q) of every molecule: dense (n_aux, n_aux) blocks, one after another
+        n2 = [n_aux * n_aux for _, n_aux in self.sizes]
+        self.out2c_base = np.concatenate([[0], np.cumsum(n2)]).astype(np.int64)
         torch.cuda.current_stream().synchronize()
 
     def n_integrals(self):
@@ -577,9 +595,42 @@ class BatchPlan:
         n_ao, n_aux = self.sizes[m]
         return out[self.out_base[m]:self.out_base[m + 1]].view(n_ao * (n_ao + 1) // 2, n_aux)
 
+    def new_two_center_output(self):
+        return self.torch.zeros(int(self.out2c_base[-1]), dtype=self.torch.float64, device=self.device)
+
+    def molecule_two_center(self, out2c, m):
+        n_aux = self.sizes[m][1]
+        return out2c[self.out2c_base[m]:self.out2c_base[m + 1]].view(n_aux, n_aux)
+
+    def two_center(self, out2c, stream=None):
+        """Float64 (P|Q) of every molecule into its block of out2c (all blocks of the matrix are written)."""
+        h = _stream_handle(stream)
+        for m, (_, n_aux) in enumerate(self.sizes):
+            base = out2c.data_ptr() + 8 * int(self.out2c_base[m])
+            for a, (la, _, da) in enumerate(self.aux):
+                for b, (lb, _, db) in enumerate(self.aux):
+                    if la > lb or (la, m) not in self.mol_ks or (lb, m) not in self.mol_ks:
+                        continue
+                    a0, a1 = self.mol_ks[(la, m)]
+                    b0, b1 = self.mol_ks[(lb, m)]
+                    rc = self.lib.eip_two_center_device(
+                        la, lb, _vp(da["off"].data_ptr() + 4 * a0), _ptr(da["C"]), _ptr(da["c"]),
+                        _vp(da["col"].data_ptr() + 4 * a0), a1 - a0, _vp(db["off"].data_ptr() + 4 * b0), _ptr(db["C"]),
+                        _ptr(db["c"]), _vp(db["col"].data_ptr() + 4 * b0), b1 - b0, int(la == lb), 1, _vp(base),
+                        _c_ll(n_aux), _c_double(self.omega), h)
+                    if rc != 0:
+                        raise RuntimeError(f"eip_two_center_device({la},{lb}) failed with {rc}")
+        return out2c
+
     def run(self, out, stream=None, zero=True):
+        """(ij|P) of every molecule into its packed block of out; rows of screened-out shell pairs are
+        zeroed by a kernel (no memset: the launch sequence can be captured in a graph)."""
         if zero:
-            out.zero_()
+            for m, rows in self.zero_rows.items():
+                rc = self.lib.eip_zero_rows(_ptr(rows), int(rows.numel()), _c_ll(self.sizes[m][1]),
+                                            _vp(out.data_ptr() + 8 * int(self.out_base[m])), _stream_handle(stream))
+                if rc != 0:
+                    raise RuntimeError(f"eip_zero_rows failed with {rc}")
         for (pi, ai), (it, end) in self.items.items():
             (la, lb), _, _, dp = self.pairs[pi]
             lc, n_ks, da = self.aux[ai]

@@ -102,14 +102,22 @@ def test_batch_plan_matches_single_molecule_plans():
     torch.cuda.synchronize()
     with torch.cuda.graph(g):
         batch.run(out)
+    singles = [ThreeCenterPlan(mol, pd, ad) for mol, pd, ad in mols]
+    ref3 = [p.run() for p in singles]
+    ref2 = [p.two_center() for p in singles]
+    torch.cuda.synchronize()
     for replay in range(3):
         out.fill_(float("nan"))
         g.replay()
         torch.cuda.synchronize()
-        for m, (mol, pd, ad) in enumerate(mols):
-            single = ThreeCenterPlan(mol, pd, ad).run()
-            torch.cuda.synchronize()
-            assert torch.equal(batch.molecule_output(out, m), single), (replay, m)
+        for m in range(len(mols)):
+            assert torch.equal(batch.molecule_output(out, m), ref3[m]), (replay, m)
+        # eager work between replays, as a caller would have
+        torch.cuda.synchronize()
+    out2 = batch.two_center(batch.new_two_center_output())
+    torch.cuda.synchronize()
+    for m in range(len(mols)):
+        assert torch.equal(batch.molecule_two_center(out2, m), ref2[m]), m
 
 
 def _block_plan(n_atom=10, box=8.0):
