@@ -198,13 +198,15 @@ def mixed_leg(threshold=1e-6, block_bytes=40e9, n_atom=None, tune=None):
     return res
 
 
-def serving_leg(n_jobs, rate, budget=200e9, mixed=True):
+def serving_leg(n_jobs, rate, budget=200e9, mixed=True, part=0, latency_file=None):
     """Poisson stream of integral jobs (atom counts lognormal, median 40, clipped 10-250; 4s3p2d1f
     orbital and s-g auxiliary generators) through the continuous-batching server."""
     import torch
     from electron_integral_playground_amd import generators
     from electron_integral_playground_amd.serving import IntegralServer, run_poisson
-    jobs = generators.serving_jobs(n_jobs=n_jobs, rate=rate)
+    # part k of a longer stream: its own arrival/size seed and job seeds
+    jobs = generators.serving_jobs(n_jobs=n_jobs, rate=rate, seed=generators.SEED_BASE + 5 + 1000 * part)
+    jobs = [(t, n, seed + 100000 * part) for t, n, seed in jobs]
     # warm-up: one small job through a throw-away server (kernel loading, allocator)
     warm = IntegralServer(budget_bytes=budget, mixed=mixed)
     warm.submit(0, generators.serving_molecule(10, 1))
@@ -226,6 +228,9 @@ def serving_leg(n_jobs, rate, budget=200e9, mixed=True):
            "steps": server.n_steps, "launches": server.n_launches,
            "median_atoms": float(np.median([n for _, n, _ in jobs])),
            "atoms_range": [int(min(n for _, n, _ in jobs)), int(max(n for _, n, _ in jobs))]}
+    if latency_file:
+        pathlib.Path(latency_file).write_text(json.dumps({"part": part, "latencies_s": lat.tolist(),
+                                                          "integrals": n_int, "span_s": span}))
     log(f"serving: {res}")
     return res
 
@@ -312,6 +317,8 @@ def main():
     ap.add_argument("--serving-jobs", type=int, default=2000)
     ap.add_argument("--serving-rate", type=float, default=1.0, help="Poisson arrival rate, jobs/s")
     ap.add_argument("--serving-plain", action="store_true", help="serve in plain float64 instead of mixed")
+    ap.add_argument("--serving-part", type=int, default=0, help="part index of a stream run in several calls")
+    ap.add_argument("--serving-latencies", metavar="JSON", help="write every job's latency to this file")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 4 GiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -328,8 +335,9 @@ def main():
 
     torch.cuda.set_device(0)
     if args.serving:
-        print(json.dumps({"serving": serving_leg(args.serving_jobs, args.serving_rate,
-                                                 mixed=not args.serving_plain)}), flush=True)
+        print(json.dumps({"serving": serving_leg(args.serving_jobs, args.serving_rate, mixed=not args.serving_plain,
+                                                 part=args.serving_part, latency_file=args.serving_latencies)}),
+              flush=True)
         return
     if args.batched_only:
         print(json.dumps({"batched": batched_leg()}), flush=True)

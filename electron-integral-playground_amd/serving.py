@@ -39,6 +39,36 @@ def _q_aux(plan, J2c):
     return out
 
 
+class OutputArena:
+    """One device allocation for all unit outputs, carved first-fit (no allocator fragmentation between
+    jobs of different sizes).  Offsets and sizes are in float64 elements."""
+
+    def __init__(self, torch, n_elems, device):
+        self.buf = torch.empty(int(n_elems), dtype=torch.float64, device=device)
+        self.free = [(0, int(n_elems))]            # sorted, non-adjacent (offset, size)
+
+    def alloc(self, n):
+        for k, (off, size) in enumerate(self.free):
+            if size >= n:
+                if size == n:
+                    self.free.pop(k)
+                else:
+                    self.free[k] = (off + n, size - n)
+                return off
+        return None
+
+    def release(self, off, n):
+        self.free.append((off, n))
+        self.free.sort()
+        merged = []
+        for o, sz in self.free:
+            if merged and merged[-1][0] + merged[-1][1] == o:
+                merged[-1] = (merged[-1][0], merged[-1][1] + sz)
+            else:
+                merged.append((o, sz))
+        self.free = merged
+
+
 @dataclass
 class Unit:
     job: "Job"
@@ -46,7 +76,8 @@ class Unit:
     rows: int                     # compact output rows
     bytes: int
     row_off: List[int]            # first output row of each block
-    out: object = None            # device tensor (rows, n_aux) while in flight / kept
+    out: object = None            # device tensor (rows, n_aux) while in flight / kept (a view of the arena)
+    arena_off: int = -1
     checksum: object = None
 
 
@@ -95,6 +126,8 @@ class IntegralServer:
         self.in_flight = deque()      # (event, [units])
         self.inputs_used = 0.0        # bytes of resident job inputs (pair tables, metric)
         self.outputs_used = 0.0       # bytes of unit outputs in flight (or kept)
+        # unit outputs live in one arena: the budget minus the share reserved for resident inputs
+        self.arena = OutputArena(torch, (1.0 - INPUT_FRACTION) * self.budget // 8, self.device)
         self.jobs: Dict[int, Job] = {}
         self.completed: List[Job] = []
         self.n_launches = 0
@@ -194,14 +227,18 @@ class IntegralServer:
 
     # -- steps ----------------------------------------------------------------------------------------
     def _form_step(self):
-        """Pending units (FIFO) whose outputs fit the free budget."""
-        units, free = [], self.budget - self.inputs_used - self.outputs_used
-        if not self.in_flight and self.outputs_used == 0 and self.pending_units and self.pending_units[0].bytes > free:
-            # progress guarantee for a single job whose inputs alone crowd the budget
-            free = self.pending_units[0].bytes
-        while self.pending_units and self.pending_units[0].bytes <= free:
-            u = self.pending_units.popleft()
-            free -= u.bytes
+        """Pending units (FIFO) whose outputs the arena can place.  A unit is at most UNIT_CAP_FRACTION of
+        the budget and the arena is (1 - INPUT_FRACTION) of it, so the head unit always fits once the
+        steps in flight have retired."""
+        units = []
+        while self.pending_units:
+            u = self.pending_units[0]
+            off = self.arena.alloc(u.rows * u.job.n_aux)
+            if off is None:
+                break
+            self.pending_units.popleft()
+            u.arena_off = off
+            u.out = self.arena.buf[off:off + u.rows * u.job.n_aux].view(u.rows, u.job.n_aux)
             units.append(u)
         return units
 
@@ -210,7 +247,6 @@ class IntegralServer:
         step_bytes = 0
         with torch.cuda.stream(self.compute_stream):
             for u in units:
-                u.out = torch.empty((u.rows, u.job.n_aux), dtype=torch.float64, device=self.device)
                 step_bytes += u.bytes
             self.outputs_used += step_bytes
             # per angular class: items of every unit, one job-table entry per (unit, pair class, aux class)
@@ -278,10 +314,12 @@ class IntegralServer:
             _, units, _ = self.in_flight.popleft()
             with torch.cuda.stream(self.compute_stream):
                 for u in units:
-                    u.checksum = torch.stack([u.out.sum(), (u.out * u.out).sum()])
+                    # consumed on the device: sum and 2-norm (reductions, no full-size temporaries)
+                    u.checksum = torch.stack([u.out.sum(), torch.linalg.vector_norm(u.out)])
                     if not self.keep_outputs:
                         u.out = None
-                    if not self.keep_outputs:
+                        # stream-ordered reuse: later launches on this stream run after the checksum
+                        self.arena.release(u.arena_off, u.rows * u.job.n_aux)
                         self.outputs_used -= u.bytes
                     job = u.job
                     job.units_done += 1
