@@ -62,6 +62,8 @@ def pair_stride(la, lb, spherical):
 # one is contracted, and the contraction reads wave-uniform LDS broadcasts instead of waiting on scalar
 # loads.
 STAGE_LDS = True
+STAGE_BATCH_BYTES = 2048      # rows of small classes are staged this many bytes at a time ...
+STAGE_MAX_ROWS = 16           # ... but at most this many rows (P of each row: 32 bytes, 2 lanes)
 
 
 # ---------------------------------------------------------------------------------------------
@@ -256,21 +258,24 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64"):
     stride = pair_stride(la, lb, spherical)
     esize = 4 if f32 else 8
     row_bytes = stride * esize
-    n_pieces = (row_bytes + 1023) // 1024                  # 64 lanes x 16 bytes per global->LDS load
+    # K consecutive rows per buffer (small classes): one wait per K primitive pairs instead of per pair
+    K = max(1, min(STAGE_MAX_ROWS, STAGE_BATCH_BYTES // row_bytes))
+    n_pieces = (K * row_bytes + 1023) // 1024              # 64 lanes x 16 bytes per global->LDS load
     if STAGE_LDS:
         out.append(f"    __shared__ __attribute__((aligned(16))) {ty} sE[2][{n_pieces * 1024 // esize}];")
-        out.append("    __shared__ __attribute__((aligned(16))) double sP[2][4];")
+        out.append(f"    __shared__ __attribute__((aligned(16))) double sP[2][{4 * K}];")
         out.append("    const int lane = (int) threadIdx.x;")
         out.append(f"    const char* __restrict__ Ebase = (const char*) JD({'E32' if f32 else 'E'});")
         out.append("    const char* __restrict__ Pbase = (const char*) JD(pp_P);")
-        # stage(pp, buf): issue the asynchronous loads of row pp into buffer buf
-        out.append("#define EIP_STAGE(PP, BUF) do { \\")
+        # stage(pp, nrows, buf): issue the asynchronous loads of rows [pp, pp + nrows) into buffer buf
+        out.append("#define EIP_STAGE(PP, NR, BUF) do { \\")
         out.append(f"        const char* src_ = Ebase + (size_t) (PP) * {row_bytes} + 16 * lane; \\")
+        out.append(f"        const int nbytes_ = (NR) * {row_bytes}; \\")
         for r in range(n_pieces):
-            cond = "" if (r + 1) * 1024 <= row_bytes else f"if (16 * lane < {row_bytes - r * 1024}) "
+            cond = "" if (r + 1) * 1024 <= row_bytes else f"if (16 * lane + {r * 1024} < nbytes_) "
             out.append(f"        {cond}__builtin_amdgcn_global_load_lds((const void*) (src_ + {r * 1024}), "
                        f"(__attribute__((address_space(3))) void*) ((char*) &sE[BUF][0] + {r * 1024}), 16, 0, 0); \\")
-        out.append("        if (lane < 2) __builtin_amdgcn_global_load_lds((const void*) (Pbase + (size_t) (PP) * 32 + 16 * lane), "
+        out.append("        if (lane < 2 * (NR)) __builtin_amdgcn_global_load_lds((const void*) (Pbase + (size_t) (PP) * 32 + 16 * lane), "
                    "(__attribute__((address_space(3))) void*) &sP[BUF][0], 16, 0, 0); \\")
         out.append("    } while (0)")
     if STAGE_LDS:
@@ -288,8 +293,8 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64"):
         out.append("      const double Cx = eip::ro(JD(ka_C))[4 * kp + 0], Cy = eip::ro(JD(ka_C))[4 * kp + 1], Cz = eip::ro(JD(ka_C))[4 * kp + 2];")
         out.append("      const double q = eip::ro(JD(ka_C))[4 * kp + 3];")
         out.append("      const double ck = eip::ro(JD(ka_coef))[kp];")
-    if STAGE_LDS:
-        out.append("      if (pp0 < pp1) EIP_STAGE(pp0, 0);")
+    if STAGE_LDS and K == 1:
+        out.append("      if (pp0 < pp1) EIP_STAGE(pp0, 1, 0);")
         out.append("      for (int pp = pp0, buf = 0; pp < pp1; ++pp, buf ^= 1) {")
         # vmcnt(0): the asynchronous global->LDS loads of row pp have landed (the compiler does not
         # track them across the loop back-edge); the barrier orders the LDS accesses.
@@ -297,6 +302,15 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64"):
         out.append("        __syncthreads();")
         out.append(f"        const {ty}* __restrict__ E = &sE[buf][0];")
         out.append("        const double Px = sP[buf][0], Py = sP[buf][1], Pz = sP[buf][2], p = sP[buf][3];")
+    elif STAGE_LDS:
+        out.append(f"      if (pp0 < pp1) EIP_STAGE(pp0, min({K}, pp1 - pp0), 0);")
+        out.append(f"      for (int b0 = pp0, buf = 0; b0 < pp1; b0 += {K}, buf ^= 1) {{")
+        out.append("        __builtin_amdgcn_s_waitcnt(0x0F70);")
+        out.append("        __syncthreads();")
+        out.append(f"        const int nr = min({K}, pp1 - b0);")
+        out.append("        for (int r = 0; r < nr; ++r) {")
+        out.append(f"        const {ty}* __restrict__ E = &sE[buf][r * {stride}];")
+        out.append("        const double Px = sP[buf][4 * r + 0], Py = sP[buf][4 * r + 1], Pz = sP[buf][4 * r + 2], p = sP[buf][4 * r + 3];")
     else:
         out.append("      for (int pp = pp0; pp < pp1; ++pp) {")
         out.append(f"        const auto E = eip::ro(JD({'E32' if f32 else 'E'})) + (size_t) pp * {stride};")
@@ -325,8 +339,10 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64"):
     out.append(f"        const {ty} T = ({ty}) (zeta * (dPQx * dPQx + dPQy * dPQy + dPQz * dPQz));")
     out.append(f"        {ty} F[{L + 1}];")
     out.append(f"        eip::boys{'32' if f32 else '64'}<{L}>(T, F);")
-    if STAGE_LDS:
-        out.append("        if (pp + 1 < pp1) EIP_STAGE(pp + 1, buf ^ 1);   // after the Boys table loads: their wait excludes it")
+    if STAGE_LDS and K == 1:
+        out.append("        if (pp + 1 < pp1) EIP_STAGE(pp + 1, 1, buf ^ 1);   // after the Boys table loads: their wait excludes it")
+    elif STAGE_LDS:
+        out.append(f"        if (r == 0 && b0 + {K} < pp1) EIP_STAGE(b0 + {K}, min({K}, pp1 - b0 - {K}), buf ^ 1);")
     out.append(f"        const {ty} m2z = ({ty}) (-2.0 * zeta);")
     out.append(f"        const {ty} R000_0 = ({ty}) pf * F[0];")
     for n in range(1, L + 1):
@@ -356,6 +372,8 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64"):
             out.append("        }")
             k += 1
     out.append("      }")
+    if STAGE_LDS and K > 1:
+        out.append("      }")
     out.append("    }")
     out.append("    if (!active) return;")
     out.append("    const int col = eip::ro(JD(ks_col))[ksh];")

@@ -176,6 +176,25 @@ int prepare_pairs(DevPairs& d, int la, int lb, bool sph, const double* P_p, cons
 }
 
 // Zero whole rows of a row-major matrix: one workgroup per listed row, 16-byte stores.
+// Zero spans of doubles: span b = [spans[2b], spans[2b] + spans[2b+1]), each at most 8192 doubles and
+// starting on a 16-byte boundary when the output is; one block per span, 16-byte non-temporal stores.
+__global__ void __launch_bounds__(256) zero_spans_kernel(const long long* __restrict__ spans, int n_spans,
+                                                         double* __restrict__ out)
+{
+    for (int b = blockIdx.x; b < n_spans; b += gridDim.x) {
+        const long long off = spans[2 * b], n = spans[2 * b + 1];
+        double* p = out + off;
+        const long long head = (reinterpret_cast<uintptr_t>(p) & 15) ? 1 : 0;
+        if (head && threadIdx.x == 0 && n > 0) p[0] = 0.0;
+        typedef double zero2 __attribute__((ext_vector_type(2)));
+        zero2* q = reinterpret_cast<zero2*>(p + head);
+        const long long n2 = (n - head) / 2;
+        const zero2 z = {0.0, 0.0};
+        for (long long k = threadIdx.x; k < n2; k += blockDim.x) __builtin_nontemporal_store(z, q + k);
+        if (((n - head) & 1) && threadIdx.x == 0) p[n - 1] = 0.0;
+    }
+}
+
 __global__ void __launch_bounds__(256) zero_rows_kernel(const int* __restrict__ rows, int n_rows, long long ld,
                                                         double* __restrict__ out)
 {
@@ -567,6 +586,14 @@ int eip_zero_rows(const int* d_rows, int n_rows, long long ld, double* d_out, vo
     if (n_rows <= 0) return 0;
     const int grid = n_rows < 65536 ? n_rows : 65536;
     hipLaunchKernelGGL(zero_rows_kernel, dim3(grid), dim3(256), 0, (hipStream_t) stream, d_rows, n_rows, ld, d_out);
+    return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int eip_zero_spans(const long long* d_spans, int n_spans, double* d_out, void* stream)
+{
+    if (n_spans <= 0) return 0;
+    const int grid = n_spans < 65536 ? n_spans : 65536;
+    hipLaunchKernelGGL(zero_spans_kernel, dim3(grid), dim3(256), 0, (hipStream_t) stream, d_spans, n_spans, d_out);
     return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 

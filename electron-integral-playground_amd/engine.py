@@ -92,6 +92,22 @@ def _ranges(counts, first):
 DIFFUSE_PAIR_EXPONENT = 0.2
 
 
+def zero_spans(rows, ld, max_len=8192):
+    """Sorted packed rows -> (offset, count) spans of doubles covering them, consecutive rows merged and
+    spans cut at max_len (one block each)."""
+    rows = np.asarray(rows, dtype=np.int64)
+    if rows.size == 0:
+        return np.zeros((0, 2), dtype=np.int64)
+    brk = np.flatnonzero(np.diff(rows) != 1) + 1
+    starts = rows[np.concatenate([[0], brk])]
+    lens = np.diff(np.concatenate([[0], brk, [rows.size]]))
+    off, cnt = starts * ld, lens * ld
+    pieces = -(-cnt // max_len)
+    rep_off = np.repeat(off, pieces) + max_len * (np.arange(pieces.sum()) - np.repeat(np.cumsum(pieces) - pieces, pieces))
+    end = np.repeat(off + cnt, pieces)
+    return np.stack([rep_off, np.minimum(end - rep_off, max_len)], axis=1)
+
+
 def group_pairs(pd):
     """Stable grouping of one class's primitive pairs by (i_ao_start, j_ao_start); within a shell pair
     the diffuse pairs (p < DIFFUSE_PAIR_EXPONENT) come first, each part in list order (descending bound)."""
@@ -226,6 +242,7 @@ class ThreeCenterPlan:
             hi, lo = np.maximum(i, j), np.minimum(i, j)
             written[(hi * (hi + 1) // 2 + lo).ravel()] = True
         self.zero_rows = T(np.flatnonzero(~written).astype(np.int32), torch.int32)
+        self.zero_spans = T(zero_spans(np.flatnonzero(~written), self.n_aux), torch.int64)
         self._job_tables = {}
         torch.cuda.current_stream().synchronize()
 
@@ -277,10 +294,10 @@ class ThreeCenterPlan:
     def zero_unwritten(self, out, stream=None):
         """Zero the packed rows of shell pairs that are screened out (every other element is written
         exactly once by the class kernels)."""
-        rc = self.lib.eip_zero_rows(_ptr(self.zero_rows), int(self.zero_rows.numel()), _c_ll(self.n_aux), _ptr(out),
-                                    _stream_handle(stream))
+        rc = self.lib.eip_zero_spans(_ptr(self.zero_spans), int(self.zero_spans.shape[0]), _ptr(out),
+                                     _stream_handle(stream))
         if rc != 0:
-            raise RuntimeError(f"eip_zero_rows failed with {rc}")
+            raise RuntimeError(f"eip_zero_spans failed with {rc}")
 
     def run(self, out=None, stream=None, zero=True):
         """All classes into ``out`` (packed layout); rows of screened-out shell pairs are zero."""
@@ -577,9 +594,13 @@ class BatchPlan:
                 j = (sj[:, None, None] + np.arange(2 * lb + 1)[None, None, :]).astype(np.int64)
                 hi, lo = np.maximum(i, j), np.minimum(i, j)
                 written[(hi * (hi + 1) // 2 + lo).ravel()] = True
-            rows = np.flatnonzero(~written).astype(np.int32)
+            rows = np.flatnonzero(~written)
             if len(rows):
-                self.zero_rows[m] = T(rows, torch.int32)
+                sp = zero_spans(rows, mol.n_aux)
+                sp[:, 0] += int(self.out_base[m])
+                self.zero_rows[m] = sp
+        self.zero_spans = T(np.concatenate(list(self.zero_rows.values())) if self.zero_rows
+                            else np.zeros((0, 2), np.int64), torch.int64)
         # (P|Q) of every molecule: dense (n_aux, n_aux) blocks, one after another
         n2 = [n_aux * n_aux for _, n_aux in self.sizes]
         self.out2c_base = np.concatenate([[0], np.cumsum(n2)]).astype(np.int64)
@@ -625,12 +646,11 @@ class BatchPlan:
     def run(self, out, stream=None, zero=True):
         """(ij|P) of every molecule into its packed block of out; rows of screened-out shell pairs are
         zeroed by a kernel (no memset: the launch sequence can be captured in a graph)."""
-        if zero:
-            for m, rows in self.zero_rows.items():
-                rc = self.lib.eip_zero_rows(_ptr(rows), int(rows.numel()), _c_ll(self.sizes[m][1]),
-                                            _vp(out.data_ptr() + 8 * int(self.out_base[m])), _stream_handle(stream))
-                if rc != 0:
-                    raise RuntimeError(f"eip_zero_rows failed with {rc}")
+        if zero and self.zero_spans.shape[0]:
+            rc = self.lib.eip_zero_spans(_ptr(self.zero_spans), int(self.zero_spans.shape[0]), _ptr(out),
+                                         _stream_handle(stream))
+            if rc != 0:
+                raise RuntimeError(f"eip_zero_spans failed with {rc}")
         for (pi, ai), (it, end) in self.items.items():
             (la, lb), _, _, dp = self.pairs[pi]
             lc, n_ks, da = self.aux[ai]

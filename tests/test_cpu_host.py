@@ -227,3 +227,20 @@ def test_mixed_split_matches_brute_force():
                 f64 = (p[sel] < DIFFUSE_PAIR_EXPONENT) | (ub[sel] * qmax >= thr)
                 k = s[sp, g]
                 assert f64[:k].all() and not f64[k:].any(), (thr, g, sp)
+
+
+def test_zero_spans_cover_exactly_the_rows():
+    from electron_integral_playground_amd.engine import zero_spans
+    rng = np.random.default_rng(3)
+    for ld in (1, 7, 10240):
+        rows = np.unique(rng.integers(0, 400, 150))
+        sp = zero_spans(rows, ld, max_len=1000)
+        assert np.all(sp[:, 1] > 0) and np.all(sp[:, 1] <= 1000)
+        covered = np.zeros(400 * ld, dtype=int)
+        for off, n in sp:
+            covered[off:off + n] += 1
+        want = np.zeros(400 * ld, dtype=int)
+        for r in rows:
+            want[r * ld:(r + 1) * ld] = 1
+        assert np.array_equal(covered, want)
+    assert zero_spans([], 5).shape == (0, 2)
