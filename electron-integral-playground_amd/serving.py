@@ -16,6 +16,7 @@ does not depend on which other jobs share its launches.
 """
 
 import concurrent.futures as cf
+import threading
 import time
 from collections import deque
 from dataclasses import dataclass, field
@@ -28,6 +29,7 @@ from .engine import ThreeCenterPlan
 
 UNIT_CAP_FRACTION = 0.25      # a unit's output may use at most this share of the budget
 INPUT_FRACTION = 0.25         # admitted jobs' resident inputs (pair tables, metric) use at most this share
+BYPASS_SECONDS = 5.0          # a blocked head unit may be overtaken by smaller units for this long
 
 
 def _q_aux(plan, J2c):
@@ -79,6 +81,7 @@ class Unit:
     out: object = None            # device tensor (rows, n_aux) while in flight / kept (a view of the arena)
     arena_off: int = -1
     checksum: object = None
+    blocked_since: Optional[float] = None
 
 
 @dataclass
@@ -118,7 +121,8 @@ class IntegralServer:
         self.max_steps_in_flight = max_steps_in_flight
         self.device = torch.device(device)
         self.compute_stream = torch.cuda.Stream(device=self.device)
-        self.setup_stream = torch.cuda.Stream(device=self.device)
+        self._tls = threading.local()                # one setup stream per preparation thread
+        self._inputs_cv = threading.Condition()
         self.pool = cf.ThreadPoolExecutor(prep_workers)
         self.log = log
         self.prepared = deque()       # host-prepared jobs awaiting device setup and admission
@@ -139,19 +143,33 @@ class IntegralServer:
         are built on a host thread."""
         job = Job(job_id, time.perf_counter() if arrival is None else arrival, molecule)
         self.jobs[job_id] = job
-        fut = self.pool.submit(self._prepare_host, job)
+        fut = self.pool.submit(self._prepare, job)
         self.prepared.append((job, fut))
         return job
 
-    @staticmethod
-    def _prepare_host(job):
+    def _prepare(self, job):
+        """Host thread: geometry/basis, pair tables, then -- once the job's resident inputs fit the input
+        budget -- device setup on this thread's own stream.  Returns nothing; errors are re-raised
+        through the future."""
         if callable(job.molecule):
             job.molecule = job.molecule()
         mol = job.molecule
         arrays = pair_utility.form_pair_arrays(mol, pair_utility.REFERENCE_SCREENING_THRESHOLD)
         pairs = pair_utility.form_primitive_pair_data(mol, arrays)
         aux = pair_utility.form_auxiliary_pair_data(mol)
-        return arrays, pairs, aux
+        host = (arrays, pairs, aux)
+        est = self.estimate_input_bytes(mol, host)
+        with self._inputs_cv:
+            # a job is always admitted when no other job's inputs are resident
+            self._inputs_cv.wait_for(lambda: self.inputs_used == 0
+                                     or self.inputs_used + est <= INPUT_FRACTION * self.budget)
+            self.inputs_used += est
+        try:
+            self._setup_device(job, host)
+        finally:
+            with self._inputs_cv:
+                self.inputs_used += (job.input_bytes if job.plan is not None else 0) - est
+                self._inputs_cv.notify_all()
 
     # -- footprint --------------------------------------------------------------------------------
     @staticmethod
@@ -170,13 +188,16 @@ class IntegralServer:
         arrays, pairs, aux = host
         mol = job.molecule
         job.n_ao, job.n_aux = mol.n_ao, mol.n_aux
-        with torch.cuda.stream(self.setup_stream):
+        stream = getattr(self._tls, "stream", None)
+        if stream is None:
+            stream = self._tls.stream = torch.cuda.Stream(device=self.device)
+        with torch.cuda.stream(stream):
             plan = ThreeCenterPlan(mol, pairs, aux, device=self.device)
             job.J2c = plan.two_center()
-            q = _q_aux(plan, job.J2c)                     # synchronises the setup stream only
+            q = _q_aux(plan, job.J2c)                     # synchronises this thread's stream only
             if self.mixed:
                 plan.setup_mixed(arrays, q, threshold=self.threshold, granularity=self.granularity)
-            self.setup_stream.synchronize()
+            stream.synchronize()
         job.plan = plan
         job.input_bytes = plan.device_bytes() + job.J2c.numel() * 8
         # units: the whole job, or blocks of shell pairs
@@ -202,27 +223,19 @@ class IntegralServer:
         return n_pp * (120 + 12 * 400) + molecule.n_aux ** 2 * 8
 
     def _admit(self):
-        """Device setup for prepared jobs, in arrival order, while their resident inputs stay within
-        INPUT_FRACTION of the budget (a job is always admitted when none is resident).  Output memory is
-        taken per unit when a step is formed; since inputs never exceed INPUT_FRACTION and a unit never
-        exceeds UNIT_CAP_FRACTION, the head unit always fits once the steps in flight have retired."""
+        """Jobs whose preparation (host and device) has finished join the pending units, in arrival
+        order."""
         while self.prepared:
             job, fut = self.prepared[0]
             if not fut.done():
                 break
-            host = fut.result()
-            est = self.estimate_input_bytes(job.molecule, host)
-            if self.inputs_used > 0 and self.inputs_used + est > INPUT_FRACTION * self.budget:
-                break
             self.prepared.popleft()
-            try:
-                self._setup_device(job, host)
-            except Exception as exc:                      # a job that cannot run is reported, not fatal
+            exc = fut.exception()
+            if exc is not None:                           # a job that cannot run is reported, not fatal
                 job.error = repr(exc)
                 job.completion = time.perf_counter()
                 self.completed.append(job)
                 continue
-            self.inputs_used += job.input_bytes
             self.pending_units.extend(job.units)
 
     # -- steps ----------------------------------------------------------------------------------------
@@ -230,16 +243,23 @@ class IntegralServer:
         """Pending units (FIFO) whose outputs the arena can place.  A unit is at most UNIT_CAP_FRACTION of
         the budget and the arena is (1 - INPUT_FRACTION) of it, so the head unit always fits once the
         steps in flight have retired."""
-        units = []
+        units, kept = [], deque()
+        now = time.perf_counter()
         while self.pending_units:
-            u = self.pending_units[0]
+            u = self.pending_units.popleft()
             off = self.arena.alloc(u.rows * u.job.n_aux)
             if off is None:
-                break
-            self.pending_units.popleft()
+                kept.append(u)
+                if u.blocked_since is None:
+                    u.blocked_since = now
+                if now - u.blocked_since > BYPASS_SECONDS:
+                    break                        # the oldest blocked unit has waited long enough: no bypass
+                continue                         # smaller units behind it may still fit
             u.arena_off = off
             u.out = self.arena.buf[off:off + u.rows * u.job.n_aux].view(u.rows, u.job.n_aux)
             units.append(u)
+        kept.extend(self.pending_units)
+        self.pending_units = kept
         return units
 
     def _launch_step(self, units):
@@ -329,7 +349,9 @@ class IntegralServer:
                         if not self.keep_outputs:
                             job.plan = None
                             job.J2c = None
-                            self.inputs_used -= job.input_bytes
+                            with self._inputs_cv:
+                                self.inputs_used -= job.input_bytes
+                                self._inputs_cv.notify_all()
 
     def poll(self):
         """One scheduling iteration: retire finished steps, admit prepared jobs, launch a step."""
