@@ -78,15 +78,6 @@ class AuxClass:
     dev: Dict[str, object] = field(default_factory=dict)
 
 
-def _ranges(counts, first):
-    """For each owner r, counts[r] consecutive items k0 = first[r] + 64 b: returns (owner, k0)."""
-    counts = np.asarray(counts, dtype=np.int64)
-    owner = np.repeat(np.arange(len(counts), dtype=np.int64), counts)
-    start = np.concatenate([[0], np.cumsum(counts)[:-1]])
-    b = np.arange(owner.size, dtype=np.int64) - np.repeat(start, counts)
-    return owner.astype(np.int32), (np.repeat(np.asarray(first, dtype=np.int64), counts) + 64 * b).astype(np.int32)
-
-
 # Primitive pairs with p = a + b below this exponent are "diffuse": the mixed-precision split always
 # evaluates them in float64.
 DIFFUSE_PAIR_EXPONENT = 0.2

@@ -171,17 +171,6 @@ class IntegralServer:
                 self.inputs_used += (job.input_bytes if job.plan is not None else 0) - est
                 self._inputs_cv.notify_all()
 
-    # -- footprint --------------------------------------------------------------------------------
-    @staticmethod
-    def projected_bytes(molecule, pairs):
-        """Compact (ij|P) bytes: significant shell pairs' rows x n_aux float64."""
-        from .engine import group_pairs
-        rows = 0
-        for (la, lb), pd in pairs.items():
-            _, si, _, _ = group_pairs(pd)
-            rows += len(si) * (2 * la + 1) * (2 * lb + 1)
-        return rows * molecule.n_aux * 8
-
     # -- device setup and admission -----------------------------------------------------------------
     def _setup_device(self, job, host):
         torch = self.torch
