@@ -135,7 +135,21 @@ def mixed_leg(threshold=1e-6, block_bytes=40e9, n_atom=None, tune=None):
     torch.cuda.synchronize()
     t_metric = e0.elapsed_time(e1) * 1e-3
     q_aux = q_aux_from_metric(plan, J2c)
+    # the downstream factorisation of the metric, on the device in float64
+    from electron_integral_playground_amd.engine import metric_cholesky
+    chol = {}
+    try:
+        e0.record(stream)
+        L = metric_cholesky(J2c)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        chol = {"metric_cholesky_s": e0.elapsed_time(e1) * 1e-3,
+                "metric_cholesky_residual": float(((L @ L.T) - J2c).abs().max().item() / J2c.abs().max().item())}
+        del L
+    except ValueError as exc:
+        chol = {"metric_cholesky": str(exc)}
     del J2c
+    torch.cuda.empty_cache()
     if tune:
         from electron_integral_playground_amd.engine import autotune
         autotune(plan, save=tune, log=log, block_bytes=4e9)
@@ -192,7 +206,7 @@ def mixed_leg(threshold=1e-6, block_bytes=40e9, n_atom=None, tune=None):
            "plain_fp64_integrals_per_s": n_int / t_plain, "mixed_integrals_per_s": n_int / t_mixed,
            "mixed_over_plain": t_plain / t_mixed, "plain_s": t_plain, "mixed_s": t_mixed,
            "max_abs_err_vs_fp64": max_abs.item(), "rms_err_vs_fp64": (sum_sq.item() / n_el) ** 0.5,
-           "max_abs_value": max_ref.item(), "metric_fp64_s": t_metric,
+           "max_abs_value": max_ref.item(), "metric_fp64_s": t_metric, **chol,
            "output": f"{n_el * 8 / 1e12:.2f} TB of significant-shell-pair rows, streamed in {len(blocks)} blocks"}
     log(f"mixed leg: {res}")
     return res

@@ -470,6 +470,17 @@ class ThreeCenterPlan:
         return out
 
 
+def metric_cholesky(J2c):
+    """Lower Cholesky factor L (L L^T = (P|Q)) of the float64 metric on the device (the solver library
+    through torch.linalg), the factorisation a density-fitting consumer applies to (P|ij).  Raises when
+    the metric is not numerically positive definite."""
+    import torch
+    L, info = torch.linalg.cholesky_ex(J2c, upper=False)
+    if int(info.item()) != 0:
+        raise ValueError(f"metric is not positive definite (leading minor {int(info.item())})")
+    return L
+
+
 def packed_to_dense(packed, n_ao):
     """(n_ao(n_ao+1)/2, n_aux) packed -> (n_aux, n_ao, n_ao) dense (reference layout), on the device."""
     import torch

@@ -41,6 +41,43 @@ def _q_aux(plan, J2c):
     return out
 
 
+class InputGate:
+    """Admission of jobs' resident inputs (pair tables, metric) within a byte limit.
+
+    Jobs pass in ticket (submission) order, so no job overtakes a large one waiting for room; a job is
+    always let through when no inputs are resident.  Every job past the gate can run to completion and
+    release its inputs, so waiting at the gate always ends."""
+
+    def __init__(self, limit):
+        self.limit = float(limit)
+        self.used = 0.0
+        self._cv = threading.Condition()
+        self._tickets = 0
+        self._next = 0
+
+    def ticket(self):
+        with self._cv:
+            t = self._tickets
+            self._tickets += 1
+            return t
+
+    def enter(self, ticket, nbytes, failed=False):
+        """Block until it is this ticket's turn and nbytes fit (a failed job only takes its turn)."""
+        with self._cv:
+            self._cv.wait_for(lambda: ticket == self._next and (
+                failed or self.used == 0 or self.used + nbytes <= self.limit))
+            if not failed:
+                self.used += nbytes
+            self._next += 1
+            self._cv.notify_all()
+
+    def adjust(self, delta):
+        """Change the reservation (estimate -> actual, or release when a job finishes)."""
+        with self._cv:
+            self.used += delta
+            self._cv.notify_all()
+
+
 class OutputArena:
     """One device allocation for all unit outputs, carved first-fit (no allocator fragmentation between
     jobs of different sizes).  Offsets and sizes are in float64 elements."""
@@ -98,6 +135,7 @@ class Job:
     completion: Optional[float] = None
     input_bytes: int = 0
     error: Optional[str] = None
+    ticket: int = 0
 
     @property
     def latency(self):
@@ -122,13 +160,12 @@ class IntegralServer:
         self.device = torch.device(device)
         self.compute_stream = torch.cuda.Stream(device=self.device)
         self._tls = threading.local()                # one setup stream per preparation thread
-        self._inputs_cv = threading.Condition()
+        self.gate = InputGate(INPUT_FRACTION * self.budget)
         self.pool = cf.ThreadPoolExecutor(prep_workers)
         self.log = log
         self.prepared = deque()       # host-prepared jobs awaiting device setup and admission
         self.pending_units = deque()  # units of admitted jobs not yet launched
         self.in_flight = deque()      # (event, [units])
-        self.inputs_used = 0.0        # bytes of resident job inputs (pair tables, metric)
         self.outputs_used = 0.0       # bytes of unit outputs in flight (or kept)
         # unit outputs live in one arena: the budget minus the share reserved for resident inputs
         self.arena = OutputArena(torch, (1.0 - INPUT_FRACTION) * self.budget // 8, self.device)
@@ -143,33 +180,31 @@ class IntegralServer:
         are built on a host thread."""
         job = Job(job_id, time.perf_counter() if arrival is None else arrival, molecule)
         self.jobs[job_id] = job
+        job.ticket = self.gate.ticket()
         fut = self.pool.submit(self._prepare, job)
         self.prepared.append((job, fut))
         return job
 
     def _prepare(self, job):
-        """Host thread: geometry/basis, pair tables, then -- once the job's resident inputs fit the input
-        budget -- device setup on this thread's own stream.  Returns nothing; errors are re-raised
-        through the future."""
-        if callable(job.molecule):
-            job.molecule = job.molecule()
-        mol = job.molecule
-        arrays = pair_utility.form_pair_arrays(mol, pair_utility.REFERENCE_SCREENING_THRESHOLD)
-        pairs = pair_utility.form_primitive_pair_data(mol, arrays)
-        aux = pair_utility.form_auxiliary_pair_data(mol)
-        host = (arrays, pairs, aux)
-        est = self.estimate_input_bytes(mol, host)
-        with self._inputs_cv:
-            # a job is always admitted when no other job's inputs are resident
-            self._inputs_cv.wait_for(lambda: self.inputs_used == 0
-                                     or self.inputs_used + est <= INPUT_FRACTION * self.budget)
-            self.inputs_used += est
+        """Host thread: geometry/basis, pair tables, then -- once the job's resident inputs pass the input
+        gate -- device setup on this thread's own stream.  Errors are re-raised through the future."""
+        host, error, est = None, None, 0
+        try:
+            if callable(job.molecule):
+                job.molecule = job.molecule()
+            mol = job.molecule
+            arrays = pair_utility.form_pair_arrays(mol, pair_utility.REFERENCE_SCREENING_THRESHOLD)
+            host = (arrays, pair_utility.form_primitive_pair_data(mol, arrays), pair_utility.form_auxiliary_pair_data(mol))
+            est = self.estimate_input_bytes(mol, host)
+        except Exception as exc:                          # still takes its turn at the gate
+            error = exc
+        self.gate.enter(job.ticket, est, failed=error is not None)
+        if error is not None:
+            raise error
         try:
             self._setup_device(job, host)
         finally:
-            with self._inputs_cv:
-                self.inputs_used += (job.input_bytes if job.plan is not None else 0) - est
-                self._inputs_cv.notify_all()
+            self.gate.adjust((job.input_bytes if job.plan is not None else 0) - est)
 
     # -- device setup and admission -----------------------------------------------------------------
     def _setup_device(self, job, host):
@@ -212,13 +247,15 @@ class IntegralServer:
         return n_pp * (120 + 12 * 400) + molecule.n_aux ** 2 * 8
 
     def _admit(self):
-        """Jobs whose preparation (host and device) has finished join the pending units, in arrival
-        order."""
+        """Every job whose preparation (host and device) has finished joins the pending units, whatever
+        the state of jobs submitted before it (their inputs may be waiting for room that only finishing
+        jobs release)."""
+        waiting = deque()
         while self.prepared:
-            job, fut = self.prepared[0]
+            job, fut = self.prepared.popleft()
             if not fut.done():
-                break
-            self.prepared.popleft()
+                waiting.append((job, fut))
+                continue
             exc = fut.exception()
             if exc is not None:                           # a job that cannot run is reported, not fatal
                 job.error = repr(exc)
@@ -226,6 +263,7 @@ class IntegralServer:
                 self.completed.append(job)
                 continue
             self.pending_units.extend(job.units)
+        self.prepared = waiting
 
     # -- steps ----------------------------------------------------------------------------------------
     def _form_step(self):
@@ -338,9 +376,7 @@ class IntegralServer:
                         if not self.keep_outputs:
                             job.plan = None
                             job.J2c = None
-                            with self._inputs_cv:
-                                self.inputs_used -= job.input_bytes
-                                self._inputs_cv.notify_all()
+                            self.gate.adjust(-job.input_bytes)
 
     def poll(self):
         """One scheduling iteration: retire finished steps, admit prepared jobs, launch a step."""
@@ -397,6 +433,6 @@ def run_poisson(server, jobs, make_molecule, log=None):
             logged = len(server.completed)
             log(f"  {logged}/{n} jobs done at {time.perf_counter() - t0:.1f}s, steps in flight "
                 f"{len(server.in_flight)}, pending units {len(server.pending_units)}, inputs "
-                f"{server.inputs_used / 1e9:.1f} GB, outputs {server.outputs_used / 1e9:.1f} GB")
+                f"{server.gate.used / 1e9:.1f} GB, outputs {server.outputs_used / 1e9:.1f} GB")
     server.torch.cuda.synchronize()
     return time.perf_counter() - t0

@@ -244,3 +244,53 @@ def test_zero_spans_cover_exactly_the_rows():
             want[r * ld:(r + 1) * ld] = 1
         assert np.array_equal(covered, want)
     assert zero_spans([], 5).shape == (0, 2)
+
+
+def test_input_gate_never_deadlocks_and_respects_order():
+    """Threads reserve inputs through the gate in ticket order; a job larger than the limit waits until
+    nothing is resident; jobs finish (release) in an arbitrary order.  Everything passes within the
+    timeout, the limit holds except for a lone oversized job, and passage follows ticket order."""
+    import random
+    import threading
+    from electron_integral_playground_amd.serving import InputGate
+    gate = InputGate(100)
+    rng = random.Random(5)
+    sizes = [rng.choice([10, 30, 60, 150]) for _ in range(60)]
+    order, peak, lock = [], [0.0], threading.Lock()
+    resident = []
+
+    def job(i, t):
+        failed = (i % 17 == 5)
+        gate.enter(t, sizes[i], failed=failed)
+        with lock:
+            order.append(t)
+            if not failed:
+                resident.append(i)
+                assert gate.used <= gate.limit or len(resident) == 1, (gate.used, resident)
+                peak[0] = max(peak[0], gate.used)
+
+    def releaser(stop):
+        while not stop.is_set() or resident:
+            with lock:
+                if resident:
+                    i = resident.pop(rng.randrange(len(resident)))
+                else:
+                    i = None
+            if i is not None:
+                gate.adjust(-sizes[i])
+            threading.Event().wait(0.001)
+
+    stop = threading.Event()
+    rel = threading.Thread(target=releaser, args=(stop,))
+    rel.start()
+    tickets = [gate.ticket() for _ in sizes]
+    threads = [threading.Thread(target=job, args=(i, t)) for i, t in enumerate(tickets)]
+    for th in reversed(threads):         # start out of order: the gate still admits in ticket order
+        th.start()
+    for th in threads:
+        th.join(timeout=30)
+    stop.set()
+    rel.join(timeout=30)
+    assert not any(th.is_alive() for th in threads), "gate deadlocked"
+    assert order == sorted(order)
+    assert gate.used == 0

@@ -185,3 +185,21 @@ def test_row_blocks_mixed_within_bound():
             torch.cuda.synchronize()
             err = (B[:n] - A[:n]).abs().max().item()
             assert err <= bound, (thr, b, err)
+
+
+def test_metric_cholesky_on_device():
+    """L L^T reproduces the float64 metric, and L^-1 (P|ij) whitens a block of the three-centre tensor."""
+    import torch
+    from electron_integral_playground_amd.engine import ThreeCenterPlan, metric_cholesky
+    mol = generators.make_molecule("small")
+    plan = ThreeCenterPlan(mol, pair_utility.form_pair_data(mol, 1e-14), pair_utility.form_auxiliary_pair_data(mol))
+    J2c = plan.two_center()
+    L = metric_cholesky(J2c)
+    torch.cuda.synchronize()
+    assert torch.allclose(L @ L.T, J2c, rtol=1e-12, atol=1e-10 * J2c.abs().max().item())
+    J3c = plan.run()
+    B = torch.linalg.solve_triangular(L, J3c.T.contiguous(), upper=False)
+    torch.cuda.synchronize()
+    # (ij|kl) ~ B^T B: the fitted Coulomb matrix is symmetric positive semi-definite
+    V = B.T @ B
+    assert torch.allclose(V, V.T, atol=1e-10)
