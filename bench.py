@@ -1,15 +1,19 @@
-"""Benchmark: three-centre (ij|P) integrals per second on one MI355X, plain FP64 path.
+"""Benchmark of the McMurchie-Davidson three-centre path on one MI355X; prints one JSON line.
 
-Workload (one "step"): every (ij|P) of the 80-atom plain workload -- 80 atoms in a 14 A box, orbital
-basis 3s2p1d with 1-6 primitives per shell, auxiliary s-f with 8 even-tempered exponents per l
-(n_ao = 1120, n_aux = 10240) -- written into the device-resident packed output
-J3c_packed[i >= j, P] (51.4 GB).  Inputs (primitive pairs, pair matrices, auxiliary shells) are resident
-in HBM before timing starts.  value = n_aux * n_ao (n_ao + 1) / 2 / (seconds per step).
+value: the plain FP64 workload -- every (ij|P) of 80 atoms in a 14 A box, orbital basis 3s2p1d with 1-6
+primitives per shell, auxiliary s-f with 8 even-tempered exponents per l (n_ao = 1120, n_aux = 10240),
+written into the device-resident packed output J3c_packed[i >= j, P] (51.4 GB) with inputs resident in
+HBM before timing.  value = n_aux * n_ao (n_ao + 1) / 2 / (seconds per step).
 
-Also reported: the roofline of the dominant kernel (HIP-event time of its launches on the launch
-stream, algorithmic flops from the fixed per-triplet model) and a CPU baseline -- the reference's
-OpenMP C++ (oracle/_ref) on a bounded sample of the same workload, with a parity check of that sample
-against the GPU output.
+Beside it, in the same line:
+* roofline of the dominant kernel class (HIP-event time on the launch stream, algorithmic flops from
+  the fixed per-triplet model, HBM traffic from the committed rocprofv3 counter summary);
+* cpu_baseline: the reference's OpenMP C++ (oracle/_ref) on a bounded sample, with parity of the sample;
+* mixed_precision: the 200-atom workload (n_ao 6000, n_aux 40000, 3.9 TB of output streamed in blocks),
+  plain FP64 against the FP32/FP64 split, max abs and RMS error on the device, metric Cholesky;
+* batched: 256 molecules (lognormal sizes) in waves of HIP-graph-captured batch plans;
+* serving: a short Poisson stream through the continuous-batching server (p50/p99 latency);
+  `--serving --serving-jobs N` runs the long stream alone.
 """
 
 import argparse
@@ -329,7 +333,10 @@ def main():
     ap.add_argument("--batched-only", action="store_true", help="run only the batched workload")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched workload")
     ap.add_argument("--serving-jobs", type=int, default=2000)
-    ap.add_argument("--serving-rate", type=float, default=1.0, help="Poisson arrival rate, jobs/s")
+    ap.add_argument("--no-serving", action="store_true", help="skip the short serving run of the default line")
+    ap.add_argument("--line-serving-jobs", type=int, default=100,
+                    help="jobs of the short serving run included in the default line")
+    ap.add_argument("--serving-rate", type=float, default=1.8, help="Poisson arrival rate, jobs/s")
     ap.add_argument("--serving-plain", action="store_true", help="serve in plain float64 instead of mixed")
     ap.add_argument("--serving-part", type=int, default=0, help="part index of a stream run in several calls")
     ap.add_argument("--serving-latencies", metavar="JSON", help="write every job's latency to this file")
@@ -442,7 +449,11 @@ def main():
         mixed = mixed_leg(n_atom=args.mixed_atoms)
     if not args.no_batched:
         batched = batched_leg()
-    line = {"metric": "3c integrals/s (plain FP64, device-resident packed output)", "value": value,
+    serving = None
+    if not args.no_serving:
+        torch.cuda.empty_cache()
+        serving = serving_leg(args.line_serving_jobs, args.serving_rate)
+    line = {"metric": "3c integrals/s, mixed vs plain FP64 at <=1e-10 err; p50/p99 job latency", "value": value,
             "unit": "3c integrals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f64", "data": "synthetic (seeded generators, plain workload)",
@@ -454,6 +465,7 @@ def main():
             "three_center_variants": variants,
             "mixed_precision": mixed,
             "batched": batched,
+            "serving": serving,
             "max_abs_err_vs_reference_sample": err}
     print(json.dumps(line), flush=True)
 
