@@ -176,8 +176,8 @@ int prepare_pairs(DevPairs& d, int la, int lb, bool sph, const double* P_p, cons
 }
 
 // Zero whole rows of a row-major matrix: one workgroup per listed row, 16-byte stores.
-// Zero spans of doubles: span b = [spans[2b], spans[2b] + spans[2b+1]), each at most 8192 doubles and
-// starting on a 16-byte boundary when the output is; one block per span, 16-byte non-temporal stores.
+// Zero spans of doubles: span b = [spans[2b], spans[2b] + spans[2b+1]), each at most 16384 doubles; one
+// block per span, 16-byte stores.
 __global__ void __launch_bounds__(256) zero_spans_kernel(const long long* __restrict__ spans, int n_spans,
                                                          double* __restrict__ out)
 {
@@ -186,11 +186,9 @@ __global__ void __launch_bounds__(256) zero_spans_kernel(const long long* __rest
         double* p = out + off;
         const long long head = (reinterpret_cast<uintptr_t>(p) & 15) ? 1 : 0;
         if (head && threadIdx.x == 0 && n > 0) p[0] = 0.0;
-        typedef double zero2 __attribute__((ext_vector_type(2)));
-        zero2* q = reinterpret_cast<zero2*>(p + head);
+        double2* q = reinterpret_cast<double2*>(p + head);
         const long long n2 = (n - head) / 2;
-        const zero2 z = {0.0, 0.0};
-        for (long long k = threadIdx.x; k < n2; k += blockDim.x) __builtin_nontemporal_store(z, q + k);
+        for (long long k = threadIdx.x; k < n2; k += blockDim.x) q[k] = make_double2(0.0, 0.0);
         if (((n - head) & 1) && threadIdx.x == 0) p[n - 1] = 0.0;
     }
 }

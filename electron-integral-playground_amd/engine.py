@@ -83,7 +83,7 @@ class AuxClass:
 DIFFUSE_PAIR_EXPONENT = 0.2
 
 
-def zero_spans(rows, ld, max_len=8192):
+def zero_spans(rows, ld, max_len=16384):
     """Sorted packed rows -> (offset, count) spans of doubles covering them, consecutive rows merged and
     spans cut at max_len (one block each)."""
     rows = np.asarray(rows, dtype=np.int64)

@@ -145,8 +145,8 @@ int eip_three_center_work(int i_L, int j_L, int k_L, int use_f32, const int* d_i
  * pairs, which no three-centre kernel writes. */
 int eip_zero_rows(const int* d_rows, int n_rows, long long ld, double* d_out, void* stream);
 
-/* Zero n_spans spans of d_out: span s covers doubles [d_spans[2s], d_spans[2s] + d_spans[2s+1]) (each at
- * most 8192 long); the screened-out rows of a packed output merged into contiguous runs. */
+/* Zero n_spans spans of d_out: span s covers doubles [d_spans[2s], d_spans[2s] + d_spans[2s+1]) (any
+ * length; one workgroup per span); the screened-out rows of a packed output merged into contiguous runs. */
 int eip_zero_spans(const long long* d_spans, int n_spans, double* d_out, void* stream);
 
 /* (P|Q) blocks of classes (i_L | j_L), aux l <= 4: d_out[(col_i + m_i) * ld + col_j + m_j]; with
