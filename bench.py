@@ -411,6 +411,15 @@ def main():
     for key, ms, fl in per_class[:8]:
         log(f"  class {key}: {ms:.3f} ms  {fl / ms / 1e9:.2f} TFLOP/s (model)")
 
+    # the same step with every launch on one stream, for comparison (not the reported value)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(max(1, args.steps // 2)):
+        plan.run(out, concurrent=False)
+    torch.cuda.synchronize()
+    dt_serial = (time.perf_counter() - t0) / max(1, args.steps // 2)
+    log(f"single-stream step {dt_serial * 1e3:.2f} ms")
+
     # timed region
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -455,7 +464,7 @@ def main():
         serving = serving_leg(args.line_serving_jobs, args.serving_rate)
     line = {"metric": "3c integrals/s, mixed vs plain FP64 at <=1e-10 err; p50/p99 job latency", "value": value,
             "unit": "3c integrals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "ms_per_step": dt * 1e3, "single_stream_ms_per_step": dt_serial * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f64", "data": "synthetic (seeded generators, plain workload)",
             "config": {"workload": "plain: 80 atoms, 14 A box, 3s2p1d (1-6 prim), aux s-f 8 exp/l ratio 2.2",
                        "n_ao": mol.n_ao, "n_aux": mol.n_aux,

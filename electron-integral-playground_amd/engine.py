@@ -38,6 +38,54 @@ def _stream_handle(stream):
     return _vp(s.cuda_stream)
 
 
+_SIDE_STREAMS = {}
+# Streams the independent launches of one run are spread over (besides the caller's stream).  Class
+# kernels write disjoint output regions, so they may run concurrently; the tail of one launch is then
+# filled by the next class instead of idling.  All streams have the default priority.
+N_SIDE_STREAMS = 3
+
+
+class _fork:
+    """Fork ``n`` side streams (default priority, cached per device) from ``stream`` on entry and join
+    them back on exit: work queued on a side stream inside the block runs concurrently with the work on
+    ``stream``, and everything queued on ``stream`` after the block waits for all of it.  Fork and join
+    are event waits, so the pattern is legal inside HIP graph capture.  Yields [] when n == 0."""
+
+    def __init__(self, stream, n):
+        self.stream, self.n = stream, n
+
+    def __enter__(self):
+        if self.n <= 0:
+            self.sides = []
+            return []
+        import torch
+        main = self.stream if self.stream is not None else torch.cuda.current_stream()
+        dev = main.device.index if main.device.index is not None else torch.cuda.current_device()
+        pool = _SIDE_STREAMS.setdefault(dev, [])
+        while len(pool) < self.n:
+            pool.append(torch.cuda.Stream(device=dev))
+        self.main, self.sides = main, pool[:self.n]
+        for s in self.sides:
+            s.wait_stream(main)
+        return self.sides
+
+    def __exit__(self, *exc):
+        for s in self.sides:
+            self.main.wait_stream(s)
+        return False
+
+
+def _assign_lpt(costs, n):
+    """Longest-processing-time assignment of independent launches to n streams: index lists per stream."""
+    load = [0.0] * n
+    out = [[] for _ in range(n)]
+    for i in sorted(range(len(costs)), key=lambda i: -costs[i]):
+        k = min(range(n), key=load.__getitem__)
+        out[k].append(i)
+        load[k] += costs[i]
+    return out
+
+
 def three_center_flops_per_triplet(la, lb, lc):
     """Fixed algorithmic flop model of one primitive triplet (FMA = 2):
     Boys 20(L+1)+20, R recursion 2 C(L+4, 4), pair E 9(la+1)(lb+1)(la+lb+2)/2, auxiliary contraction
@@ -290,15 +338,35 @@ class ThreeCenterPlan:
         if rc != 0:
             raise RuntimeError(f"eip_zero_spans failed with {rc}")
 
-    def run(self, out=None, stream=None, zero=True):
-        """All classes into ``out`` (packed layout); rows of screened-out shell pairs are zero."""
+    def run(self, out=None, stream=None, zero=True, concurrent=True):
+        """All classes into ``out`` (packed layout); rows of screened-out shell pairs are zero.
+
+        With ``concurrent`` the zeroing (HBM-write-bound) runs on a side stream beside the class kernels
+        (FP64-VALU-bound), and the class launches are spread over N_SIDE_STREAMS + 1 streams by
+        estimated cost: every launch writes output elements no other launch writes (its pair class's
+        rows, its auxiliary class's columns; zeroed rows belong to no class).  ``stream`` waits for all
+        of them before the call returns, so later work on it is ordered as after a single-stream run."""
         if out is None:
             out = self.new_output()
-        if zero:
-            self.zero_unwritten(out, stream)
-        for pi in range(len(self.pairs)):
-            for ai in range(len(self.aux)):
-                self.launch_class(pi, ai, out, stream)
+        zero = zero and self.zero_spans.shape[0] > 0
+        if not concurrent:
+            if zero:
+                self.zero_unwritten(out, stream)
+            for pi in range(len(self.pairs)):
+                for ai in range(len(self.aux)):
+                    self.launch_class(pi, ai, out, stream)
+            return out
+        keys = [(pi, ai) for pi in range(len(self.pairs)) for ai in range(len(self.aux))]
+        costs = [self.pairs[pi].n_pp * self.aux[ai].n_prim *
+                 three_center_flops_per_triplet(self.pairs[pi].la, self.pairs[pi].lb, self.aux[ai].lc)
+                 for pi, ai in keys]
+        with _fork(stream, N_SIDE_STREAMS + int(zero)) as sides:
+            if zero:
+                self.zero_unwritten(out, sides[0])
+            lanes = [stream] + sides[int(zero):]
+            for k, idx in enumerate(_assign_lpt(costs, len(lanes))):
+                for i in idx:
+                    self.launch_class(*keys[i], out, lanes[k])
         return out
 
     # -- (P|Q) ------------------------------------------------------------------------------------
@@ -412,16 +480,17 @@ class ThreeCenterPlan:
         """Mixed-precision (ij|P) into the packed float64 output (requires ``setup_mixed``)."""
         if out is None:
             out = self.new_output()
-        if zero:
-            self.zero_unwritten(out, stream)
         keys = list(self.mixed)
         jobs = self.job_table([self.job_words(pi, ai, out.data_ptr(), self.n_aux, self.mixed[(pi, ai)]["perm"])
                                for pi, ai in keys])
-        for j, (pi, ai) in enumerate(keys):
-            m = self.mixed[(pi, ai)]
-            self.launch_work(pi, ai, m["it64"], jobs[j], pp=m["pp64"], layout=0, accumulate=0, stream=stream)
-            self.launch_work(pi, ai, m["it32"], jobs[j], pp=m["pp32"], layout=0, accumulate=m["acc32"],
-                             use_f32=True, stream=stream)
+        with _fork(stream, int(zero and self.zero_spans.shape[0] > 0)) as sides:
+            if sides:
+                self.zero_unwritten(out, sides[0])
+            for j, (pi, ai) in enumerate(keys):
+                m = self.mixed[(pi, ai)]
+                self.launch_work(pi, ai, m["it64"], jobs[j], pp=m["pp64"], layout=0, accumulate=0, stream=stream)
+                self.launch_work(pi, ai, m["it32"], jobs[j], pp=m["pp32"], layout=0, accumulate=m["acc32"],
+                                 use_f32=True, stream=stream)
         return out
 
     def mixed_items(self, pi, ai, sp0, sp1):
@@ -648,21 +717,23 @@ class BatchPlan:
     def run(self, out, stream=None, zero=True):
         """(ij|P) of every molecule into its packed block of out; rows of screened-out shell pairs are
         zeroed by a kernel (no memset: the launch sequence can be captured in a graph)."""
-        if zero and self.zero_spans.shape[0]:
-            rc = self.lib.eip_zero_spans(_ptr(self.zero_spans), int(self.zero_spans.shape[0]), _ptr(out),
-                                         _stream_handle(stream))
-            if rc != 0:
-                raise RuntimeError(f"eip_zero_spans failed with {rc}")
-        for (pi, ai), (it, end) in self.items.items():
-            (la, lb), _, _, dp = self.pairs[pi]
-            lc, n_ks, da = self.aux[ai]
-            rc = self.lib.eip_three_center_items_batched(
-                la, lb, lc, 0, _ptr(it), _ptr(end), it.shape[0], _ptr(dp["off"]), _ptr(dp["si"]), _ptr(dp["sj"]),
-                _ptr(dp["base"]), _ptr(dp["ld"]), _ptr(dp["P"]), _ptr(dp["E"]), _ptr(dp["E32"]), _vp(0),
-                _ptr(da["off"]), _ptr(da["col"]), _ptr(da["C"]), _ptr(da["c"]), n_ks, _ptr(out),
-                _c_double(self.omega), _stream_handle(stream))
-            if rc != 0:
-                raise RuntimeError(f"eip_three_center_items_batched({la},{lb},{lc}) failed with {rc}")
+        with _fork(stream, int(zero and self.zero_spans.shape[0] > 0)) as sides:
+            if sides:
+                # concurrent with the class kernels: the zeroed rows are disjoint from the written ones
+                rc = self.lib.eip_zero_spans(_ptr(self.zero_spans), int(self.zero_spans.shape[0]), _ptr(out),
+                                             _stream_handle(sides[0]))
+                if rc != 0:
+                    raise RuntimeError(f"eip_zero_spans failed with {rc}")
+            for (pi, ai), (it, end) in self.items.items():
+                (la, lb), _, _, dp = self.pairs[pi]
+                lc, n_ks, da = self.aux[ai]
+                rc = self.lib.eip_three_center_items_batched(
+                    la, lb, lc, 0, _ptr(it), _ptr(end), it.shape[0], _ptr(dp["off"]), _ptr(dp["si"]), _ptr(dp["sj"]),
+                    _ptr(dp["base"]), _ptr(dp["ld"]), _ptr(dp["P"]), _ptr(dp["E"]), _ptr(dp["E32"]), _vp(0),
+                    _ptr(da["off"]), _ptr(da["col"]), _ptr(da["C"]), _ptr(da["c"]), n_ks, _ptr(out),
+                    _c_double(self.omega), _stream_handle(stream))
+                if rc != 0:
+                    raise RuntimeError(f"eip_three_center_items_batched({la},{lb},{lc}) failed with {rc}")
         return out
 
 

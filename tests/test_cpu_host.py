@@ -246,6 +246,21 @@ def test_zero_spans_cover_exactly_the_rows():
     assert zero_spans([], 5).shape == (0, 2)
 
 
+def test_stream_assignment_covers_every_launch_once():
+    """The concurrent run spreads class launches over streams: each launch lands on exactly one stream
+    and the largest stream load is within the largest single cost of the mean (LPT bound)."""
+    from electron_integral_playground_amd.engine import _assign_lpt
+    rng = np.random.default_rng(11)
+    for n in (1, 2, 4):
+        costs = list(rng.uniform(0.1, 10.0, 27))
+        lanes = _assign_lpt(costs, n)
+        assert len(lanes) == n
+        assert sorted(i for lane in lanes for i in lane) == list(range(len(costs)))
+        loads = [sum(costs[i] for i in lane) for lane in lanes]
+        assert max(loads) <= sum(costs) / n + max(costs) + 1e-12
+    assert _assign_lpt([], 3) == [[], [], []]
+
+
 def test_input_gate_never_deadlocks_and_respects_order():
     """Threads reserve inputs through the gate in ticket order; a job larger than the limit waits until
     nothing is resident; jobs finish (release) in an arbitrary order.  Everything passes within the

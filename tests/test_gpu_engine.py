@@ -36,6 +36,10 @@ def test_plan_matches_oracle(workload):
     out2 = plan.run(out=torch.full_like(out, float("nan")))
     torch.cuda.synchronize()
     assert torch.equal(out, out2)
+    # launches spread over several streams (the default) equal the single-stream order bit for bit
+    out3 = plan.run(out=torch.full_like(out, float("nan")), concurrent=False)
+    torch.cuda.synchronize()
+    assert torch.equal(out, out3)
 
 
 def _q_aux_from_metric(plan, J2c):
