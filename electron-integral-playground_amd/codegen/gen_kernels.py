@@ -65,6 +65,17 @@ STAGE_LDS = True
 STAGE_BATCH_BYTES = 2048      # rows of small classes are staged this many bytes at a time ...
 STAGE_MAX_ROWS = 16           # ... but at most this many rows (P of each row: 32 bytes, 2 lanes)
 
+# How a three-centre kernel feeds the wave-uniform pair-matrix entries to the contraction:
+# "lds": rows staged into LDS as above, each entry read back as an LDS broadcast;
+# "dpp": the row of a primitive pair lives in registers spread over the lanes -- entry k in VGPR k / 16
+#        at lane k % 16 of every row of 16 lanes (the 4 rows hold the same 16 entries) -- loaded with
+#        plain vector loads one primitive pair ahead, and each entry reaches all 64 lanes by a DPP row
+#        broadcast (v_mov_b64_dpp row_newbcast), so the contraction never waits on memory.
+STAGE = "lds"
+# Rows longer than this many entries stay in LDS even in "dpp" mode (the register copy of a row costs
+# two VGPRs per 16 entries).
+DPP_MAX_ENTRIES = 512
+
 
 # ---------------------------------------------------------------------------------------------
 # Hermite E (pair) code: VRR + HRR per axis
@@ -266,9 +277,32 @@ def leibniz_plan(la, lb, lc, ms):
 # Three-centre kernel
 # ---------------------------------------------------------------------------------------------
 
-def emit_j3c(la, lb, lc, m0, m1, precision="f64"):
-    """Spherical three-centre kernel for aux components [m0, m1) of class (la lb | lc)."""
+def _aux_prefactor_lines(lc, indent="      "):
+    """Per auxiliary primitive (hoisted out of the primitive-pair loop): hl = (-1/(2q))^lc, the common
+    factor of the auxiliary side's Hermite expansion, and ckq = ck 2 pi^(5/2) hl / q, the part of the
+    Coulomb prefactor 2 pi^(5/2) / (p q sqrt(p + q)) that depends on the auxiliary primitive only."""
+    lines = [f"{indent}const double inv_q = 1.0 / q;"]
+    if lc > 0:
+        lines.append(f"{indent}const double hq = -0.5 * inv_q;")
+        lines.append(f"{indent}double hl = hq;")
+        for _ in range(lc - 1):
+            lines.append(f"{indent}hl *= hq;")
+    else:
+        lines.append(f"{indent}const double hl = 1.0;")
+    lines.append(f"{indent}const double ckq = ck * eip::kTwoPi25 * hl * inv_q;")
+    return lines
+
+
+
+def emit_j3c(la, lb, lc, m0, m1, precision="f64", stage=None):
+    """Spherical three-centre kernel for aux components [m0, m1) of class (la lb | lc); ``stage`` is
+    "lds" or "dpp" (default STAGE), the way pair-matrix entries reach the lanes."""
     spherical = True
+    stage = stage or STAGE
+    if stage not in ("lds", "dpp"):
+        raise ValueError(f"unknown staging mode {stage!r}")
+    use_dpp = stage == "dpp" and pair_nnz(la, lb, spherical) <= DPP_MAX_ENTRIES
+    use_lds = not use_dpp and STAGE_LDS
     f32 = precision == "f32"
     ty = "float" if f32 else "double"
     lit = (lambda v: fmt(v) + "f") if f32 else fmt
@@ -323,7 +357,7 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64"):
     # K consecutive rows per buffer (small classes): one wait per K primitive pairs instead of per pair
     K = max(1, min(STAGE_MAX_ROWS, STAGE_BATCH_BYTES // row_bytes))
     n_pieces = (K * row_bytes + 1023) // 1024              # 64 lanes x 16 bytes per global->LDS load
-    if STAGE_LDS:
+    if use_lds:
         out.append(f"    __shared__ __attribute__((aligned(16))) {ty} sE[2][{n_pieces * 1024 // esize}];")
         out.append(f"    __shared__ __attribute__((aligned(16))) double sP[2][{4 * K}];")
         out.append("    const int lane = (int) threadIdx.x;")
@@ -340,9 +374,18 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64"):
         out.append("        if (lane < 2 * (NR)) __builtin_amdgcn_global_load_lds((const void*) (Pbase + (size_t) (PP) * 32 + 16 * lane), "
                    "(__attribute__((address_space(3))) void*) &sP[BUF][0], 16, 0, 0); \\")
         out.append("    } while (0)")
-    if STAGE_LDS:
-        # Every lane takes part in staging, so the loop over auxiliary primitives runs to the wave's
-        # largest primitive count; lanes past their own count contribute with coefficient 0.
+    nv = (nnz + 15) // 16
+    if use_dpp:
+        out.append("    const int l16 = (int) threadIdx.x & 15;")
+        out.append(f"    const auto Erow = eip::ro(JD({'E32' if f32 else 'E'}));")
+        out.append("    const auto Prow = eip::ro(JD(pp_P));")
+        for v in range(nv):
+            idx = f"{16 * v} + l16" if 16 * v + 15 < nnz else f"min({16 * v} + l16, {nnz - 1})"
+            out.append(f"    const int ei_{v} = {idx};")
+    if use_lds or use_dpp:
+        # Every lane takes part in staging (DPP broadcasts need every lane active), so the loop over
+        # auxiliary primitives runs to the wave's largest primitive count; lanes past their own count
+        # contribute with coefficient 0.
         out.append("    const int nkp_wave = eip::wave_max(kp1 - kp0);")
         out.append("    for (int t_kp = 0; t_kp < nkp_wave; ++t_kp) {")
         out.append("      const bool kp_on = kp0 + t_kp < kp1;")
@@ -350,12 +393,28 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64"):
         out.append("      const double Cx = eip::ro(JD(ka_C))[4 * kp + 0], Cy = eip::ro(JD(ka_C))[4 * kp + 1], Cz = eip::ro(JD(ka_C))[4 * kp + 2];")
         out.append("      const double q = eip::ro(JD(ka_C))[4 * kp + 3];")
         out.append("      const double ck = kp_on ? eip::ro(JD(ka_coef))[kp] : 0.0;")
+        out.extend(_aux_prefactor_lines(lc))
     else:
         out.append("    for (int kp = kp0; kp < kp1; ++kp) {")
         out.append("      const double Cx = eip::ro(JD(ka_C))[4 * kp + 0], Cy = eip::ro(JD(ka_C))[4 * kp + 1], Cz = eip::ro(JD(ka_C))[4 * kp + 2];")
         out.append("      const double q = eip::ro(JD(ka_C))[4 * kp + 3];")
         out.append("      const double ck = eip::ro(JD(ka_coef))[kp];")
-    if STAGE_LDS and K == 1:
+        out.extend(_aux_prefactor_lines(lc))
+    if use_dpp:
+        # Row and centre of the first primitive pair.  In the loop, each register Er_v is reloaded with
+        # the next pair's entries right after the contraction's last use of it, so one set of
+        # registers holds the row and the loads overlap the rest of the iteration.
+        out.append(f"      {ty} " + ", ".join(f"Er_{v} = 0" for v in range(nv)) + ";")
+        out.append("      double Pn0 = 0, Pn1 = 0, Pn2 = 0, Pn3 = 0;")
+        out.append("      if (pp0 < pp1) {")
+        for v in range(nv):
+            out.append(f"        Er_{v} = Erow[(size_t) pp0 * {stride} + ei_{v}];")
+        out.append("        Pn0 = Prow[4 * (size_t) pp0 + 0]; Pn1 = Prow[4 * (size_t) pp0 + 1]; Pn2 = Prow[4 * (size_t) pp0 + 2]; Pn3 = Prow[4 * (size_t) pp0 + 3];")
+        out.append("      }")
+        out.append("      for (int pp = pp0; pp < pp1; ++pp) {")
+        out.append("        const double Px = Pn0, Py = Pn1, Pz = Pn2, p = Pn3;")
+        out.append("        const int ppn = min(pp + 1, pp1 - 1);   // the last pair reloads its own row")
+    elif use_lds and K == 1:
         out.append("      if (pp0 < pp1) EIP_STAGE(pp0, 1, 0);")
         out.append("      for (int pp = pp0, buf = 0; pp < pp1; ++pp, buf ^= 1) {")
         # vmcnt(0): the asynchronous global->LDS loads of row pp have landed (the compiler does not
@@ -364,7 +423,7 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64"):
         out.append("        __syncthreads();")
         out.append(f"        const {ty}* __restrict__ E = &sE[buf][0];")
         out.append("        const double Px = sP[buf][0], Py = sP[buf][1], Pz = sP[buf][2], p = sP[buf][3];")
-    elif STAGE_LDS:
+    elif use_lds:
         out.append(f"      if (pp0 < pp1) EIP_STAGE(pp0, min({K}, pp1 - pp0), 0);")
         out.append(f"      for (int b0 = pp0, buf = 0; b0 < pp1; b0 += {K}, buf ^= 1) {{")
         out.append("        __builtin_amdgcn_s_waitcnt(0x0F70);")
@@ -379,31 +438,28 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64"):
         out.append("        const auto Pp = eip::ro(JD(pp_P)) + 4 * (size_t) pp;\n        const double Px = Pp[0], Py = Pp[1], Pz = Pp[2], p = Pp[3];")
     # geometry and prefactor in double always (cheap), then cast
     out.append("        const double dPQx = Px - Cx, dPQy = Py - Cy, dPQz = Pz - Cz;")
+    # zeta = p q / (p + q) and pf = ck 2 pi^(5/2) hl / (p q sqrt(p + q)) from a reciprocal square root
+    # and a reciprocal (hardware estimates refined by two Newton steps: a few ulp), not IEEE divisions
     out.append("        double zeta, pf;")
     out.append("        if (a.omega == 0.0) {")
-    out.append("            zeta = p * q / (p + q);")
-    out.append("            pf = ck * eip::kTwoPi25 / (p * q * sqrt(p + q));")
+    out.append("            const double rs = eip::rsq_nr(p + q);")
+    out.append("            zeta = p * q * (rs * rs);")
+    out.append("            pf = ckq * eip::rcp_nr(p) * rs;")
     out.append("        } else {")
     out.append("            const double w2 = a.omega * a.omega;")
     out.append("            zeta = 1.0 / (1.0 / p + 1.0 / q + 1.0 / w2);")
-    out.append("            pf = ck * eip::kTwoPi25 * a.omega / (p * q * sqrt(p * q + (p + q) * w2));")
+    out.append("            pf = ck * eip::kTwoPi25 * hl * a.omega / (p * q * sqrt(p * q + (p + q) * w2));")
     out.append("        }")
-    if lc > 0:
-        # common factor (-1)^lc (1/2q)^lc of the auxiliary side
-        out.append("        {")
-        out.append("            const double h = 0.5 / q;")
-        out.append("            double hp = h;")
-        for _ in range(lc - 1):
-            out.append("            hp *= h;")
-        out.append(f"            pf *= {'-' if lc % 2 else ''}hp;")
-        out.append("        }")
     out.append(f"        const {ty} PQx = ({ty}) dPQx, PQy = ({ty}) dPQy, PQz = ({ty}) dPQz;")
     out.append(f"        const {ty} T = ({ty}) (zeta * (dPQx * dPQx + dPQy * dPQy + dPQz * dPQz));")
     out.append(f"        {ty} F[{L + 1}];")
     out.append(f"        eip::boys{'32' if f32 else '64'}<{L}>(T, F);")
-    if STAGE_LDS and K == 1:
+    if use_dpp:
+        # centre of the next primitive pair (scalar loads), after the Boys table loads
+        out.append("        Pn0 = Prow[4 * (size_t) ppn + 0]; Pn1 = Prow[4 * (size_t) ppn + 1]; Pn2 = Prow[4 * (size_t) ppn + 2]; Pn3 = Prow[4 * (size_t) ppn + 3];")
+    elif use_lds and K == 1:
         out.append("        if (pp + 1 < pp1) EIP_STAGE(pp + 1, 1, buf ^ 1);   // after the Boys table loads: their wait excludes it")
-    elif STAGE_LDS:
+    elif use_lds:
         out.append(f"        if (r == 0 && b0 + {K} < pp1) EIP_STAGE(b0 + {K}, min({K}, pp1 - b0 - {K}), buf ^ 1);")
     out.append(f"        const {ty} m2z = ({ty}) (-2.0 * zeta);")
     out.append(f"        const {ty} R000_0 = ({ty}) pf * F[0];")
@@ -481,13 +537,19 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64"):
             vnames[m] = vn
         while k < len(layout) and layout[k][0] == t:
             row = layout[k][1]
-            out.append(f"        {{ const {ty} e = E[{k}];")
+            if use_dpp:
+                out.append(f"        {{ const {ty} e = eip::bcast16<{k % 16}>(Er_{k // 16});")
+            else:
+                out.append(f"        {{ const {ty} e = E[{k}];")
             for m in ms:
                 out.append(f"          acc_{row}_{m} = fma{'f' if f32 else ''}(e, {vnames[m]}, acc_{row}_{m});")
             out.append("        }")
+            if use_dpp and (k % 16 == 15 or k == nnz - 1):
+                v = k // 16
+                out.append(f"        Er_{v} = Erow[(size_t) ppn * {stride} + ei_{v}];")
             k += 1
     out.append("      }")
-    if STAGE_LDS and K > 1:
+    if use_lds and K > 1:
         out.append("      }")
     out.append("    }")
     out.append("    if (!active) return;")
@@ -527,7 +589,7 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64"):
             out.append(f"      if (ok_{ij}) o[row_{ij} + {m}] = o_{ij}_{m} + (double) acc_{ij}_{m};")
     out.append("    }")
     out.append("#undef JD")
-    if STAGE_LDS:
+    if use_lds:
         out.append("#undef EIP_STAGE")
     out.append("}")
     out.append(f'extern "C" __global__ void __launch_bounds__(64) {name}(const J3cArgs a) {{ {name}_body<false>(a); }}')
@@ -650,15 +712,27 @@ def emit_j2c(li, lj):
                 for s in sj:
                     needed.add((t[0] + s[0], t[1] + s[1], t[2] + s[2]))
     _emit_R(out, L, needed, "double")
-    for mi, (r1, t1) in enumerate(ri):
-        for mj, (r2, t2) in enumerate(rj):
+    # Two stages: W_mj(t) = sum_s c_j(mj, s) R_{t+s} for every Hermite t of the i side, then
+    # acc_mi,mj += sum_t c_i(mi, t) W_mj(t).  Only the two sides' own transform coefficients appear (a
+    # few dozen constants), not their products -- those would all be hoisted out of the primitive loops
+    # into SGPRs -- and the flop count drops from sum_mi,mj |t_i| |s_j| to (|T_i| + n_i) per mj.
+    ts_i = sorted({t for _, ti in ri for t in ti}, key=lambda t: (sum(t), -t[0], -t[1]))
+    for mj, (r2, t2) in enumerate(rj):
+        for t in ts_i:
+            pieces = []
+            for s, (q2, j2) in t2.items():
+                assert j2 == 0
+                coef = SqrtRational(r2, q2).to_float()
+                rn = _r_name(0, (t[0] + s[0], t[1] + s[1], t[2] + s[2]))
+                pieces.append(rn if coef == 1.0 else f"{fmt(coef)} * {rn}")
+            out.append(f"        const double W_{mj}_{t[0]}_{t[1]}_{t[2]} = {' + '.join(pieces)};")
+        for mi, (r1, t1) in enumerate(ri):
             pieces = []
             for t, (q1, j1) in t1.items():
-                for s, (q2, j2) in t2.items():
-                    assert j1 == 0 and j2 == 0
-                    coef = SqrtRational(r1 * r2, q1 * q2).to_float()
-                    rn = _r_name(0, (t[0] + s[0], t[1] + s[1], t[2] + s[2]))
-                    pieces.append(rn if coef == 1.0 else f"{fmt(coef)} * {rn}")
+                assert j1 == 0
+                coef = SqrtRational(r1, q1).to_float()
+                wn = f"W_{mj}_{t[0]}_{t[1]}_{t[2]}"
+                pieces.append(wn if coef == 1.0 else f"{fmt(coef)} * {wn}")
             out.append(f"        acc_{mi}_{mj} += {' + '.join(pieces)};")
     out.append("      }")
     out.append("    }")

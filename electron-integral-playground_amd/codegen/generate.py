@@ -7,6 +7,7 @@ l <= 4, spherical output); every other class the reference's C ABI accepts (orbi
 l <= 9, Cartesian output) goes to the hand-written generic kernels in csrc/generic_kernels.hip.
 """
 
+import json
 import os
 import sys
 
@@ -34,9 +35,21 @@ def _write(path, text):
             f.write(text)
 
 
+STAGE_CHOICE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "stage_choice.json")
+
+
+def stage_table(path=STAGE_CHOICE):
+    """{"<la><lb><lc>_<prec>": "lds" | "dpp"}: the measured staging mode per class (choose_stage.py)."""
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
+
+
 def generate(out_dir, shrink=None):
     """shrink: {(la, lb, lc): k} lowers the register budget of a class by 2^k (see m_chunks)."""
     shrink = shrink or {}
+    stages = stage_table()
     os.makedirs(out_dir, exist_ok=True)
     boys_path = os.path.join(out_dir, "boys_table.inc")
     if not os.path.exists(boys_path):
@@ -90,7 +103,7 @@ def generate(out_dir, shrink=None):
             cases = []
             for lc, cls_plans in plans.items():
                 for m0, m1 in sorted({c for plan in cls_plans for c in plan}):
-                    name, code = emit_j3c(la, lb, lc, m0, m1, prec)
+                    name, code = emit_j3c(la, lb, lc, m0, m1, prec, stage=stages.get(f"{la}{lb}{lc}_{prec}"))
                     src.append(code)
                     cases.append(f"        case {lc * 10000 + m0 * 100 + m1}: if (a.jobs) hipLaunchKernelGGL({name}_jobs, dim3(nblocks), dim3(64), 0, s, a);"
                                  f" else hipLaunchKernelGGL({name}, dim3(nblocks), dim3(64), 0, s, a); return 0;")

@@ -587,10 +587,11 @@ int eip_zero_rows(const int* d_rows, int n_rows, long long ld, double* d_out, vo
     return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
-int eip_zero_spans(const long long* d_spans, int n_spans, double* d_out, void* stream)
+int eip_zero_spans(const long long* d_spans, int n_spans, double* d_out, int max_blocks, void* stream)
 {
     if (n_spans <= 0) return 0;
-    const int grid = n_spans < 65536 ? n_spans : 65536;
+    const int cap = max_blocks > 0 && max_blocks < 65536 ? max_blocks : 65536;
+    const int grid = n_spans < cap ? n_spans : cap;
     hipLaunchKernelGGL(zero_spans_kernel, dim3(grid), dim3(256), 0, (hipStream_t) stream, d_spans, n_spans, d_out);
     return hipGetLastError() == hipSuccess ? 0 : 2;
 }

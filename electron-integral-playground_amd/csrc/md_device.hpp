@@ -32,9 +32,34 @@ __device__ __forceinline__ int xcd_remap(int b, int nblk)
     return x * per + (x < rem ? x : rem) + i;
 }
 
+// 1/x and 1/sqrt(x) for positive normal x: the hardware estimate (v_rcp_f64, v_rsq_f64) refined by two
+// Newton steps, accurate to a few ulp (IEEE division and square root cost three to four times more
+// instructions).
+__device__ __forceinline__ double rcp_nr(const double x)
+{
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
+
+__device__ __forceinline__ double rsq_nr(const double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    y = y * fma(-h, y * y, 1.5);
+    return y * fma(-h, y * y, 1.5);
+}
+
+constexpr double kHalfSqrtPi = 0.88622692545275801365;   // sqrt(pi) / 2
+
 template <int L>
 __device__ __forceinline__ void boys64(const double T, double* __restrict__ F)
 {
+    // e^{-T} once for both branches: a wave whose lanes fall on both sides of the switch runs both
+    double eT = 0.0;
+    if constexpr (L > 0) eT = exp(-T);
     if (T < BOYS_T_SWITCH) {
         const int k = __double2int_rn(T * BOYS_H_INV);
         const double x = k * (1.0 / BOYS_H_INV) - T;     // -(T - T0)
@@ -44,17 +69,15 @@ __device__ __forceinline__ void boys64(const double T, double* __restrict__ F)
         for (int j = BOYS_K64 - 1; j >= 0; --j) acc = fma(x * (1.0 / (j + 1)), acc, row[j]);
         F[L] = acc;
         if constexpr (L > 0) {
-            const double eT = exp(-T);
             const double twoT = T + T;
 #pragma unroll
             for (int m = L - 1; m >= 0; --m) F[m] = fma(twoT, F[m + 1], eT) * (1.0 / (2 * m + 1));
         }
     } else {
-        const double invT = 1.0 / T;
-        F[0] = 0.5 * sqrt(kPi * invT);
+        const double rT = rsq_nr(T);
+        F[0] = kHalfSqrtPi * rT;                          // sqrt(pi / T) / 2
         if constexpr (L > 0) {
-            const double eT = exp(-T);
-            const double half_invT = 0.5 * invT;
+            const double half_invT = 0.5 * (rT * rT);
 #pragma unroll
             for (int m = 0; m < L; ++m) F[m + 1] = fma((double) (2 * m + 1), F[m], -eT) * half_invT;
         }
@@ -81,12 +104,12 @@ __device__ __forceinline__ void boys32(const float T, float* __restrict__ F)
     } else {
         // large T: the upward recursion runs in float64 and is rounded once per order
         const double Td = T;
-        const double invT = 1.0 / Td;
-        double f = 0.5 * sqrt(kPi * invT);
+        const double rT = rsq_nr(Td);
+        double f = kHalfSqrtPi * rT;
         F[0] = (float) f;
         if constexpr (L > 0) {
             const double eT = exp(-Td);
-            const double half_invT = 0.5 * invT;
+            const double half_invT = 0.5 * (rT * rT);
 #pragma unroll
             for (int m = 0; m < L; ++m) { f = fma((double) (2 * m + 1), f, -eT) * half_invT; F[m + 1] = (float) f; }
         }
@@ -99,6 +122,16 @@ __device__ __forceinline__ int wave_max(int v)
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) v = max(v, __shfl_xor(v, d, 64));
     return __builtin_amdgcn_readfirstlane(v);
+}
+
+// Lane K of each row of 16 lanes, broadcast to every lane of that row (DPP row_newbcast, 32- or
+// 64-bit move).  Callers keep the same value in all four rows, so every lane of the wave receives it.
+// Must be reached with all lanes active: a disabled source lane yields 0 (bound_ctrl).
+template <int K, class T>
+__device__ __forceinline__ T bcast16(T v)
+{
+    static_assert(K >= 0 && K < 16, "row lane");
+    return __builtin_amdgcn_mov_dpp(v, 0x150 + K, 0xf, 0xf, true);
 }
 
 // Address-space views of read-only kernel inputs.  Data the kernels only read (pair matrices, primitive

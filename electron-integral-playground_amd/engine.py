@@ -42,7 +42,10 @@ _SIDE_STREAMS = {}
 # Streams the independent launches of one run are spread over (besides the caller's stream).  Class
 # kernels write disjoint output regions, so they may run concurrently; the tail of one launch is then
 # filled by the next class instead of idling.  All streams have the default priority.
-N_SIDE_STREAMS = 3
+N_SIDE_STREAMS = 5
+# Grid cap of the zeroing kernel when it runs beside the class kernels (0: one workgroup per span).
+# Measured on the plain workload: caps of 256-2048 workgroups were no faster than no cap.
+ZERO_BLOCKS_BESIDE_COMPUTE = 0
 
 
 class _fork:
@@ -330,11 +333,11 @@ class ThreeCenterPlan:
         if rc != 0:
             raise RuntimeError(f"eip_three_center_items({pc.la},{pc.lb},{ac.lc}) failed with {rc}")
 
-    def zero_unwritten(self, out, stream=None):
+    def zero_unwritten(self, out, stream=None, max_blocks=0):
         """Zero the packed rows of shell pairs that are screened out (every other element is written
-        exactly once by the class kernels)."""
+        exactly once by the class kernels); ``max_blocks`` caps the grid (0: one block per span)."""
         rc = self.lib.eip_zero_spans(_ptr(self.zero_spans), int(self.zero_spans.shape[0]), _ptr(out),
-                                     _stream_handle(stream))
+                                     int(max_blocks), _stream_handle(stream))
         if rc != 0:
             raise RuntimeError(f"eip_zero_spans failed with {rc}")
 
@@ -362,7 +365,7 @@ class ThreeCenterPlan:
                  for pi, ai in keys]
         with _fork(stream, N_SIDE_STREAMS + int(zero)) as sides:
             if zero:
-                self.zero_unwritten(out, sides[0])
+                self.zero_unwritten(out, sides[0], ZERO_BLOCKS_BESIDE_COMPUTE)
             lanes = [stream] + sides[int(zero):]
             for k, idx in enumerate(_assign_lpt(costs, len(lanes))):
                 for i in idx:
@@ -485,7 +488,7 @@ class ThreeCenterPlan:
                                for pi, ai in keys])
         with _fork(stream, int(zero and self.zero_spans.shape[0] > 0)) as sides:
             if sides:
-                self.zero_unwritten(out, sides[0])
+                self.zero_unwritten(out, sides[0], ZERO_BLOCKS_BESIDE_COMPUTE)
             for j, (pi, ai) in enumerate(keys):
                 m = self.mixed[(pi, ai)]
                 self.launch_work(pi, ai, m["it64"], jobs[j], pp=m["pp64"], layout=0, accumulate=0, stream=stream)
@@ -721,7 +724,7 @@ class BatchPlan:
             if sides:
                 # concurrent with the class kernels: the zeroed rows are disjoint from the written ones
                 rc = self.lib.eip_zero_spans(_ptr(self.zero_spans), int(self.zero_spans.shape[0]), _ptr(out),
-                                             _stream_handle(sides[0]))
+                                             ZERO_BLOCKS_BESIDE_COMPUTE, _stream_handle(sides[0]))
                 if rc != 0:
                     raise RuntimeError(f"eip_zero_spans failed with {rc}")
             for (pi, ai), (it, end) in self.items.items():

@@ -146,8 +146,11 @@ int eip_three_center_work(int i_L, int j_L, int k_L, int use_f32, const int* d_i
 int eip_zero_rows(const int* d_rows, int n_rows, long long ld, double* d_out, void* stream);
 
 /* Zero n_spans spans of d_out: span s covers doubles [d_spans[2s], d_spans[2s] + d_spans[2s+1]) (any
- * length; one workgroup per span); the screened-out rows of a packed output merged into contiguous runs. */
-int eip_zero_spans(const long long* d_spans, int n_spans, double* d_out, void* stream);
+ * length; one workgroup per span); the screened-out rows of a packed output merged into contiguous runs.
+ * max_blocks > 0 caps the grid (workgroups then loop over spans): a small grid streams the zeros in the
+ * background while compute kernels on other streams keep the rest of the GPU; 0 = one workgroup per
+ * span up to 65536. */
+int eip_zero_spans(const long long* d_spans, int n_spans, double* d_out, int max_blocks, void* stream);
 
 /* (P|Q) blocks of classes (i_L | j_L), aux l <= 4: d_out[(col_i + m_i) * ld + col_j + m_j]; with
  * same_class, blocks with col_i > col_j are skipped; with mirror, the transposed block is written too. */
