@@ -340,6 +340,9 @@ def main():
     ap.add_argument("--serving-plain", action="store_true", help="serve in plain float64 instead of mixed")
     ap.add_argument("--serving-part", type=int, default=0, help="part index of a stream run in several calls")
     ap.add_argument("--serving-latencies", metavar="JSON", help="write every job's latency to this file")
+    ap.add_argument("--single-stream", action="store_true",
+                    help="time the steps with every launch on one stream (profiling runs: kernel durations "
+                         "then do not overlap)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 4 GiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -351,7 +354,7 @@ def main():
     import torch
     __graft_entry__.load_package()
     from electron_integral_playground_amd import generators, pair_utility
-    from electron_integral_playground_amd.engine import (ThreeCenterPlan, apply_tuning, autotune,
+    from electron_integral_playground_amd.engine import (N_SIDE_STREAMS, ThreeCenterPlan, apply_tuning, autotune,
                                                           three_center_flops_per_triplet)
 
     torch.cuda.set_device(0)
@@ -388,7 +391,7 @@ def main():
         variants = apply_tuning().get("variants", {})
 
     for _ in range(args.warmup):
-        plan.run(out)
+        plan.run(out, concurrent=not args.single_stream)
     torch.cuda.synchronize()
     log("warmup done")
 
@@ -412,19 +415,21 @@ def main():
         log(f"  class {key}: {ms:.3f} ms  {fl / ms / 1e9:.2f} TFLOP/s (model)")
 
     # the same step with every launch on one stream, for comparison (not the reported value)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(max(1, args.steps // 2)):
-        plan.run(out, concurrent=False)
-    torch.cuda.synchronize()
-    dt_serial = (time.perf_counter() - t0) / max(1, args.steps // 2)
-    log(f"single-stream step {dt_serial * 1e3:.2f} ms")
+    dt_serial = None
+    if not args.single_stream:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(max(1, args.steps // 2)):
+            plan.run(out, concurrent=False)
+        torch.cuda.synchronize()
+        dt_serial = (time.perf_counter() - t0) / max(1, args.steps // 2)
+        log(f"single-stream step {dt_serial * 1e3:.2f} ms")
 
     # timed region
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        plan.run(out)
+        plan.run(out, concurrent=not args.single_stream)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
     n_int = plan.n_integrals()
@@ -464,7 +469,8 @@ def main():
         serving = serving_leg(args.line_serving_jobs, args.serving_rate)
     line = {"metric": "3c integrals/s, mixed vs plain FP64 at <=1e-10 err; p50/p99 job latency", "value": value,
             "unit": "3c integrals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": dt * 1e3, "single_stream_ms_per_step": dt_serial * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "ms_per_step": dt * 1e3, "single_stream_ms_per_step": dt_serial * 1e3 if dt_serial else None,
+            "streams": 1 if args.single_stream else N_SIDE_STREAMS + 2, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f64", "data": "synthetic (seeded generators, plain workload)",
             "config": {"workload": "plain: 80 atoms, 14 A box, 3s2p1d (1-6 prim), aux s-f 8 exp/l ratio 2.2",
                        "n_ao": mol.n_ao, "n_aux": mol.n_aux,
