@@ -163,6 +163,7 @@ def mixed_leg(threshold=1e-6, block_bytes=40e9, n_atom=None, tune=None):
     log(f"mixed workload: n_ao={mol.n_ao} n_aux={mol.n_aux} primitive pairs={sum(len(p) for p in arrays.values())}"
         f" host pair list {t_pairs:.1f}s, setup {t_setup:.1f}s; split {stats}")
     blocks = plan.row_blocks(block_bytes)
+    plan.prepare_blocks(blocks)
     max_rows = max(plan.block_rows(*b) for b in blocks)
     A = torch.empty((max_rows, mol.n_aux), dtype=torch.float64, device="cuda")
     B = torch.empty_like(A)

@@ -89,6 +89,26 @@ def _assign_lpt(costs, n):
     return out
 
 
+def xcd_logical(n, n_xcd=8):
+    """Item index that block b of an n-block launch reads (the kernels' eip::xcd_remap): XCD x = b % 8
+    owns the contiguous logical range [x per + min(x, rem), ...)."""
+    b = np.arange(n, dtype=np.int64)
+    per, rem = divmod(n, n_xcd)
+    x = b % n_xcd
+    return x * per + np.minimum(x, rem) + b // n_xcd
+
+
+def longest_first(costs):
+    """Order in which to store work items so that the launch dispatches them longest-first: block b
+    (dispatched b-th, on XCD b % 8) processes the b-th most expensive item.  Long items then start first
+    and short ones fill the tail, instead of one long item finishing alone at the end."""
+    costs = np.asarray(costs)
+    order = np.argsort(-costs, kind="stable")
+    stored = np.empty(len(costs), dtype=np.int64)
+    stored[xcd_logical(len(costs))] = order
+    return stored
+
+
 def three_center_flops_per_triplet(la, lb, lc):
     """Fixed algorithmic flop model of one primitive triplet (FMA = 2):
     Boys 20(L+1)+20, R recursion 2 C(L+4, 4), pair E 9(la+1)(lb+1)(la+lb+2)/2, auxiliary contraction
@@ -267,14 +287,23 @@ class ThreeCenterPlan:
             d["off"] = T(off, torch.int32)
             d["col"] = T(ks, torch.int32)
             self.aux.append(ac)
-        # work items: every (shell pair, block of 64 auxiliary shells)
+        # work items: every (shell pair, block of 64 auxiliary shells), in shell-pair order (sliced by
+        # shell-pair ranges when streaming) and stored for longest-first dispatch (whole-class launches)
         self.items: Dict[Tuple[int, int], object] = {}
+        self.items_lpt: Dict[Tuple[int, int], object] = {}
         for pi, pc in enumerate(self.pairs):
+            n_pp_sp = np.diff(pc.sp_pp_off).astype(np.int64)
             for ai, ac in enumerate(self.aux):
                 kb = (ac.n_ks + 63) // 64
                 sp = np.repeat(np.arange(pc.n_sp, dtype=np.int32), kb)
                 k0 = np.tile(np.arange(kb, dtype=np.int32) * 64, pc.n_sp)
-                self.items[(pi, ai)] = T(np.stack([sp, k0], axis=1), torch.int32)
+                items = np.stack([sp, k0], axis=1)
+                self.items[(pi, ai)] = T(items, torch.int32)
+                # cost of an item: primitive pairs x the block's largest auxiliary primitive count
+                nkp = np.diff(ac.ks_prim_off.astype(np.int64))
+                kmax = np.maximum.reduceat(nkp, np.arange(kb) * 64) if len(nkp) else np.ones(kb, np.int64)
+                cost = n_pp_sp[sp] * kmax[k0 // 64]
+                self.items_lpt[(pi, ai)] = T(items[longest_first(cost)], torch.int32)
         # packed rows no kernel writes (shell pairs screened out entirely): zeroed by run()
         written = np.zeros(self.n_packed, dtype=bool)
         for pc in self.pairs:
@@ -323,7 +352,7 @@ class ThreeCenterPlan:
 
     def launch_class(self, pi, ai, out, stream=None, use_f32=False):
         pc, ac = self.pairs[pi], self.aux[ai]
-        it = self.items[(pi, ai)]
+        it = self.items_lpt[(pi, ai)]
         dp, da = pc.dev, ac.dev
         rc = self.lib.eip_three_center_items(
             pc.la, pc.lb, ac.lc, int(use_f32), _ptr(it), _vp(0), it.shape[0], _ptr(dp["off"]), _ptr(dp["si"]),
@@ -532,14 +561,45 @@ class ThreeCenterPlan:
         jobs = self.job_table(entries)
         for ai, ac in enumerate(self.aux):
             if not mixed:
-                G = (ac.n_ks + 63) // 64
-                self.launch_work(pi, ai, self.items[(pi, ai)][sp0 * G:sp1 * G], jobs[ai], layout=1, stream=stream)
+                self.launch_work(pi, ai, self._block_items(pi, ai, sp0, sp1), jobs[ai], layout=1, stream=stream)
                 continue
+            # the split's launches stay in shell-pair order: its auxiliary shells are permuted by Q_P,
+            # and longest-first dispatch measured slower there (the items of one shell pair then leave
+            # the XCD that holds its pair-matrix rows)
             it64, pp64, it32, pp32, acc32 = self.mixed_items(pi, ai, sp0, sp1)
             self.launch_work(pi, ai, it64, jobs[ai], pp=pp64, layout=1, accumulate=0, stream=stream)
             self.launch_work(pi, ai, it32, jobs[ai], pp=pp32, layout=1, accumulate=acc32, use_f32=True,
                              stream=stream)
         return out
+
+    def _block_items(self, pi, ai, sp0, sp1):
+        """Work items of shell pairs [sp0, sp1) (all auxiliary blocks) for one float64 launch, stored for
+        longest-first dispatch; built on first use and kept (a table is never freed while a launch that
+        reads it may be in flight)."""
+        cache = self.__dict__.setdefault("_block_cache", {})
+        key = (pi, ai, sp0, sp1)
+        hit = cache.get(key)
+        if hit is not None:
+            return hit
+        torch = self.torch
+        G = (self.aux[ai].n_ks + 63) // 64
+        it = self.items[(pi, ai)][sp0 * G:sp1 * G]
+        cost = np.repeat(np.diff(self.pairs[pi].sp_pp_off)[sp0:sp1].astype(np.int64), G)
+        if len(cost) > 1:
+            order = torch.as_tensor(longest_first(cost), dtype=torch.int64, device=self.device)
+            it = it.index_select(0, order)
+            # the launch may go to another stream than the one that reordered the table
+            if self.device.type == "cuda":
+                torch.cuda.current_stream(self.device).synchronize()
+        cache[key] = it
+        return it
+
+    def prepare_blocks(self, blocks):
+        """Build the dispatch tables of the float64 ``run_block`` for every (pi, sp0, sp1) in ``blocks``
+        ahead of a timed or captured sequence (a first use builds them on the host and synchronises)."""
+        for pi, sp0, sp1 in blocks:
+            for ai in range(len(self.aux)):
+                self._block_items(pi, ai, sp0, sp1)
 
 
 def metric_cholesky(J2c):
@@ -590,6 +650,7 @@ class BatchPlan:
         aux_keys = sorted({k for _, _, ad in molecules for k in ad})
         stream = _stream_handle(None)
         self.pairs, self.aux = [], []
+        pair_n_pp = []  # primitive pairs per shell pair, per pair class
         mol_sp = {}   # (pair key, molecule) -> (sp_lo, sp_hi)
         for key in pair_keys:
             la, lb = key
@@ -621,6 +682,7 @@ class BatchPlan:
             if rc != 0:
                 raise RuntimeError(f"eip_pair_matrix{key} failed with {rc}")
             self.pairs.append((key, n_sp, n_pp, d))
+            pair_n_pp.append(np.diff(np.array(off, np.int64)))
         mol_ks = {}
         for lc in aux_keys:
             C, c, off, col = [], [], [0], []
@@ -653,8 +715,10 @@ class BatchPlan:
                     it_end.append(np.full((s1 - s0) * nb, k1, np.int32))
                 if it_sp:
                     sp = np.concatenate(it_sp)
-                    self.items[(pi, ai)] = (T(np.stack([sp, np.concatenate(it_k0)], axis=1), torch.int32),
-                                            T(np.concatenate(it_end), torch.int32))
+                    items, end = np.stack([sp, np.concatenate(it_k0)], axis=1), np.concatenate(it_end)
+                    # longest-first dispatch (cost: primitive pairs of the item's shell pair)
+                    order = longest_first(pair_n_pp[pi][sp])
+                    self.items[(pi, ai)] = (T(items[order], torch.int32), T(end[order], torch.int32))
                     self.n_triplet_items += len(sp)
         self.mol_ks = mol_ks
         # per molecule: packed rows that no kernel writes (shell pairs screened out entirely)

@@ -309,3 +309,46 @@ def test_input_gate_never_deadlocks_and_respects_order():
     assert not any(th.is_alive() for th in threads), "gate deadlocked"
     assert order == sorted(order)
     assert gate.used == 0
+
+
+def test_longest_first_dispatch_order():
+    """Items stored by longest_first are read by block b (through the kernels' XCD remap) in descending
+    cost; the remap itself is a permutation of the blocks."""
+    from electron_integral_playground_amd.engine import longest_first, xcd_logical
+    rng = np.random.default_rng(5)
+    for n in (1, 7, 8, 9, 64, 1001):
+        logical = xcd_logical(n)
+        assert sorted(logical.tolist()) == list(range(n))
+        # reference remap, as written in the kernels (eip::xcd_remap)
+        per, rem = divmod(n, 8)
+        ref = [(b % 8) * per + min(b % 8, rem) + b // 8 for b in range(n)]
+        assert logical.tolist() == ref
+        costs = rng.integers(1, 40, n)
+        stored = longest_first(costs)
+        assert sorted(stored.tolist()) == list(range(n))
+        seen = costs[stored[logical]]          # cost of the item block b processes
+        assert np.all(np.diff(seen) <= 0)
+
+
+def test_block_tables_are_permutations_of_the_block_items():
+    """run_block's per-block dispatch table holds exactly the block's items, longest first."""
+    import torch
+    from electron_integral_playground_amd.engine import AuxClass, PairClass, ThreeCenterPlan, xcd_logical
+    rng = np.random.default_rng(2)
+    n_sp, n_ks = 40, 130
+    n_pp_sp = rng.integers(1, 9, n_sp)
+    off = np.concatenate([[0], np.cumsum(n_pp_sp)]).astype(np.int32)
+    plan = object.__new__(ThreeCenterPlan)
+    plan.torch, plan.device = torch, torch.device("cpu")
+    plan.pairs = [PairClass(1, 1, int(off[-1]), n_sp, None, None, off)]
+    plan.aux = [AuxClass(1, n_ks, n_ks, None, np.arange(n_ks + 1, dtype=np.int32))]
+    G = (n_ks + 63) // 64
+    items = np.stack([np.repeat(np.arange(n_sp, dtype=np.int32), G), np.tile(np.arange(G, dtype=np.int32) * 64, n_sp)], 1)
+    plan.items = {(0, 0): torch.as_tensor(items)}
+    for sp0, sp1 in ((0, n_sp), (7, 23), (5, 6)):
+        got = plan._block_items(0, 0, sp0, sp1).numpy()
+        ref = items[sp0 * G:sp1 * G]
+        assert sorted(map(tuple, got.tolist())) == sorted(map(tuple, ref.tolist()))
+        seen = n_pp_sp[got[xcd_logical(len(got)), 0]]      # work of the item block b processes
+        assert np.all(np.diff(seen) <= 0)
+
