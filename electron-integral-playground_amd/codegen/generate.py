@@ -17,7 +17,7 @@ from .gen_kernels import (emit_j2c, emit_j3c, emit_nuc, emit_pairE, m_chunks, pa
 SPEC_MAX_L = 3
 SPEC_MAX_AUX_L = 4
 # Register budgets (doubles per lane) of the chunk-plan variants; variant 0 is the default.
-VARIANT_BUDGETS = (200, 240, 160)
+VARIANT_BUDGETS = (200, 240, 160, 120, 320)
 
 HEADER = """// Generated by codegen/generate.py -- do not edit.
 #include "../md_device.hpp"
