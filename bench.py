@@ -112,7 +112,7 @@ def q_aux_from_metric(plan, J2c):
     return out
 
 
-def mixed_leg(threshold=1e-6, block_bytes=40e9, n_atom=None, tune=None):
+def mixed_leg(threshold=1e-6, block_bytes=40e9, n_atom=None, tune=None, aux_order="natural"):
     """Mixed-precision workload (200 atoms, 20 A box, 4s3p2d1f, auxiliary s-g, n_ao 6000, n_aux 40000):
     the (ij|P) tensor (3.9 TB over the significant shell pairs) is streamed through two device buffers
     in blocks of shell pairs; each block is computed by the plain float64 kernels and by the mixed
@@ -158,12 +158,12 @@ def mixed_leg(threshold=1e-6, block_bytes=40e9, n_atom=None, tune=None):
         from electron_integral_playground_amd.engine import autotune
         autotune(plan, save=tune, log=log, block_bytes=4e9)
         autotune(plan, save=tune, log=log, block_bytes=4e9, use_f32=True)
-    stats = plan.setup_mixed(arrays, q_aux, threshold=threshold)
+    stats = plan.setup_mixed(arrays, q_aux, threshold=threshold, aux_order=aux_order)
     t_setup = time.perf_counter() - t0
     log(f"mixed workload: n_ao={mol.n_ao} n_aux={mol.n_aux} primitive pairs={sum(len(p) for p in arrays.values())}"
         f" host pair list {t_pairs:.1f}s, setup {t_setup:.1f}s; split {stats}")
     blocks = plan.row_blocks(block_bytes)
-    plan.prepare_blocks(blocks)
+    plan.prepare_blocks(blocks, mixed=True)
     max_rows = max(plan.block_rows(*b) for b in blocks)
     A = torch.empty((max_rows, mol.n_aux), dtype=torch.float64, device="cuda")
     B = torch.empty_like(A)
@@ -207,7 +207,7 @@ def mixed_leg(threshold=1e-6, block_bytes=40e9, n_atom=None, tune=None):
     res = {"workload": f"mixed: {len(mol.elements)} atoms, 4s3p2d1f (1-6 prim), aux s-g 8 exp/l",
            "n_ao": mol.n_ao, "n_aux": mol.n_aux,
            "primitive_pairs": int(sum(len(p) for p in arrays.values())),
-           "threshold": threshold, "f32_triplet_fraction": stats["f32_triplets"] / (stats["f32_triplets"] + stats["f64_triplets"]),
+           "threshold": threshold, "aux_order": aux_order, "f32_triplet_fraction": stats["f32_triplets"] / (stats["f32_triplets"] + stats["f64_triplets"]),
            "plain_fp64_integrals_per_s": n_int / t_plain, "mixed_integrals_per_s": n_int / t_mixed,
            "mixed_over_plain": t_plain / t_mixed, "plain_s": t_plain, "mixed_s": t_mixed,
            "max_abs_err_vs_fp64": max_abs.item(), "rms_err_vs_fp64": (sum_sq.item() / n_el) ** 0.5,
@@ -330,6 +330,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision workload")
     ap.add_argument("--mixed-only", action="store_true", help="run only the mixed-precision workload")
     ap.add_argument("--mixed-atoms", type=int, default=None, help="atoms of the mixed workload (default 200)")
+    ap.add_argument("--mixed-aux-order", choices=("qp", "natural"), default="natural",
+                    help="auxiliary-shell grouping of the mixed split (see ThreeCenterPlan.setup_mixed)")
     ap.add_argument("--serving", action="store_true", help="run only the serving workload")
     ap.add_argument("--batched-only", action="store_true", help="run only the batched workload")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched workload")
@@ -368,7 +370,8 @@ def main():
         print(json.dumps({"batched": batched_leg()}), flush=True)
         return
     if args.mixed_only:
-        print(json.dumps({"mixed_precision": mixed_leg(n_atom=args.mixed_atoms, tune=args.tune)}), flush=True)
+        print(json.dumps({"mixed_precision": mixed_leg(n_atom=args.mixed_atoms, tune=args.tune,
+                                                       aux_order=args.mixed_aux_order)}), flush=True)
         return
     t0 = time.perf_counter()
     mol = generators.make_molecule("plain")
@@ -461,7 +464,7 @@ def main():
     del out, plan
     torch.cuda.empty_cache()
     if not args.no_mixed:
-        mixed = mixed_leg(n_atom=args.mixed_atoms)
+        mixed = mixed_leg(n_atom=args.mixed_atoms, aux_order=args.mixed_aux_order)
     if not args.no_batched:
         batched = batched_leg()
     serving = None

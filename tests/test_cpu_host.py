@@ -213,9 +213,13 @@ def test_mixed_split_matches_brute_force():
     n_ks = 150
     q = 10.0 ** rng.uniform(-3, 1, n_ks)
     ks_prim_off = np.concatenate([[0], np.cumsum(rng.integers(1, 3, n_ks))])
-    for thr in (1e-6, 1e-9):
-        s, perm, gprim = mixed_split(off, p, ub, q, ks_prim_off, thr)
-        assert sorted(perm.tolist()) == list(range(n_ks)) and np.all(np.diff(q[perm]) <= 0)
+    for thr, order in ((1e-6, "qp"), (1e-9, "qp"), (1e-6, "natural"), (1e-9, "natural")):
+        s, perm, gprim = mixed_split(off, p, ub, q, ks_prim_off, thr, aux_order=order)
+        assert sorted(perm.tolist()) == list(range(n_ks))
+        if order == "qp":
+            assert np.all(np.diff(q[perm]) <= 0)
+        else:
+            assert perm.tolist() == list(range(n_ks))
         G = (n_ks + 63) // 64
         assert s.shape == (n_sp, G)
         for g in range(G):
@@ -346,9 +350,46 @@ def test_block_tables_are_permutations_of_the_block_items():
     items = np.stack([np.repeat(np.arange(n_sp, dtype=np.int32), G), np.tile(np.arange(G, dtype=np.int32) * 64, n_sp)], 1)
     plan.items = {(0, 0): torch.as_tensor(items)}
     for sp0, sp1 in ((0, n_sp), (7, 23), (5, 6)):
-        got = plan._block_items(0, 0, sp0, sp1).numpy()
+        got = plan._block_items(0, 0, sp0, sp1)[0].numpy()
         ref = items[sp0 * G:sp1 * G]
         assert sorted(map(tuple, got.tolist())) == sorted(map(tuple, ref.tolist()))
         seen = n_pp_sp[got[xcd_logical(len(got)), 0]]      # work of the item block b processes
         assert np.all(np.diff(seen) <= 0)
+
+
+def test_block_tables_follow_the_current_mixed_split():
+    """Mixed block tables (natural auxiliary order) are rebuilt for every setup_mixed call -- a table
+    built for one split must never serve another -- and each is a permutation of the split's items."""
+    import torch
+    from electron_integral_playground_amd.engine import AuxClass, PairClass, ThreeCenterPlan
+    from electron_integral_playground_amd.pair_utility import PairArrays
+    rng = np.random.default_rng(2)
+    n_sp, n_ks = 40, 130
+    off = np.concatenate([[0], np.cumsum(rng.integers(1, 9, n_sp))]).astype(np.int32)
+    n_pp = int(off[-1])
+    plan = object.__new__(ThreeCenterPlan)
+    plan.torch, plan.device = torch, torch.device("cpu")
+    plan.pairs = [PairClass(1, 1, n_pp, n_sp, None, None, off)]
+    plan.aux = [AuxClass(1, n_ks, n_ks, None, np.arange(n_ks + 1, dtype=np.int32))]
+    plan._pair_orders = [np.arange(n_pp)]
+    plan._pair_p = [rng.uniform(0.3, 5.0, n_pp)]
+    ub = np.sort(rng.uniform(1e-9, 1e-4, n_pp))[::-1]
+    pl = {(1, 1): PairArrays(np.zeros(n_pp, np.int64), np.zeros(n_pp, np.int64), np.zeros(n_pp, np.int64),
+                             np.zeros(n_pp, np.int64), ub)}
+    q = [rng.uniform(0.05, 5.0, n_ks)]
+
+    def rows(t, pp):
+        a = t.numpy() if pp is None else np.concatenate([t.numpy(), pp.numpy()], 1)
+        return sorted(map(tuple, a.tolist()))
+
+    for gran in ("pair", "item", "pair"):
+        plan.setup_mixed(pl, q, threshold=1e-6, granularity=gran, aux_order="natural")
+        m = plan.mixed[(0, 0)]
+        for sp0, sp1 in ((0, n_sp), (7, 23)):
+            for kind in ("f64", "f32"):
+                it, pp = plan._block_items(0, 0, sp0, sp1, kind)
+                a, b = np.searchsorted(m["sp" + kind[1:]], [sp0, sp1])
+                ref_pp = m["pp" + kind[1:]]
+                assert (pp is None) == (ref_pp is None)
+                assert rows(it, pp) == rows(m["it" + kind[1:]][a:b], ref_pp[a:b] if ref_pp is not None else None)
 

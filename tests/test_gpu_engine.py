@@ -175,9 +175,11 @@ def test_row_blocks_mixed_within_bound():
     rows = max(plan.block_rows(*b) for b in blocks)
     A = torch.empty((rows, mol.n_aux), dtype=torch.float64, device="cuda")
     B = torch.empty_like(A)
-    for thr, bound, gran in ((1e-6, 1e-10, "pair"), (1e-6, 1e-10, "item"), (0.0, 0.0, "pair"), (0.0, 0.0, "item")):
-        stats = plan.setup_mixed(arrays, q, threshold=thr, granularity=gran)
-        print(f"case threshold={thr} granularity={gran} split={stats}", flush=True)
+    for thr, bound, gran, order in ((1e-6, 1e-10, "pair", "qp"), (1e-6, 1e-10, "item", "qp"),
+                                    (1e-6, 1e-10, "item", "natural"), (1e-6, 1e-10, "pair", "natural"),
+                                    (0.0, 0.0, "pair", "qp"), (0.0, 0.0, "item", "natural")):
+        stats = plan.setup_mixed(arrays, q, threshold=thr, granularity=gran, aux_order=order)
+        print(f"case threshold={thr} granularity={gran} aux_order={order} split={stats}", flush=True)
         if thr > 0 and gran == "pair":
             assert stats["f32_triplets"] > 0
         if thr == 0:
