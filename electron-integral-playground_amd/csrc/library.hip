@@ -193,6 +193,33 @@ __global__ void __launch_bounds__(256) zero_spans_kernel(const long long* __rest
     }
 }
 
+// Background form of zero_spans_kernel for a small grid beside compute kernels: each wave waits for
+// its stores after every kPaceStores of them (vmcnt counts stores on this target), so the bytes in flight
+// -- and the write bandwidth taken from the compute kernels' own output stores -- stay bounded.
+constexpr int kPaceStores = 8;
+
+__global__ void __launch_bounds__(256) zero_spans_paced_kernel(const long long* __restrict__ spans, int n_spans,
+                                                                double* __restrict__ out)
+{
+    for (int b = blockIdx.x; b < n_spans; b += gridDim.x) {
+        const long long off = spans[2 * b], n = spans[2 * b + 1];
+        double* p = out + off;
+        const long long head = (reinterpret_cast<uintptr_t>(p) & 15) ? 1 : 0;
+        if (head && threadIdx.x == 0 && n > 0) p[0] = 0.0;
+        double2* q = reinterpret_cast<double2*>(p + head);
+        const long long n2 = (n - head) / 2;
+        int issued = 0;
+        for (long long k = threadIdx.x; k < n2; k += blockDim.x) {
+            q[k] = make_double2(0.0, 0.0);
+            if (++issued == kPaceStores) {
+                __builtin_amdgcn_s_waitcnt(0x0F70);      // vmcnt(0)
+                issued = 0;
+            }
+        }
+        if (((n - head) & 1) && threadIdx.x == 0) p[n - 1] = 0.0;
+    }
+}
+
 __global__ void __launch_bounds__(256) zero_rows_kernel(const int* __restrict__ rows, int n_rows, long long ld,
                                                         double* __restrict__ out)
 {
@@ -590,8 +617,13 @@ int eip_zero_rows(const int* d_rows, int n_rows, long long ld, double* d_out, vo
 int eip_zero_spans(const long long* d_spans, int n_spans, double* d_out, int max_blocks, void* stream)
 {
     if (n_spans <= 0) return 0;
-    const int cap = max_blocks > 0 && max_blocks < 65536 ? max_blocks : 65536;
-    const int grid = n_spans < cap ? n_spans : cap;
+    if (max_blocks > 0) {
+        const int grid = n_spans < max_blocks ? n_spans : (max_blocks < 65536 ? max_blocks : 65536);
+        hipLaunchKernelGGL(zero_spans_paced_kernel, dim3(grid), dim3(256), 0, (hipStream_t) stream, d_spans, n_spans,
+                           d_out);
+        return hipGetLastError() == hipSuccess ? 0 : 2;
+    }
+    const int grid = n_spans < 65536 ? n_spans : 65536;
     hipLaunchKernelGGL(zero_spans_kernel, dim3(grid), dim3(256), 0, (hipStream_t) stream, d_spans, n_spans, d_out);
     return hipGetLastError() == hipSuccess ? 0 : 2;
 }

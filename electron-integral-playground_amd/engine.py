@@ -43,9 +43,10 @@ _SIDE_STREAMS = {}
 # kernels write disjoint output regions, so they may run concurrently; the tail of one launch is then
 # filled by the next class instead of idling.  All streams have the default priority.
 N_SIDE_STREAMS = 5
-# Grid cap of the zeroing kernel when it runs beside the class kernels (0: one workgroup per span).
-# Measured on the plain workload: caps of 256-2048 workgroups were no faster than no cap.
-ZERO_BLOCKS_BESIDE_COMPUTE = 0
+# Workgroups of the paced (background) zeroing kernel beside the class kernels; 0 runs the full-speed
+# kernel (one workgroup per span).  Plain workload, one box: no zeroing 14.6 ms/step, full-speed 19.4,
+# paced 128 or 256 workgroups 18.9, 64 / 512 / 1024 workgroups 19.6-20.7.
+ZERO_BLOCKS_BESIDE_COMPUTE = 256
 
 
 class _fork:

@@ -147,9 +147,9 @@ int eip_zero_rows(const int* d_rows, int n_rows, long long ld, double* d_out, vo
 
 /* Zero n_spans spans of d_out: span s covers doubles [d_spans[2s], d_spans[2s] + d_spans[2s+1]) (any
  * length; one workgroup per span); the screened-out rows of a packed output merged into contiguous runs.
- * max_blocks > 0 caps the grid (workgroups then loop over spans): a small grid streams the zeros in the
- * background while compute kernels on other streams keep the rest of the GPU; 0 = one workgroup per
- * span up to 65536. */
+ * max_blocks > 0 runs a background form on at most max_blocks workgroups (looping over spans) whose
+ * waves wait for their stores every 8 of them, bounding the write bandwidth taken from compute kernels
+ * on other streams; 0 = one workgroup per span up to 65536 at full speed. */
 int eip_zero_spans(const long long* d_spans, int n_spans, double* d_out, int max_blocks, void* stream);
 
 /* (P|Q) blocks of classes (i_L | j_L), aux l <= 4: d_out[(col_i + m_i) * ld + col_j + m_j]; with

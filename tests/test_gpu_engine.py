@@ -209,3 +209,21 @@ def test_metric_cholesky_on_device():
     # (ij|kl) ~ B^T B: the fitted Coulomb matrix is symmetric positive semi-definite
     V = B.T @ B
     assert torch.allclose(V, V.T, atol=1e-10)
+
+
+@pytest.mark.parametrize("max_blocks", [0, 7, 256])
+def test_zeroing_writes_exactly_the_screened_rows(max_blocks):
+    """The zeroing kernels (full speed, and the paced background form on a capped grid) zero every
+    screened-out packed row and touch nothing else."""
+    import torch
+    mol, _, plan = _block_plan()
+    print(f"case max_blocks={max_blocks} zero rows={plan.zero_rows.numel()}", flush=True)
+    assert plan.zero_rows.numel() > 0
+    out = plan.new_output().fill_(1.0)
+    plan.zero_unwritten(out, None, max_blocks)
+    torch.cuda.synchronize()
+    mask = torch.zeros(out.shape[0], dtype=torch.bool, device=out.device)
+    mask[plan.zero_rows.long()] = True
+    assert bool((out[mask] == 0).all().item())
+    assert bool((out[~mask] == 1.0).all().item())
+
