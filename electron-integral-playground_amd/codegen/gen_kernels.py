@@ -75,6 +75,12 @@ STAGE = "lds"
 # Rows longer than this many entries stay in LDS even in "dpp" mode (the register copy of a row costs
 # two VGPRs per 16 entries).
 DPP_MAX_ENTRIES = 512
+# Float64 "dpp" kernels whose chunk keeps at most this many doubles live (accumulators + Hermite
+# integrals) can pipeline the per-triplet prologue: the next primitive pair's zeta, prefactor, T and
+# Boys grid row are computed and fetched one iteration ahead (its centre two ahead), so the table gather
+# overlaps the current contraction.  Measured with a limit of 140 on the plain and 200-atom workloads:
+# 5-49 % slower per class (the ~30 extra registers cost occupancy), so it is off (limit 0).
+PREFETCH_MAX_DOUBLES = 0
 
 
 # ---------------------------------------------------------------------------------------------
@@ -277,6 +283,19 @@ def leibniz_plan(la, lb, lc, ms):
 # Three-centre kernel
 # ---------------------------------------------------------------------------------------------
 
+def _prologue_lines(P, indent):
+    """Per-triplet prologue of the primitive pair whose centre and exponent are P0..P3 (double): the
+    separation from the auxiliary centre, zeta, the prefactor and T, into ndx/ndy/ndz, nzeta, npf, nT."""
+    return [
+        f"{indent}{{ const double dx = {P}0 - Cx, dy = {P}1 - Cy, dz = {P}2 - Cz, pe = {P}3;",
+        f"{indent}  double z_, f_;",
+        f"{indent}  if (a.omega == 0.0) {{ const double rs = eip::rsq_nr(pe + q); z_ = pe * q * (rs * rs); f_ = ckq * eip::rcp_nr(pe) * rs; }}",
+        f"{indent}  else {{ const double w2 = a.omega * a.omega; z_ = 1.0 / (1.0 / pe + 1.0 / q + 1.0 / w2);",
+        f"{indent}         f_ = ck * eip::kTwoPi25 * hl * a.omega / (pe * q * sqrt(pe * q + (pe + q) * w2)); }}",
+        f"{indent}  nzeta = z_; npf = f_; ndx = dx; ndy = dy; ndz = dz; nT = z_ * (dx * dx + dy * dy + dz * dz); }}",
+    ]
+
+
 def _aux_prefactor_lines(lc, indent="      "):
     """Per auxiliary primitive (hoisted out of the primitive-pair loop): hl = (-1/(2q))^lc, the common
     factor of the auxiliary side's Hermite expansion, and ckq = ck 2 pi^(5/2) hl / q, the part of the
@@ -303,6 +322,9 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64", stage=None):
         raise ValueError(f"unknown staging mode {stage!r}")
     use_dpp = stage == "dpp" and pair_nnz(la, lb, spherical) <= DPP_MAX_ENTRIES
     use_lds = not use_dpp and STAGE_LDS
+    nij_ = n_out(la, spherical) * n_out(lb, spherical)
+    prefetch = (use_dpp and precision != "f32" and not LEIBNIZ_AUX
+                and nij_ * (m1 - m0) + _needed_R(la, lb, lc, m0, m1) <= PREFETCH_MAX_DOUBLES)
     f32 = precision == "f32"
     ty = "float" if f32 else "double"
     lit = (lambda v: fmt(v) + "f") if f32 else fmt
@@ -411,9 +433,31 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64", stage=None):
             out.append(f"        Er_{v} = Erow[(size_t) pp0 * {stride} + ei_{v}];")
         out.append("        Pn0 = Prow[4 * (size_t) pp0 + 0]; Pn1 = Prow[4 * (size_t) pp0 + 1]; Pn2 = Prow[4 * (size_t) pp0 + 2]; Pn3 = Prow[4 * (size_t) pp0 + 3];")
         out.append("      }")
+        if prefetch:
+            # prologue of the first pair and its Boys row; centre of the second
+            out.append("      double Pm0 = 0, Pm1 = 0, Pm2 = 0, Pm3 = 0;")
+            out.append("      double nzeta = 0, npf = 0, ndx = 0, ndy = 0, ndz = 0, nT = 0;")
+            out.append(f"      double nrow[BOYS_K64 + 1];")
+            out.append("      if (pp0 < pp1) {")
+            out.append("        const int pq1 = min(pp0 + 1, pp1 - 1);")
+            out.append("        Pm0 = Prow[4 * (size_t) pq1 + 0]; Pm1 = Prow[4 * (size_t) pq1 + 1]; Pm2 = Prow[4 * (size_t) pq1 + 2]; Pm3 = Prow[4 * (size_t) pq1 + 3];")
+            out.extend(_prologue_lines("Pn", "        "))
+            out.append(f"        eip::boys64_fetch<{L}>(nT, nrow);")
+            out.append("      }")
         out.append("      for (int pp = pp0; pp < pp1; ++pp) {")
-        out.append("        const double Px = Pn0, Py = Pn1, Pz = Pn2, p = Pn3;")
-        out.append("        const int ppn = min(pp + 1, pp1 - 1);   // the last pair reloads its own row")
+        if prefetch:
+            out.append("        const double zeta = nzeta, pf = npf, dPQx = ndx, dPQy = ndy, dPQz = ndz, Tc = nT;")
+            out.append("        double crow[BOYS_K64 + 1];")
+            out.append("        for (int j = 0; j <= BOYS_K64; ++j) crow[j] = nrow[j];")
+            out.append("        const int ppn = min(pp + 1, pp1 - 1);   // the last pair reloads its own row")
+            out.append("        // the next pair (centre loaded one iteration ago): prologue and Boys row now, used next iteration")
+            out.extend(_prologue_lines("Pm", "        "))
+            out.append(f"        eip::boys64_fetch<{L}>(nT, nrow);")
+            out.append("        { const int pq2 = min(pp + 2, pp1 - 1);")
+            out.append("          Pm0 = Prow[4 * (size_t) pq2 + 0]; Pm1 = Prow[4 * (size_t) pq2 + 1]; Pm2 = Prow[4 * (size_t) pq2 + 2]; Pm3 = Prow[4 * (size_t) pq2 + 3]; }")
+        else:
+            out.append("        const double Px = Pn0, Py = Pn1, Pz = Pn2, p = Pn3;")
+            out.append("        const int ppn = min(pp + 1, pp1 - 1);   // the last pair reloads its own row")
     elif use_lds and K == 1:
         out.append("      if (pp0 < pp1) EIP_STAGE(pp0, 1, 0);")
         out.append("      for (int pp = pp0, buf = 0; pp < pp1; ++pp, buf ^= 1) {")
@@ -436,25 +480,34 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64", stage=None):
         out.append("      for (int pp = pp0; pp < pp1; ++pp) {")
         out.append(f"        const auto E = eip::ro(JD({'E32' if f32 else 'E'})) + (size_t) pp * {stride};")
         out.append("        const auto Pp = eip::ro(JD(pp_P)) + 4 * (size_t) pp;\n        const double Px = Pp[0], Py = Pp[1], Pz = Pp[2], p = Pp[3];")
-    # geometry and prefactor in double always (cheap), then cast
-    out.append("        const double dPQx = Px - Cx, dPQy = Py - Cy, dPQz = Pz - Cz;")
-    # zeta = p q / (p + q) and pf = ck 2 pi^(5/2) hl / (p q sqrt(p + q)) from a reciprocal square root
-    # and a reciprocal (hardware estimates refined by two Newton steps: a few ulp), not IEEE divisions
-    out.append("        double zeta, pf;")
-    out.append("        if (a.omega == 0.0) {")
-    out.append("            const double rs = eip::rsq_nr(p + q);")
-    out.append("            zeta = p * q * (rs * rs);")
-    out.append("            pf = ckq * eip::rcp_nr(p) * rs;")
-    out.append("        } else {")
-    out.append("            const double w2 = a.omega * a.omega;")
-    out.append("            zeta = 1.0 / (1.0 / p + 1.0 / q + 1.0 / w2);")
-    out.append("            pf = ck * eip::kTwoPi25 * hl * a.omega / (p * q * sqrt(p * q + (p + q) * w2));")
-    out.append("        }")
-    out.append(f"        const {ty} PQx = ({ty}) dPQx, PQy = ({ty}) dPQy, PQz = ({ty}) dPQz;")
-    out.append(f"        const {ty} T = ({ty}) (zeta * (dPQx * dPQx + dPQy * dPQy + dPQz * dPQz));")
-    out.append(f"        {ty} F[{L + 1}];")
-    out.append(f"        eip::boys{'32' if f32 else '64'}<{L}>(T, F);")
-    if use_dpp:
+    if prefetch:
+        out.append(f"        const double PQx = dPQx, PQy = dPQy, PQz = dPQz;")
+        out.append(f"        const double T = Tc;")
+        out.append(f"        double F[{L + 1}];")
+        out.append(f"        eip::boys64_eval<{L}>(T, crow, F);")
+    else:
+        # geometry and prefactor in double always (cheap), then cast
+        out.append("        const double dPQx = Px - Cx, dPQy = Py - Cy, dPQz = Pz - Cz;")
+        # zeta = p q / (p + q) and pf = ck 2 pi^(5/2) hl / (p q sqrt(p + q)) from a reciprocal square
+        # root and a reciprocal (hardware estimates refined by two Newton steps: a few ulp), not IEEE
+        # divisions
+        out.append("        double zeta, pf;")
+        out.append("        if (a.omega == 0.0) {")
+        out.append("            const double rs = eip::rsq_nr(p + q);")
+        out.append("            zeta = p * q * (rs * rs);")
+        out.append("            pf = ckq * eip::rcp_nr(p) * rs;")
+        out.append("        } else {")
+        out.append("            const double w2 = a.omega * a.omega;")
+        out.append("            zeta = 1.0 / (1.0 / p + 1.0 / q + 1.0 / w2);")
+        out.append("            pf = ck * eip::kTwoPi25 * hl * a.omega / (p * q * sqrt(p * q + (p + q) * w2));")
+        out.append("        }")
+        out.append(f"        const {ty} PQx = ({ty}) dPQx, PQy = ({ty}) dPQy, PQz = ({ty}) dPQz;")
+        out.append(f"        const {ty} T = ({ty}) (zeta * (dPQx * dPQx + dPQy * dPQy + dPQz * dPQz));")
+        out.append(f"        {ty} F[{L + 1}];")
+        out.append(f"        eip::boys{'32' if f32 else '64'}<{L}>(T, F);")
+    if prefetch:
+        pass                      # centres are loaded two ahead above
+    elif use_dpp:
         # centre of the next primitive pair (scalar loads), after the Boys table loads
         out.append("        Pn0 = Prow[4 * (size_t) ppn + 0]; Pn1 = Prow[4 * (size_t) ppn + 1]; Pn2 = Prow[4 * (size_t) ppn + 2]; Pn3 = Prow[4 * (size_t) ppn + 3];")
     elif use_lds and K == 1:

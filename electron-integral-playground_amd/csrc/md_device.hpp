@@ -84,6 +84,46 @@ __device__ __forceinline__ void boys64(const double T, double* __restrict__ F)
     }
 }
 
+// boys64 split in two for software pipelining: boys64_fetch loads the grid row a Taylor evaluation of
+// F_L(T) reads (clamped to the last row, so it can be issued before the branch on T is known; a row
+// fetched for T beyond the grid is simply unused), boys64_eval evaluates from that row.
+template <int L>
+__device__ __forceinline__ void boys64_fetch(const double T, double* __restrict__ row)
+{
+    const int k = __double2int_rn(fmin(T, (double) BOYS_T_SWITCH) * BOYS_H_INV);
+    const double* __restrict__ r = boys_grid64 + k * BOYS_NCOL + L;
+#pragma unroll
+    for (int j = 0; j <= BOYS_K64; ++j) row[j] = r[j];
+}
+
+template <int L>
+__device__ __forceinline__ void boys64_eval(const double T, const double* __restrict__ row, double* __restrict__ F)
+{
+    double eT = 0.0;
+    if constexpr (L > 0) eT = exp(-T);
+    if (T < BOYS_T_SWITCH) {
+        const int k = __double2int_rn(T * BOYS_H_INV);
+        const double x = k * (1.0 / BOYS_H_INV) - T;     // -(T - T0)
+        double acc = row[BOYS_K64];
+#pragma unroll
+        for (int j = BOYS_K64 - 1; j >= 0; --j) acc = fma(x * (1.0 / (j + 1)), acc, row[j]);
+        F[L] = acc;
+        if constexpr (L > 0) {
+            const double twoT = T + T;
+#pragma unroll
+            for (int m = L - 1; m >= 0; --m) F[m] = fma(twoT, F[m + 1], eT) * (1.0 / (2 * m + 1));
+        }
+    } else {
+        const double rT = rsq_nr(T);
+        F[0] = kHalfSqrtPi * rT;                          // sqrt(pi / T) / 2
+        if constexpr (L > 0) {
+            const double half_invT = 0.5 * (rT * rT);
+#pragma unroll
+            for (int m = 0; m < L; ++m) F[m + 1] = fma((double) (2 * m + 1), F[m], -eT) * half_invT;
+        }
+    }
+}
+
 template <int L>
 __device__ __forceinline__ void boys32(const float T, float* __restrict__ F)
 {

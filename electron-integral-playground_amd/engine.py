@@ -205,10 +205,27 @@ def mixed_split(sp_pp_off, p, ub, q, ks_prim_off, threshold, aux_order="natural"
     in group g.  A primitive pair is float64 for group g when it is diffuse or
     ub * max_{P in g} Q_P >= threshold.
     """
-    if aux_order not in ("qp", "natural"):
-        raise ValueError("aux_order is 'qp' or 'natural'")
+    return mixed_split_groups(mixed_split_pairs(sp_pp_off, p, ub, threshold), q, ks_prim_off, aux_order)
+
+
+def mixed_split_pairs(sp_pp_off, p, ub, threshold):
+    """The auxiliary-independent half of mixed_split for one pair class (computed once per class)."""
     sp_pp_off = np.asarray(sp_pp_off, dtype=np.int64)
     n_sp = len(sp_pp_off) - 1
+    sp_of_pp = np.repeat(np.arange(n_sp, dtype=np.int64), np.diff(sp_pp_off))
+    diffuse = np.asarray(p) < DIFFUSE_PAIR_EXPONENT
+    nd = ~diffuse
+    with np.errstate(divide="ignore"):
+        need = threshold / np.asarray(ub, dtype=np.float64)[nd]
+    return {"n_sp": n_sp, "n_diff": np.bincount(sp_of_pp[diffuse], minlength=n_sp),
+            "sp_nd": sp_of_pp[nd], "neg_need": -need}
+
+
+def mixed_split_groups(pre, q, ks_prim_off, aux_order="natural"):
+    """mixed_split for one auxiliary class, from the pair class's mixed_split_pairs."""
+    if aux_order not in ("qp", "natural"):
+        raise ValueError("aux_order is 'qp' or 'natural'")
+    n_sp = pre["n_sp"]
     q = np.asarray(q, dtype=np.float64)
     perm = (np.argsort(-q, kind="stable") if aux_order == "qp" else np.arange(len(q))).astype(np.int32)
     G = (len(q) + 63) // 64
@@ -216,21 +233,14 @@ def mixed_split(sp_pp_off, p, ub, q, ks_prim_off, threshold, aux_order="natural"
     qmax = np.maximum.reduceat(q[perm], starts)
     kprim = np.diff(np.asarray(ks_prim_off, dtype=np.int64))[perm]
     gprim = np.add.reduceat(kprim, starts)
-    n_pp_sp = np.diff(sp_pp_off)
-    sp_of_pp = np.repeat(np.arange(n_sp, dtype=np.int64), n_pp_sp)
-    diffuse = np.asarray(p) < DIFFUSE_PAIR_EXPONENT
-    n_diff = np.bincount(sp_of_pp[diffuse], minlength=n_sp)
-    with np.errstate(divide="ignore"):
-        need = threshold / np.asarray(ub, dtype=np.float64)
     # The count for group g depends only on qmax[g]: work on the groups sorted by descending qmax.
     # g_min: the pair is float64 for the sorted groups h < g_min (qmax_h >= need there).
     by_q = np.argsort(-qmax, kind="stable")
-    g_min = np.searchsorted(-qmax[by_q], -need, side="right")
-    nd = ~diffuse
-    hist = np.bincount(sp_of_pp[nd] * (G + 1) + g_min[nd], minlength=n_sp * (G + 1)).reshape(n_sp, G + 1)
+    g_min = np.searchsorted(-qmax[by_q], pre["neg_need"], side="right")
+    hist = np.bincount(pre["sp_nd"] * (G + 1) + g_min, minlength=n_sp * (G + 1)).reshape(n_sp, G + 1)
     at_least = np.cumsum(hist[:, ::-1], axis=1)[:, ::-1]        # at_least[:, h] = #(g_min >= h)
     s = np.empty((n_sp, G), dtype=np.int64)
-    s[:, by_q] = n_diff[:, None] + at_least[:, 1:]
+    s[:, by_q] = pre["n_diff"][:, None] + at_least[:, 1:]
     return s, perm, gprim
 
 
@@ -498,9 +508,9 @@ class ThreeCenterPlan:
                             "aux_order": aux_order, "granularity": granularity}
         for pi, pc in enumerate(self.pairs):
             ub = _as_arrays(pair_list[(pc.la, pc.lb)]).upper_bound[self._pair_orders[pi]]
+            pre = mixed_split_pairs(pc.sp_pp_off, self._pair_p[pi], ub, threshold)
             for ai, ac in enumerate(self.aux):
-                split_rows, perm, gprim = mixed_split(pc.sp_pp_off, self._pair_p[pi], ub, q_aux[ai],
-                                                      ac.ks_prim_off, threshold, aux_order)
+                split_rows, perm, gprim = mixed_split_groups(pre, q_aux[ai], ac.ks_prim_off, aux_order)
                 G = len(gprim)
                 n_pp_sp = np.diff(pc.sp_pp_off).astype(np.int64)
                 sp = np.repeat(np.arange(pc.n_sp, dtype=np.int32), G)

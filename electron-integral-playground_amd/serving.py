@@ -44,16 +44,19 @@ def _q_aux(plan, J2c):
 class InputGate:
     """Admission of jobs' resident inputs (pair tables, metric) within a byte limit.
 
-    Jobs pass in ticket (submission) order, so no job overtakes a large one waiting for room; a job is
-    always let through when no inputs are resident.  Every job past the gate can run to completion and
-    release its inputs, so waiting at the gate always ends."""
+    A job reaches the gate when its host preparation is done and passes as soon as its inputs fit (or
+    nothing is resident) -- unless a job that holds an earlier ticket is already waiting at the gate for
+    room: waiting jobs pass in ticket order, so a large job is never starved by smaller ones behind it.
+    Jobs still being prepared hold no place in line, so a slow preparation never delays the jobs behind
+    it.  Every job past the gate runs to completion and releases its inputs, so waiting always ends."""
 
-    def __init__(self, limit):
+    def __init__(self, limit, on_pass=None):
         self.limit = float(limit)
         self.used = 0.0
         self._cv = threading.Condition()
         self._tickets = 0
-        self._next = 0
+        self._waiting = set()         # tickets of jobs waiting at the gate
+        self._on_pass = on_pass       # called with (ticket, tickets still waiting) under the lock
 
     def ticket(self):
         with self._cv:
@@ -62,13 +65,18 @@ class InputGate:
             return t
 
     def enter(self, ticket, nbytes, failed=False):
-        """Block until it is this ticket's turn and nbytes fit (a failed job only takes its turn)."""
+        """Block until no earlier ticket waits at the gate and nbytes fit (a failed job reserves nothing
+        and does not wait)."""
+        if failed:
+            return
         with self._cv:
-            self._cv.wait_for(lambda: ticket == self._next and (
-                failed or self.used == 0 or self.used + nbytes <= self.limit))
-            if not failed:
-                self.used += nbytes
-            self._next += 1
+            self._waiting.add(ticket)
+            self._cv.wait_for(lambda: min(self._waiting) == ticket and (
+                self.used == 0 or self.used + nbytes <= self.limit))
+            self._waiting.discard(ticket)
+            self.used += nbytes
+            if self._on_pass is not None:
+                self._on_pass(ticket, frozenset(self._waiting))
             self._cv.notify_all()
 
     def adjust(self, delta):
@@ -146,7 +154,7 @@ class IntegralServer:
     """Single-GPU continuous-batching server for (ij|P) (mixed precision or plain float64) plus the
     float64 metric (P|Q) of every job."""
 
-    def __init__(self, budget_bytes=200e9, mixed=True, threshold=1e-6, keep_outputs=False, prep_workers=8,
+    def __init__(self, budget_bytes=200e9, mixed=True, threshold=1e-6, keep_outputs=False, prep_workers=12,
                  max_steps_in_flight=2, unit_cap_bytes=None, granularity="item", device="cuda", log=None):
         import torch
         self.torch = torch

@@ -266,35 +266,39 @@ def test_stream_assignment_covers_every_launch_once():
 
 
 def test_input_gate_never_deadlocks_and_respects_order():
-    """Threads reserve inputs through the gate in ticket order; a job larger than the limit waits until
-    nothing is resident; jobs finish (release) in an arbitrary order.  Everything passes within the
-    timeout, the limit holds except for a lone oversized job, and passage follows ticket order."""
+    """Threads reserve inputs through the gate; a job larger than the limit waits until nothing is
+    resident; jobs finish (release) in an arbitrary order.  Everything passes within the timeout, the limit
+    holds except for a lone oversized job, and a job never passes while an earlier ticket is waiting at
+    the gate."""
     import random
     import threading
+    import time
     from electron_integral_playground_amd.serving import InputGate
-    gate = InputGate(100)
+    order = []
+
+    def on_pass(ticket, still_waiting):
+        assert all(w > ticket for w in still_waiting), (ticket, sorted(still_waiting))
+        order.append(ticket)
+
+    gate = InputGate(100, on_pass=on_pass)
     rng = random.Random(5)
     sizes = [rng.choice([10, 30, 60, 150]) for _ in range(60)]
-    order, peak, lock = [], [0.0], threading.Lock()
+    lock = threading.Lock()
     resident = []
 
     def job(i, t):
         failed = (i % 17 == 5)
+        time.sleep(rng.uniform(0, 0.01))                 # host preparation of uneven length
         gate.enter(t, sizes[i], failed=failed)
-        with lock:
-            order.append(t)
-            if not failed:
+        if not failed:
+            with lock:
                 resident.append(i)
                 assert gate.used <= gate.limit or len(resident) == 1, (gate.used, resident)
-                peak[0] = max(peak[0], gate.used)
 
     def releaser(stop):
         while not stop.is_set() or resident:
             with lock:
-                if resident:
-                    i = resident.pop(rng.randrange(len(resident)))
-                else:
-                    i = None
+                i = resident.pop(rng.randrange(len(resident))) if resident else None
             if i is not None:
                 gate.adjust(-sizes[i])
             threading.Event().wait(0.001)
@@ -304,15 +308,35 @@ def test_input_gate_never_deadlocks_and_respects_order():
     rel.start()
     tickets = [gate.ticket() for _ in sizes]
     threads = [threading.Thread(target=job, args=(i, t)) for i, t in enumerate(tickets)]
-    for th in reversed(threads):         # start out of order: the gate still admits in ticket order
+    for th in reversed(threads):
         th.start()
     for th in threads:
         th.join(timeout=30)
     stop.set()
     rel.join(timeout=30)
     assert not any(th.is_alive() for th in threads), "gate deadlocked"
-    assert order == sorted(order)
-    assert gate.used == 0
+    assert gate.used == 0 and not gate._waiting
+    assert sorted(order) == sorted(t for i, t in enumerate(tickets) if i % 17 != 5)
+
+
+def test_input_gate_slow_preparation_does_not_block_later_jobs():
+    """A job whose preparation has not finished holds no place in line: later jobs pass the gate."""
+    import threading
+    from electron_integral_playground_amd.serving import InputGate
+    gate = InputGate(100)
+    t_slow, t_fast = gate.ticket(), gate.ticket()
+    done = threading.Event()
+
+    def fast():
+        gate.enter(t_fast, 10)
+        done.set()
+
+    th = threading.Thread(target=fast)
+    th.start()
+    assert done.wait(5), "a later job waited for an earlier job that had not reached the gate"
+    th.join()
+    gate.enter(t_slow, 10)
+    assert gate.used == 20
 
 
 def test_longest_first_dispatch_order():
