@@ -384,6 +384,7 @@ def main():
         f"host pair list {t_host:.1f}s")
     plan = ThreeCenterPlan(mol, pairs, aux)
     out = plan.new_output()
+    J2c = torch.zeros((mol.n_aux, mol.n_aux), dtype=torch.float64, device="cuda")   # (P|Q), part of each step
     stream = torch.cuda.current_stream()
     if args.copy_probe:
         a = torch.empty(1 << 29, dtype=torch.float64, device="cuda")
@@ -397,7 +398,7 @@ def main():
         variants = apply_tuning().get("variants", {})
 
     for _ in range(args.warmup):
-        plan.run(out, concurrent=not args.single_stream)
+        plan.run(out, concurrent=not args.single_stream, metric_out=J2c)
     torch.cuda.synchronize()
     log("warmup done")
 
@@ -426,7 +427,7 @@ def main():
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(max(1, args.steps // 2)):
-            plan.run(out, concurrent=False)
+            plan.run(out, concurrent=False, metric_out=J2c)
         torch.cuda.synchronize()
         dt_serial = (time.perf_counter() - t0) / max(1, args.steps // 2)
         log(f"single-stream step {dt_serial * 1e3:.2f} ms")
@@ -435,7 +436,7 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        plan.run(out, concurrent=not args.single_stream)
+        plan.run(out, concurrent=not args.single_stream, metric_out=J2c)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
     n_int = plan.n_integrals()
@@ -463,7 +464,7 @@ def main():
         log(f"cpu baseline {cpu}; sample max |GPU - reference| = {err:.3e}")
 
     mixed = batched = None
-    del out, plan
+    del out, plan, J2c
     torch.cuda.empty_cache()
     if not args.no_mixed:
         mixed = mixed_leg(n_atom=args.mixed_atoms, aux_order=args.mixed_aux_order)
@@ -481,7 +482,8 @@ def main():
             "config": {"workload": "plain: 80 atoms, 14 A box, 3s2p1d (1-6 prim), aux s-f 8 exp/l ratio 2.2",
                        "n_ao": mol.n_ao, "n_aux": mol.n_aux,
                        "primitive_pairs": int(sum(len(p.P_p) for p in pairs.values())),
-                       "output": "packed (n_ao(n_ao+1)/2, n_aux) float64 on device"},
+                       "output": "packed (n_ao(n_ao+1)/2, n_aux) float64 (ij|P) plus dense (n_aux, n_aux) float64 "
+                                 "(P|Q), on device, every step"},
             "roofline": roofline, "cpu_baseline": cpu,
             "three_center_variants": variants,
             "mixed_precision": mixed,

@@ -390,8 +390,10 @@ class ThreeCenterPlan:
         if rc != 0:
             raise RuntimeError(f"eip_zero_spans failed with {rc}")
 
-    def run(self, out=None, stream=None, zero=True, concurrent=True):
-        """All classes into ``out`` (packed layout); rows of screened-out shell pairs are zero.
+    def run(self, out=None, stream=None, zero=True, concurrent=True, metric_out=None):
+        """All classes into ``out`` (packed layout); rows of screened-out shell pairs are zero.  With
+        ``metric_out`` (a zeroed or reused (n_aux, n_aux) float64 tensor) the metric (P|Q) is computed
+        in the same call, on one of the streams.
 
         With ``concurrent`` the zeroing (HBM-write-bound) runs on a side stream beside the class kernels
         (FP64-VALU-bound), and the class launches are spread over N_SIDE_STREAMS + 1 streams by
@@ -404,6 +406,8 @@ class ThreeCenterPlan:
         if not concurrent:
             if zero:
                 self.zero_unwritten(out, stream)
+            if metric_out is not None:
+                self.two_center(metric_out, stream)
             for pi in range(len(self.pairs)):
                 for ai in range(len(self.aux)):
                     self.launch_class(pi, ai, out, stream)
@@ -416,6 +420,9 @@ class ThreeCenterPlan:
             if zero:
                 self.zero_unwritten(out, sides[0], ZERO_BLOCKS_BESIDE_COMPUTE)
             lanes = [stream] + sides[int(zero):]
+            if metric_out is not None:
+                # the metric goes first on the last lane (LPT gives it the least class work)
+                self.two_center(metric_out, lanes[-1])
             for k, idx in enumerate(_assign_lpt(costs, len(lanes))):
                 for i in idx:
                     self.launch_class(*keys[i], out, lanes[k])

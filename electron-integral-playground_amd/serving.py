@@ -30,6 +30,11 @@ from .engine import ThreeCenterPlan
 UNIT_CAP_FRACTION = 0.25      # a unit's output may use at most this share of the budget
 INPUT_FRACTION = 0.25         # admitted jobs' resident inputs (pair tables, metric) use at most this share
 BYPASS_SECONDS = 5.0          # a blocked head unit may be overtaken by smaller units for this long
+SETUP_CONCURRENCY = 3         # jobs in device setup at once (their transient allocations are not gated)
+# Device memory that PyTorch keeps cached (freed, not returned) is unavailable to the HIP runtime's own
+# allocations -- e.g. the scratch memory a queue needs when it first runs a kernel with a private
+# segment.  Finished jobs' inputs are returned to the device once the cache holds more than this.
+CACHE_SLACK_BYTES = 8e9
 
 
 def _q_aux(plan, J2c):
@@ -169,6 +174,7 @@ class IntegralServer:
         self.compute_stream = torch.cuda.Stream(device=self.device)
         self._tls = threading.local()                # one setup stream per preparation thread
         self.gate = InputGate(INPUT_FRACTION * self.budget)
+        self._setup_slots = threading.Semaphore(SETUP_CONCURRENCY)
         self.pool = cf.ThreadPoolExecutor(prep_workers)
         self.log = log
         self.prepared = deque()       # host-prepared jobs awaiting device setup and admission
@@ -210,7 +216,8 @@ class IntegralServer:
         if error is not None:
             raise error
         try:
-            self._setup_device(job, host)
+            with self._setup_slots:
+                self._setup_device(job, host)
         finally:
             self.gate.adjust((job.input_bytes if job.plan is not None else 0) - est)
 
@@ -364,6 +371,7 @@ class IntegralServer:
     def _retire(self):
         """Consume the units of finished steps (checksum on the device) and free their memory."""
         torch = self.torch
+        released = False
         while self.in_flight and self.in_flight[0][0].query():
             now = time.perf_counter()
             _, units, _ = self.in_flight.popleft()
@@ -385,6 +393,10 @@ class IntegralServer:
                             job.plan = None
                             job.J2c = None
                             self.gate.adjust(-job.input_bytes)
+                            released = True
+            if released and torch.cuda.memory_reserved(self.device) - torch.cuda.memory_allocated(self.device) \
+                    > CACHE_SLACK_BYTES:
+                torch.cuda.empty_cache()
 
     def poll(self):
         """One scheduling iteration: retire finished steps, admit prepared jobs, launch a step."""
@@ -441,6 +453,7 @@ def run_poisson(server, jobs, make_molecule, log=None):
             logged = len(server.completed)
             log(f"  {logged}/{n} jobs done at {time.perf_counter() - t0:.1f}s, steps in flight "
                 f"{len(server.in_flight)}, pending units {len(server.pending_units)}, inputs "
-                f"{server.gate.used / 1e9:.1f} GB, outputs {server.outputs_used / 1e9:.1f} GB")
+                f"{server.gate.used / 1e9:.1f} GB, outputs {server.outputs_used / 1e9:.1f} GB, device memory "
+                f"reserved {server.torch.cuda.memory_reserved(server.device) / 1e9:.1f} GB")
     server.torch.cuda.synchronize()
     return time.perf_counter() - t0

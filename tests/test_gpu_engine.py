@@ -40,6 +40,13 @@ def test_plan_matches_oracle(workload):
     out3 = plan.run(out=torch.full_like(out, float("nan")), concurrent=False)
     torch.cuda.synchronize()
     assert torch.equal(out, out3)
+    # the metric computed in the same call (on a side stream, or in order) equals two_center()
+    ref2 = plan.two_center()
+    for concurrent in (True, False):
+        J = torch.full_like(ref2, float("nan"))
+        out4 = plan.run(out=torch.full_like(out, float("nan")), concurrent=concurrent, metric_out=J)
+        torch.cuda.synchronize()
+        assert torch.equal(out, out4) and torch.equal(J, ref2), concurrent
 
 
 def _q_aux_from_metric(plan, J2c):
