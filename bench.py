@@ -239,7 +239,12 @@ def serving_leg(n_jobs, rate, budget=200e9, mixed=True, part=0, latency_file=Non
     done = [j for j in server.completed if j.error is None]
     lat = np.array([j.latency for j in done])
     n_int = sum(j.n_aux * j.n_ao * (j.n_ao + 1) // 2 for j in done)
+    ph = [j.phases() for j in done if j.phases() is not None]
+    phases = {k: {"p50": float(np.percentile([x[k] for x in ph], 50)),
+                  "p90": float(np.percentile([x[k] for x in ph], 90)),
+                  "p99": float(np.percentile([x[k] for x in ph], 99))} for k in (ph[0] if ph else {})}
     res = {"jobs": n_jobs, "completed": len(done), "errors": len(server.completed) - len(done),
+           "latency_phases_s": phases,
            "rate_jobs_per_s": rate, "budget_GB": budget / 1e9, "mixed_precision": mixed,
            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
            "mean_latency_s": float(lat.mean()), "max_latency_s": float(lat.max()),

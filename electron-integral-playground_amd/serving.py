@@ -149,10 +149,24 @@ class Job:
     input_bytes: int = 0
     error: Optional[str] = None
     ticket: int = 0
+    # host clock: preparation done, past the input gate, device setup done, first unit launched
+    t_prepared: Optional[float] = None
+    t_gate: Optional[float] = None
+    t_setup: Optional[float] = None
+    t_first_launch: Optional[float] = None
 
     @property
     def latency(self):
         return None if self.completion is None else self.completion - self.arrival
+
+    def phases(self):
+        """Seconds spent in host preparation, waiting at the input gate, device setup, waiting for a
+        step, and from the first launch to completion."""
+        t = [self.arrival, self.t_prepared, self.t_gate, self.t_setup, self.t_first_launch, self.completion]
+        if any(x is None for x in t):
+            return None
+        return {"prepare": t[1] - t[0], "gate": t[2] - t[1], "setup": t[3] - t[2], "queue": t[4] - t[3],
+                "run": t[5] - t[4]}
 
 
 class IntegralServer:
@@ -212,7 +226,9 @@ class IntegralServer:
             est = self.estimate_input_bytes(mol, host)
         except Exception as exc:                          # still takes its turn at the gate
             error = exc
+        job.t_prepared = time.perf_counter()
         self.gate.enter(job.ticket, est, failed=error is not None)
+        job.t_gate = time.perf_counter()
         if error is not None:
             raise error
         try:
@@ -220,6 +236,7 @@ class IntegralServer:
                 self._setup_device(job, host)
         finally:
             self.gate.adjust((job.input_bytes if job.plan is not None else 0) - est)
+        job.t_setup = time.perf_counter()
 
     # -- device setup and admission -----------------------------------------------------------------
     def _setup_device(self, job, host):
@@ -282,26 +299,44 @@ class IntegralServer:
 
     # -- steps ----------------------------------------------------------------------------------------
     def _form_step(self):
-        """Pending units (FIFO) whose outputs the arena can place.  A unit is at most UNIT_CAP_FRACTION of
-        the budget and the arena is (1 - INPUT_FRACTION) of it, so the head unit always fits once the
-        steps in flight have retired."""
-        units, kept = [], deque()
+        """Pending units whose outputs the arena can place, taken round-robin across jobs (each job's
+        units in order): a job with many units (one larger than the per-unit cap) shares every step
+        with the jobs behind it instead of holding them until all its units have run.  A unit is at most
+        UNIT_CAP_FRACTION of the budget and the arena is (1 - INPUT_FRACTION) of it, so a blocked unit
+        fits once the steps in flight have retired; after BYPASS_SECONDS blocked, no later unit is taken
+        before it."""
         now = time.perf_counter()
-        while self.pending_units:
-            u = self.pending_units.popleft()
-            off = self.arena.alloc(u.rows * u.job.n_aux)
-            if off is None:
-                kept.append(u)
-                if u.blocked_since is None:
-                    u.blocked_since = now
-                if now - u.blocked_since > BYPASS_SECONDS:
-                    break                        # the oldest blocked unit has waited long enough: no bypass
-                continue                         # smaller units behind it may still fit
-            u.arena_off = off
-            u.out = self.arena.buf[off:off + u.rows * u.job.n_aux].view(u.rows, u.job.n_aux)
-            units.append(u)
-        kept.extend(self.pending_units)
-        self.pending_units = kept
+        queues, index = [], {}
+        for u in self.pending_units:
+            k = id(u.job)
+            if k not in index:
+                index[k] = len(queues)
+                queues.append(deque())
+            queues[index[k]].append(u)
+        units, taken, closed, stop = [], set(), set(), False
+        progress = True
+        while progress and not stop:
+            progress = False
+            for qi, q in enumerate(queues):
+                if not q or qi in closed:
+                    continue
+                u = q[0]
+                off = self.arena.alloc(u.rows * u.job.n_aux)
+                if off is None:
+                    if u.blocked_since is None:
+                        u.blocked_since = now
+                    if now - u.blocked_since > BYPASS_SECONDS:
+                        stop = True              # the blocked unit has waited long enough: no bypass
+                        break
+                    closed.add(qi)               # this job's later units wait behind its head
+                    continue
+                q.popleft()
+                u.arena_off = off
+                u.out = self.arena.buf[off:off + u.rows * u.job.n_aux].view(u.rows, u.job.n_aux)
+                units.append(u)
+                taken.add(id(u))
+                progress = True
+        self.pending_units = deque(u for u in self.pending_units if id(u) not in taken)
         return units
 
     def _launch_step(self, units):
@@ -367,6 +402,10 @@ class IntegralServer:
             ev.record(self.compute_stream)
         self.in_flight.append((ev, units, keep))
         self.n_steps += 1
+        now = time.perf_counter()
+        for u in units:
+            if u.job.t_first_launch is None:
+                u.job.t_first_launch = now
 
     def _retire(self):
         """Consume the units of finished steps (checksum on the device) and free their memory."""

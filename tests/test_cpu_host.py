@@ -417,3 +417,31 @@ def test_block_tables_follow_the_current_mixed_split():
                 assert (pp is None) == (ref_pp is None)
                 assert rows(it, pp) == rows(m["it" + kind[1:]][a:b], ref_pp[a:b] if ref_pp is not None else None)
 
+
+
+def test_serving_steps_take_units_round_robin_across_jobs():
+    """A step takes pending units job by job in turn (each job's units in order), so a job with many
+    units does not hold the jobs behind it; units that do not fit stay pending in their order."""
+    import time
+    from collections import deque
+    import torch
+    from electron_integral_playground_amd.serving import IntegralServer, Job, OutputArena, Unit
+    server = object.__new__(IntegralServer)
+    server.arena = OutputArena(torch, 4 * 10, torch.device("cpu"))     # room for 4 units of 10 elements
+
+    def job(name, n_units):
+        j = Job(name, time.perf_counter(), None, n_ao=1, n_aux=1)
+        j.units = [Unit(j, [], 10, 80, [0]) for _ in range(n_units)]
+        return j
+
+    a, b, c = job("a", 5), job("b", 1), job("c", 2)
+    server.pending_units = deque(a.units + b.units + c.units)
+    step = server._form_step()
+
+    def names(units):
+        return [(u.job.job_id, next(k for k, x in enumerate(u.job.units) if x is u)) for u in units]
+
+    assert names(step) == [("a", 0), ("b", 0), ("c", 0), ("a", 1)]
+    assert names(server.pending_units) == [("a", 2), ("a", 3), ("a", 4), ("c", 1)]
+    # the arena is full: nothing else fits, everything stays pending
+    assert server._form_step() == [] and len(server.pending_units) == 4
