@@ -142,7 +142,8 @@ class Job:
     n_ao: int = 0
     n_aux: int = 0
     plan: Optional[ThreeCenterPlan] = None
-    J2c: object = None
+    J2c: object = None                # (P|Q) on the device (keep_outputs), else reduced to J2c_checksum
+    J2c_checksum: object = None
     units: List[Unit] = field(default_factory=list)
     units_done: int = 0
     completion: Optional[float] = None
@@ -251,11 +252,16 @@ class IntegralServer:
             plan = ThreeCenterPlan(mol, pairs, aux, device=self.device)
             job.J2c = plan.two_center()
             q = _q_aux(plan, job.J2c)                     # synchronises this thread's stream only
+            if not self.keep_outputs:
+                # the metric is an output too: consumed (checksum) now instead of staying resident --
+                # at n_aux = 40 000 it is 12.8 GB of the input budget
+                job.J2c_checksum = torch.stack([job.J2c.sum(), torch.linalg.vector_norm(job.J2c)])
+                job.J2c = None
             if self.mixed:
                 plan.setup_mixed(arrays, q, threshold=self.threshold, granularity=self.granularity)
             stream.synchronize()
         job.plan = plan
-        job.input_bytes = plan.device_bytes() + job.J2c.numel() * 8
+        job.input_bytes = plan.device_bytes() + (job.J2c.numel() * 8 if job.J2c is not None else 0)
         # units: the whole job, or blocks of shell pairs
         total_rows = sum(plan.block_rows(pi, 0, pc.n_sp) for pi, pc in enumerate(plan.pairs))
         if total_rows * mol.n_aux * 8 <= self.unit_cap:

@@ -64,3 +64,20 @@ def test_server_matches_engine_blocks():
         torch.cuda.synchronize()
         assert torch.equal(out_srv[pi], ref), pi
         assert torch.equal(out_small[pi], ref), pi
+
+
+def test_consumed_outputs_match_kept_ones():
+    """Without keep_outputs the server reduces every output -- the metric right after setup, each unit of
+    (ij|P) when its step retires -- to (sum, 2-norm) on the device; these equal the same reductions of the
+    outputs a keep_outputs server holds."""
+    import torch
+    from electron_integral_playground_amd.serving import IntegralServer
+    mol = _job(10, 51)
+    kept = _drain(IntegralServer(budget_bytes=8e9, mixed=True, keep_outputs=True), [(0, mol)])
+    used = _drain(IntegralServer(budget_bytes=8e9, mixed=True, keep_outputs=False), [(0, mol)])
+    J = kept.jobs[0].J2c
+    assert used.jobs[0].J2c is None
+    assert torch.equal(used.jobs[0].J2c_checksum, torch.stack([J.sum(), torch.linalg.vector_norm(J)]))
+    assert len(kept.jobs[0].units) == len(used.jobs[0].units)
+    for uk, uu in zip(kept.jobs[0].units, used.jobs[0].units):
+        assert torch.equal(uk.checksum, uu.checksum)
