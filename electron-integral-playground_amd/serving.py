@@ -31,6 +31,8 @@ UNIT_CAP_FRACTION = 0.25      # a unit's output may use at most this share of th
 INPUT_FRACTION = 0.25         # admitted jobs' resident inputs (pair tables, metric) use at most this share
 BYPASS_SECONDS = 5.0          # a blocked head unit may be overtaken by smaller units for this long
 SETUP_CONCURRENCY = 3         # jobs in device setup at once (their transient allocations are not gated)
+LARGE_JOB_ATOMS = 80          # jobs at least this large (by the submitter's size hint) prepare in their own
+LARGE_PREP_WORKERS = 4        # pool, so a burst of long preparations never holds the small jobs
 # Device memory that PyTorch keeps cached (freed, not returned) is unavailable to the HIP runtime's own
 # allocations -- e.g. the scratch memory a queue needs when it first runs a kernel with a private
 # segment.  Finished jobs' inputs are returned to the device once the cache holds more than this.
@@ -191,6 +193,7 @@ class IntegralServer:
         self.gate = InputGate(INPUT_FRACTION * self.budget)
         self._setup_slots = threading.Semaphore(SETUP_CONCURRENCY)
         self.pool = cf.ThreadPoolExecutor(prep_workers)
+        self.large_pool = cf.ThreadPoolExecutor(LARGE_PREP_WORKERS)
         self.log = log
         self.prepared = deque()       # host-prepared jobs awaiting device setup and admission
         self.pending_units = deque()  # units of admitted jobs not yet launched
@@ -204,13 +207,15 @@ class IntegralServer:
         self.n_steps = 0
 
     # -- host preparation (thread pool) --------------------------------------------------------
-    def submit(self, job_id, molecule, arrival=None):
+    def submit(self, job_id, molecule, arrival=None, size_hint=None):
         """Accept a job (a Molecule, or a callable returning one); its geometry, basis and pair tables
-        are built on a host thread."""
+        are built on a host thread.  ``size_hint`` (atoms) routes jobs of LARGE_JOB_ATOMS or more to the
+        large-job preparation pool."""
         job = Job(job_id, time.perf_counter() if arrival is None else arrival, molecule)
         self.jobs[job_id] = job
         job.ticket = self.gate.ticket()
-        fut = self.pool.submit(self._prepare, job)
+        pool = self.large_pool if size_hint is not None and size_hint >= LARGE_JOB_ATOMS else self.pool
+        fut = pool.submit(self._prepare, job)
         self.prepared.append((job, fut))
         return job
 
@@ -465,6 +470,7 @@ class IntegralServer:
 
     def close(self):
         self.pool.shutdown(wait=True)
+        self.large_pool.shutdown(wait=True)
 
     # -- results ------------------------------------------------------------------------------------
     def job_output(self, job_id):
@@ -490,7 +496,7 @@ def run_poisson(server, jobs, make_molecule, log=None):
         now = time.perf_counter() - t0
         while i < n and jobs[i][0] <= now:
             t_arr, n_atom, seed = jobs[i]
-            server.submit(i, (lambda n=n_atom, sd=seed: make_molecule(n, sd)), arrival=t0 + t_arr)
+            server.submit(i, (lambda n=n_atom, sd=seed: make_molecule(n, sd)), arrival=t0 + t_arr, size_hint=n_atom)
             i += 1
         if not server.poll():
             time.sleep(2e-4)
