@@ -256,6 +256,7 @@ def serving_leg(n_jobs, rate, budget=200e9, mixed=True, part=0, latency_file=Non
         pathlib.Path(latency_file).write_text(json.dumps({"part": part, "latencies_s": lat.tolist(),
                                                           "job_ids": [j.job_id for j in done],
                                                           "atoms": [jobs[j.job_id][1] for j in done],
+                                                          "phases_s": [j.phases() for j in done],
                                                           "integrals": n_int, "span_s": span}))
     log(f"serving: {res}")
     return res

@@ -152,9 +152,12 @@ class Job:
     input_bytes: int = 0
     error: Optional[str] = None
     ticket: int = 0
-    # host clock: preparation done, past the input gate, device setup done, first unit launched
+    # host clock: preparation started (a pool thread took the job) and done, past the input gate,
+    # a setup slot taken, device setup done, first unit launched
+    t_prep_start: Optional[float] = None
     t_prepared: Optional[float] = None
     t_gate: Optional[float] = None
+    t_setup_start: Optional[float] = None
     t_setup: Optional[float] = None
     t_first_launch: Optional[float] = None
 
@@ -162,14 +165,17 @@ class Job:
     def latency(self):
         return None if self.completion is None else self.completion - self.arrival
 
+    PHASES = ("prep_wait", "prepare", "gate", "setup_wait", "setup", "queue", "run")
+
     def phases(self):
-        """Seconds spent in host preparation, waiting at the input gate, device setup, waiting for a
-        step, and from the first launch to completion."""
-        t = [self.arrival, self.t_prepared, self.t_gate, self.t_setup, self.t_first_launch, self.completion]
+        """Seconds spent waiting for a preparation thread, in host preparation, waiting at the input
+        gate, waiting for a setup slot, in device setup, waiting for a step, and from the first launch
+        to completion (the seven add up to the latency)."""
+        t = [self.arrival, self.t_prep_start, self.t_prepared, self.t_gate, self.t_setup_start, self.t_setup,
+             self.t_first_launch, self.completion]
         if any(x is None for x in t):
             return None
-        return {"prepare": t[1] - t[0], "gate": t[2] - t[1], "setup": t[3] - t[2], "queue": t[4] - t[3],
-                "run": t[5] - t[4]}
+        return {k: t[i + 1] - t[i] for i, k in enumerate(self.PHASES)}
 
 
 class IntegralServer:
@@ -223,6 +229,7 @@ class IntegralServer:
         """Host thread: geometry/basis, pair tables, then -- once the job's resident inputs pass the input
         gate -- device setup on this thread's own stream.  Errors are re-raised through the future."""
         host, error, est = None, None, 0
+        job.t_prep_start = time.perf_counter()
         try:
             if callable(job.molecule):
                 job.molecule = job.molecule()
@@ -239,6 +246,7 @@ class IntegralServer:
             raise error
         try:
             with self._setup_slots:
+                job.t_setup_start = time.perf_counter()
                 self._setup_device(job, host)
         finally:
             self.gate.adjust((job.input_bytes if job.plan is not None else 0) - est)

@@ -445,3 +445,17 @@ def test_serving_steps_take_units_round_robin_across_jobs():
     assert names(server.pending_units) == [("a", 2), ("a", 3), ("a", 4), ("c", 1)]
     # the arena is full: nothing else fits, everything stays pending
     assert server._form_step() == [] and len(server.pending_units) == 4
+
+
+def test_serving_job_phases_add_up_to_the_latency():
+    """The per-job phases partition arrival -> completion; a job missing a timestamp has none."""
+    from electron_integral_playground_amd.serving import Job
+    j = Job(7, 10.0, None)
+    assert j.phases() is None
+    (j.t_prep_start, j.t_prepared, j.t_gate, j.t_setup_start, j.t_setup, j.t_first_launch,
+     j.completion) = 10.5, 12.0, 12.25, 13.0, 14.5, 14.75, 17.0
+    ph = j.phases()
+    assert list(ph) == list(Job.PHASES)
+    assert ph == {"prep_wait": 0.5, "prepare": 1.5, "gate": 0.25, "setup_wait": 0.75, "setup": 1.5,
+                  "queue": 0.25, "run": 2.25}
+    assert abs(sum(ph.values()) - j.latency) < 1e-12
