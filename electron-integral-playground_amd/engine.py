@@ -110,6 +110,19 @@ def longest_first(costs):
     return stored
 
 
+def longest_first_t(costs):
+    """longest_first on a torch tensor, on its device (the same permutation; int64 result)."""
+    import torch
+    n = costs.shape[0]
+    order = torch.argsort(-costs.to(torch.int64), stable=True)
+    b = torch.arange(n, dtype=torch.int64, device=costs.device)
+    per, rem = divmod(n, 8)
+    x = b % 8
+    stored = torch.empty(n, dtype=torch.int64, device=costs.device)
+    stored[x * per + torch.clamp(x, max=rem) + b // 8] = order
+    return stored
+
+
 def three_center_flops_per_triplet(la, lb, lc):
     """Fixed algorithmic flop model of one primitive triplet (FMA = 2):
     Boys 20(L+1)+20, R recursion 2 C(L+4, 4), pair E 9(la+1)(lb+1)(la+lb+2)/2, auxiliary contraction
@@ -244,6 +257,43 @@ def mixed_split_groups(pre, q, ks_prim_off, aux_order="natural"):
     return s, perm, gprim
 
 
+def mixed_split_pairs_t(sp_pp_off, p, ub, threshold, device):
+    """mixed_split_pairs with the per-primitive-pair arrays on ``device`` (torch; same values)."""
+    import torch
+    off = torch.as_tensor(np.asarray(sp_pp_off, dtype=np.int64), device=device)
+    n_sp = off.shape[0] - 1
+    sp_of_pp = torch.arange(n_sp, dtype=torch.int64, device=device).repeat_interleave(off[1:] - off[:-1])
+    diffuse = torch.as_tensor(np.asarray(p, dtype=np.float64), device=device) < DIFFUSE_PAIR_EXPONENT
+    nd = ~diffuse
+    ub_nd = torch.as_tensor(np.asarray(ub, dtype=np.float64), device=device)[nd]
+    # a zero bound needs Q_P = inf, or nothing at threshold 0 (numpy's 0/0 = nan sorts last: float64)
+    need = torch.where(ub_nd > 0, threshold / ub_nd, torch.full_like(ub_nd, float("inf") if threshold > 0 else 0.0))
+    return {"n_sp": n_sp, "n_diff": torch.bincount(sp_of_pp[diffuse], minlength=n_sp),
+            "sp_nd": sp_of_pp[nd], "neg_need": -need}
+
+
+def mixed_split_groups_t(pre, q, ks_prim_off, aux_order="natural"):
+    """mixed_split_groups from mixed_split_pairs_t: s as an (n_sp, G) int64 tensor on the pairs'
+    device; perm and gprim (one entry per auxiliary shell or group) stay numpy."""
+    import torch
+    if aux_order not in ("qp", "natural"):
+        raise ValueError("aux_order is 'qp' or 'natural'")
+    n_sp, device = pre["n_sp"], pre["sp_nd"].device
+    q = np.asarray(q, dtype=np.float64)
+    perm = (np.argsort(-q, kind="stable") if aux_order == "qp" else np.arange(len(q))).astype(np.int32)
+    G = (len(q) + 63) // 64
+    starts = np.arange(G) * 64
+    qmax = np.maximum.reduceat(q[perm], starts)
+    gprim = np.add.reduceat(np.diff(np.asarray(ks_prim_off, dtype=np.int64))[perm], starts)
+    by_q = np.argsort(-qmax, kind="stable")
+    g_min = torch.searchsorted(torch.as_tensor(-qmax[by_q], device=device), pre["neg_need"], right=True)
+    hist = torch.bincount(pre["sp_nd"] * (G + 1) + g_min, minlength=n_sp * (G + 1)).reshape(n_sp, G + 1)
+    at_least = torch.flip(torch.cumsum(torch.flip(hist, [1]), 1), [1])
+    s = torch.empty((n_sp, G), dtype=torch.int64, device=device)
+    s[:, torch.as_tensor(by_q, device=device)] = pre["n_diff"][:, None] + at_least[:, 1:]
+    return s, perm, gprim
+
+
 _TUNING_APPLIED = False
 
 
@@ -308,34 +358,44 @@ class ThreeCenterPlan:
             d["col"] = T(ks, torch.int32)
             self.aux.append(ac)
         # work items: every (shell pair, block of 64 auxiliary shells), in shell-pair order (sliced by
-        # shell-pair ranges when streaming) and stored for longest-first dispatch (whole-class launches)
+        # shell-pair ranges when streaming), built on the device (a 250-atom molecule has ~2e8 items);
+        # whole-class launches read them stored longest-first (items_lpt, built on first use)
         self.items: Dict[Tuple[int, int], object] = {}
         self.items_lpt: Dict[Tuple[int, int], object] = {}
         for pi, pc in enumerate(self.pairs):
-            n_pp_sp = np.diff(pc.sp_pp_off).astype(np.int64)
             for ai, ac in enumerate(self.aux):
                 kb = (ac.n_ks + 63) // 64
-                sp = np.repeat(np.arange(pc.n_sp, dtype=np.int32), kb)
-                k0 = np.tile(np.arange(kb, dtype=np.int32) * 64, pc.n_sp)
-                items = np.stack([sp, k0], axis=1)
-                self.items[(pi, ai)] = T(items, torch.int32)
-                # cost of an item: primitive pairs x the block's largest auxiliary primitive count
-                nkp = np.diff(ac.ks_prim_off.astype(np.int64))
-                kmax = np.maximum.reduceat(nkp, np.arange(kb) * 64) if len(nkp) else np.ones(kb, np.int64)
-                cost = n_pp_sp[sp] * kmax[k0 // 64]
-                self.items_lpt[(pi, ai)] = T(items[longest_first(cost)], torch.int32)
-        # packed rows no kernel writes (shell pairs screened out entirely): zeroed by run()
-        written = np.zeros(self.n_packed, dtype=bool)
-        for pc in self.pairs:
-            ni, nj = 2 * pc.la + 1, 2 * pc.lb + 1
-            i = (pc.sp_i[:, None, None] + np.arange(ni)[None, :, None]).astype(np.int64)
-            j = (pc.sp_j[:, None, None] + np.arange(nj)[None, None, :]).astype(np.int64)
-            hi, lo = np.maximum(i, j), np.minimum(i, j)
-            written[(hi * (hi + 1) // 2 + lo).ravel()] = True
-        self.zero_rows = T(np.flatnonzero(~written).astype(np.int32), torch.int32)
-        self.zero_spans = T(zero_spans(np.flatnonzero(~written), self.n_aux), torch.int64)
+                sp = torch.arange(pc.n_sp, dtype=torch.int32, device=self.device).repeat_interleave(kb)
+                k0 = (torch.arange(kb, dtype=torch.int32, device=self.device) * 64).repeat(pc.n_sp)
+                self.items[(pi, ai)] = torch.stack([sp, k0], 1)
+        self._zero = None
         self._job_tables = {}
         torch.cuda.current_stream().synchronize()
+
+    def _zero_tables(self):
+        """(rows, spans) of the packed rows no kernel writes (shell pairs screened out entirely), which
+        run() zeroes; built on first use, since the compact block outputs never need them."""
+        if self._zero is None:
+            torch = self.torch
+            written = np.zeros(self.n_packed, dtype=bool)
+            for pc in self.pairs:
+                ni, nj = 2 * pc.la + 1, 2 * pc.lb + 1
+                i = (pc.sp_i[:, None, None] + np.arange(ni)[None, :, None]).astype(np.int64)
+                j = (pc.sp_j[:, None, None] + np.arange(nj)[None, None, :]).astype(np.int64)
+                hi, lo = np.maximum(i, j), np.minimum(i, j)
+                written[(hi * (hi + 1) // 2 + lo).ravel()] = True
+            rows = np.flatnonzero(~written)
+            self._zero = (torch.as_tensor(rows.astype(np.int32), device=self.device),
+                          torch.as_tensor(zero_spans(rows, self.n_aux), dtype=torch.int64, device=self.device))
+        return self._zero
+
+    @property
+    def zero_rows(self):
+        return self._zero_tables()[0]
+
+    @property
+    def zero_spans(self):
+        return self._zero_tables()[1]
 
     # -- sizes -----------------------------------------------------------------------------------
     @property
@@ -363,16 +423,36 @@ class ThreeCenterPlan:
                 return sum(size(x) for x in v)
             return 0
         total = sum(size(pc.dev) for pc in self.pairs) + sum(size(ac.dev) for ac in self.aux)
-        total += size(self.items) + size(getattr(self, "mixed", {})) + size(self.zero_rows)
+        total += size(self.items) + size(self.items_lpt) + size(getattr(self, "mixed", {})) + size(self._zero or ())
         return int(total)
 
     # -- execution -------------------------------------------------------------------------------
     def new_output(self):
         return self.torch.empty((self.n_packed, self.n_aux), dtype=self.torch.float64, device=self.device)
 
+    def class_items(self, pi, ai):
+        """Items of a whole-class launch, stored longest-first; built on first use and kept (the first
+        call synchronises the current stream, since the launch may go to another stream)."""
+        it = self.items_lpt.get((pi, ai))
+        if it is None:
+            torch = self.torch
+            pc, ac = self.pairs[pi], self.aux[ai]
+            kb = (ac.n_ks + 63) // 64
+            # cost of an item: primitive pairs x the block's largest auxiliary primitive count
+            n_pp_sp = np.diff(np.asarray(pc.sp_pp_off, dtype=np.int64))
+            nkp = np.diff(ac.ks_prim_off.astype(np.int64))
+            kmax = np.maximum.reduceat(nkp, np.arange(kb) * 64) if len(nkp) else np.ones(kb, np.int64)
+            cost = (torch.as_tensor(n_pp_sp, device=self.device).repeat_interleave(kb)
+                    * torch.as_tensor(kmax, dtype=torch.int64, device=self.device).repeat(pc.n_sp))
+            it = self.items[(pi, ai)].index_select(0, longest_first_t(cost))
+            if self.device.type == "cuda":
+                torch.cuda.current_stream(self.device).synchronize()
+            self.items_lpt[(pi, ai)] = it
+        return it
+
     def launch_class(self, pi, ai, out, stream=None, use_f32=False):
         pc, ac = self.pairs[pi], self.aux[ai]
-        it = self.items_lpt[(pi, ai)]
+        it = self.class_items(pi, ai)
         dp, da = pc.dev, ac.dev
         rc = self.lib.eip_three_center_items(
             pc.la, pc.lb, ac.lc, int(use_f32), _ptr(it), _vp(0), it.shape[0], _ptr(dp["off"]), _ptr(dp["si"]),
@@ -403,6 +483,9 @@ class ThreeCenterPlan:
         if out is None:
             out = self.new_output()
         zero = zero and self.zero_spans.shape[0] > 0
+        for pi in range(len(self.pairs)):                 # dispatch tables exist before any launch
+            for ai in range(len(self.aux)):
+                self.class_items(pi, ai)
         if not concurrent:
             if zero:
                 self.zero_unwritten(out, stream)
@@ -513,36 +596,51 @@ class ThreeCenterPlan:
         self._mixed_generation = self.__dict__.get("_mixed_generation", 0) + 1
         self.mixed_stats = {"f64_triplets": 0, "f32_triplets": 0, "threshold": threshold,
                             "aux_order": aux_order, "granularity": granularity}
+        dev = self.device
+        f64_total = torch.zeros((), dtype=torch.int64, device=dev)
+        all_total = torch.zeros((), dtype=torch.int64, device=dev)
+        # Item tables are built on the device: a 250-atom job has ~2e8 (shell pair, 64 auxiliary
+        # shells) items, whose host construction took ~20 s.  off64/off32[sp] = items of shell pairs
+        # before sp in each table (tables are in shell-pair order), so block slices need no search.
         for pi, pc in enumerate(self.pairs):
             ub = _as_arrays(pair_list[(pc.la, pc.lb)]).upper_bound[self._pair_orders[pi]]
-            pre = mixed_split_pairs(pc.sp_pp_off, self._pair_p[pi], ub, threshold)
+            pre = mixed_split_pairs_t(pc.sp_pp_off, self._pair_p[pi], ub, threshold, dev)
+            off = torch.as_tensor(np.asarray(pc.sp_pp_off, dtype=np.int64), device=dev)
+            n_pp_sp = off[1:] - off[:-1]
             for ai, ac in enumerate(self.aux):
-                split_rows, perm, gprim = mixed_split_groups(pre, q_aux[ai], ac.ks_prim_off, aux_order)
+                s, perm, gprim = mixed_split_groups_t(pre, q_aux[ai], ac.ks_prim_off, aux_order)
                 G = len(gprim)
-                n_pp_sp = np.diff(pc.sp_pp_off).astype(np.int64)
-                sp = np.repeat(np.arange(pc.n_sp, dtype=np.int32), G)
-                k0 = np.tile(np.arange(G, dtype=np.int32) * 64, pc.n_sp)
-                pp0 = np.repeat(pc.sp_pp_off[:-1].astype(np.int64), G)
-                pp1 = np.repeat(pc.sp_pp_off[1:].astype(np.int64), G)
-                split = pp0 + split_rows.ravel()
-                items = np.stack([sp, k0], axis=1)
-                trip = (pp1 - pp0) * np.tile(gprim, pc.n_sp)                 # triplets per item
+                gp = torch.as_tensor(gprim, dtype=torch.int64, device=dev)
+                sp = torch.arange(pc.n_sp, dtype=torch.int32, device=dev).repeat_interleave(G)
+                k0 = (torch.arange(G, dtype=torch.int32, device=dev) * 64).repeat(pc.n_sp)
+                items = torch.stack([sp, k0], 1)
+                npp = n_pp_sp.repeat_interleave(G)                      # pp1 - pp0 per item
+                srow = s.reshape(-1)
+                trip = npp * gp.repeat(pc.n_sp)                         # triplets per item
+                all_total += trip.sum()
+
+                def offsets(mask):
+                    per_sp = mask.reshape(pc.n_sp, G).sum(1) if G else torch.zeros(pc.n_sp, dtype=torch.int64, device=dev)
+                    return np.concatenate([[0], np.cumsum(per_sp.cpu().numpy())]).astype(np.int64)
+
                 if granularity == "pair":
-                    f32 = split < pp1
-                    f64_trip = int(np.sum(split_rows * gprim[None, :]))
-                    m = dict(it64=T(items, torch.int32), pp64=T(np.stack([pp0, split], axis=1), torch.int32),
-                             sp64=sp, it32=T(items[f32], torch.int32),
-                             pp32=T(np.stack([split[f32], pp1[f32]], axis=1), torch.int32), sp32=sp[f32],
-                             acc32=1, cost64=split - pp0, cost32=(pp1 - split)[f32])
+                    f32 = srow < npp
+                    pp0 = off[:-1].repeat_interleave(G)
+                    split = pp0 + srow
+                    f64_total += (s * gp[None, :]).sum()
+                    m = dict(it64=items, pp64=torch.stack([pp0, split], 1).to(torch.int32),
+                             off64=np.arange(pc.n_sp + 1, dtype=np.int64) * G, it32=items[f32],
+                             pp32=torch.stack([split[f32], (pp0 + npp)[f32]], 1).to(torch.int32),
+                             off32=offsets(f32), acc32=1, cost64=srow, cost32=(npp - srow)[f32])
                 else:
-                    f32 = split == pp0
-                    f64_trip = int(np.sum(trip[~f32]))
-                    m = dict(it64=T(items[~f32], torch.int32), pp64=None, sp64=sp[~f32],
-                             it32=T(items[f32], torch.int32), pp32=None, sp32=sp[f32], acc32=0,
-                             cost64=(pp1 - pp0)[~f32], cost32=(pp1 - pp0)[f32])
-                self.mixed_stats["f64_triplets"] += f64_trip
-                self.mixed_stats["f32_triplets"] += int(np.sum(trip)) - f64_trip
+                    f32 = srow == 0
+                    f64_total += trip[~f32].sum()
+                    m = dict(it64=items[~f32], pp64=None, off64=offsets(~f32), it32=items[f32], pp32=None,
+                             off32=offsets(f32), acc32=0, cost64=npp[~f32], cost32=npp[f32])
                 self.mixed[(pi, ai)] = dict(perm=T(perm, torch.int32), G=G, **m)
+        f64_trip, all_trip = int(f64_total.item()), int(all_total.item())
+        self.mixed_stats["f64_triplets"] = f64_trip
+        self.mixed_stats["f32_triplets"] = all_trip - f64_trip
         return self.mixed_stats
 
     def run_mixed(self, out=None, stream=None, zero=True):
@@ -565,8 +663,8 @@ class ThreeCenterPlan:
     def mixed_items(self, pi, ai, sp0, sp1):
         """(items64, pp64, items32, pp32, accumulate32) of shell pairs [sp0, sp1) (views; pp may be None)."""
         m = self.mixed[(pi, ai)]
-        a, b = np.searchsorted(m["sp64"], [sp0, sp1])
-        c, d = np.searchsorted(m["sp32"], [sp0, sp1])
+        a, b = int(m["off64"][sp0]), int(m["off64"][sp1])
+        c, d = int(m["off32"][sp0]), int(m["off32"][sp1])
         pp64 = m["pp64"][a:b] if m["pp64"] is not None else None
         pp32 = m["pp32"][c:d] if m["pp32"] is not None else None
         return m["it64"][a:b], pp64, m["it32"][c:d], pp32, m["acc32"]
@@ -627,15 +725,16 @@ class ThreeCenterPlan:
         if kind == "plain":
             G = (self.aux[ai].n_ks + 63) // 64
             it, pp = self.items[(pi, ai)][sp0 * G:sp1 * G], None
-            cost = np.repeat(np.diff(self.pairs[pi].sp_pp_off)[sp0:sp1].astype(np.int64), G)
+            n_pp_sp = np.diff(np.asarray(self.pairs[pi].sp_pp_off, dtype=np.int64))[sp0:sp1]
+            cost = torch.as_tensor(n_pp_sp, device=self.device).repeat_interleave(G)
         else:
             m, k = self.mixed[(pi, ai)], kind[1:]
-            a, b = np.searchsorted(m["sp" + k], [sp0, sp1])
+            a, b = int(m["off" + k][sp0]), int(m["off" + k][sp1])
             it = m["it" + k][a:b]
             pp = m["pp" + k][a:b] if m["pp" + k] is not None else None
             cost = m["cost" + k][a:b]
-        if len(cost) > 1:
-            order = torch.as_tensor(longest_first(cost), dtype=torch.int64, device=self.device)
+        if cost.shape[0] > 1:
+            order = longest_first_t(cost)
             it = it.index_select(0, order)
             pp = pp.index_select(0, order) if pp is not None else None
             # the launch may go to another stream than the one that reordered the tables

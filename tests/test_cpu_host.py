@@ -412,7 +412,7 @@ def test_block_tables_follow_the_current_mixed_split():
         for sp0, sp1 in ((0, n_sp), (7, 23)):
             for kind in ("f64", "f32"):
                 it, pp = plan._block_items(0, 0, sp0, sp1, kind)
-                a, b = np.searchsorted(m["sp" + kind[1:]], [sp0, sp1])
+                a, b = int(m["off" + kind[1:]][sp0]), int(m["off" + kind[1:]][sp1])
                 ref_pp = m["pp" + kind[1:]]
                 assert (pp is None) == (ref_pp is None)
                 assert rows(it, pp) == rows(m["it" + kind[1:]][a:b], ref_pp[a:b] if ref_pp is not None else None)
@@ -459,3 +459,92 @@ def test_serving_job_phases_add_up_to_the_latency():
     assert ph == {"prep_wait": 0.5, "prepare": 1.5, "gate": 0.25, "setup_wait": 0.75, "setup": 1.5,
                   "queue": 0.25, "run": 2.25}
     assert abs(sum(ph.values()) - j.latency) < 1e-12
+
+
+def test_longest_first_on_the_device_matches_the_host_order():
+    """longest_first_t (torch) gives longest_first's permutation, ties included."""
+    import torch
+    from electron_integral_playground_amd.engine import longest_first, longest_first_t
+    rng = np.random.default_rng(8)
+    for n in (1, 2, 8, 9, 63, 1000, 4097):
+        costs = rng.integers(0, 5, n)                 # many ties
+        got = longest_first_t(torch.as_tensor(costs)).numpy()
+        assert np.array_equal(got, longest_first(costs)), n
+
+
+def test_mixed_split_on_the_device_matches_the_host_split():
+    """mixed_split_pairs_t / mixed_split_groups_t (torch) equal the numpy split, including diffuse
+    pairs, pairs of zero bound, threshold 0 and groups of equal largest Q_P."""
+    import torch
+    from electron_integral_playground_amd.engine import (mixed_split, mixed_split_groups_t,
+                                                         mixed_split_pairs_t)
+    rng = np.random.default_rng(9)
+    for case in range(6):
+        n_sp, n_ks = int(rng.integers(1, 60)), int(rng.integers(1, 300))
+        off = np.concatenate([[0], np.cumsum(rng.integers(0, 7, n_sp))]).astype(np.int64)
+        n_pp = int(off[-1])
+        p = rng.uniform(0.05, 4.0, n_pp)
+        ub = 10.0 ** rng.uniform(-12, -3, n_pp)
+        ub[rng.random(n_pp) < 0.05] = 0.0
+        q = np.round(rng.uniform(0.01, 5.0, n_ks), 1)    # ties in Q_P and in group maxima
+        ks_prim_off = np.concatenate([[0], np.cumsum(rng.integers(1, 5, n_ks))])
+        thr = 0.0 if case == 5 else 1e-6
+        for order in ("natural", "qp"):
+            s_ref, perm_ref, gprim_ref = mixed_split(off, p, ub, q, ks_prim_off, thr, aux_order=order)
+            pre = mixed_split_pairs_t(off, p, ub, thr, torch.device("cpu"))
+            s, perm, gprim = mixed_split_groups_t(pre, q, ks_prim_off, order)
+            assert np.array_equal(s.numpy(), s_ref), (case, order)
+            assert np.array_equal(perm, perm_ref) and np.array_equal(gprim, gprim_ref)
+
+
+def test_setup_mixed_tables_match_a_host_construction():
+    """setup_mixed's device-built tables hold exactly the items, pair ranges, per-shell-pair offsets
+    and costs that the numpy split defines, for both granularities."""
+    import torch
+    from electron_integral_playground_amd.engine import AuxClass, PairClass, ThreeCenterPlan, mixed_split
+    from electron_integral_playground_amd.pair_utility import PairArrays
+    rng = np.random.default_rng(12)
+    n_sp, n_ks = 37, 150
+    off = np.concatenate([[0], np.cumsum(rng.integers(1, 9, n_sp))]).astype(np.int32)
+    n_pp = int(off[-1])
+    plan = object.__new__(ThreeCenterPlan)
+    plan.torch, plan.device = torch, torch.device("cpu")
+    ks_prim_off = np.concatenate([[0], np.cumsum(rng.integers(1, 4, n_ks))]).astype(np.int32)
+    plan.pairs = [PairClass(1, 2, n_pp, n_sp, None, None, off)]
+    plan.aux = [AuxClass(2, n_ks, int(ks_prim_off[-1]), None, ks_prim_off)]
+    plan._pair_orders = [np.arange(n_pp)]
+    plan._pair_p = [rng.uniform(0.1, 5.0, n_pp)]
+    ub = np.sort(10.0 ** rng.uniform(-10, -4, n_pp))[::-1]
+    pl = {(1, 2): PairArrays(np.zeros(n_pp, np.int64), np.zeros(n_pp, np.int64), np.zeros(n_pp, np.int64),
+                             np.zeros(n_pp, np.int64), ub)}
+    q = rng.uniform(0.05, 5.0, n_ks)
+    s, _, gprim = mixed_split(off, plan._pair_p[0], ub, q, ks_prim_off, 1e-6)
+    G = len(gprim)
+    sp = np.repeat(np.arange(n_sp), G)
+    k0 = np.tile(np.arange(G) * 64, n_sp)
+    pp0 = np.repeat(off[:-1].astype(np.int64), G)
+    pp1 = np.repeat(off[1:].astype(np.int64), G)
+    split = pp0 + s.ravel()
+    trip = (pp1 - pp0) * np.tile(gprim, n_sp)
+    for gran in ("item", "pair"):
+        stats = plan.setup_mixed(pl, [q], threshold=1e-6, granularity=gran)
+        m = plan.mixed[(0, 0)]
+        if gran == "item":
+            f32 = split == pp0
+            sel64, sel32 = ~f32, f32
+            assert m["pp64"] is None and m["pp32"] is None and m["acc32"] == 0
+            assert np.array_equal(m["cost64"].numpy(), (pp1 - pp0)[sel64])
+            assert stats["f64_triplets"] == int(trip[sel64].sum())
+        else:
+            f32 = split < pp1
+            sel64, sel32 = np.ones(len(sp), bool), f32
+            assert np.array_equal(m["pp64"].numpy(), np.stack([pp0, split], 1))
+            assert np.array_equal(m["pp32"].numpy(), np.stack([split[f32], pp1[f32]], 1))
+            assert np.array_equal(m["cost64"].numpy(), split - pp0)
+            assert np.array_equal(m["cost32"].numpy(), (pp1 - split)[f32])
+            assert stats["f64_triplets"] == int(np.sum(s * gprim[None, :])) and m["acc32"] == 1
+        assert stats["f64_triplets"] + stats["f32_triplets"] == int(trip.sum())
+        for k, sel in (("64", sel64), ("32", sel32)):
+            assert np.array_equal(m["it" + k].numpy(), np.stack([sp, k0], 1)[sel]), (gran, k)
+            counts = np.bincount(sp[sel], minlength=n_sp)
+            assert np.array_equal(m["off" + k], np.concatenate([[0], np.cumsum(counts)])), (gran, k)
