@@ -35,8 +35,12 @@ LARGE_JOB_ATOMS = 80          # jobs at least this large (by the submitter's siz
 LARGE_PREP_WORKERS = 4        # pool, so a burst of long preparations never holds the small jobs
 # Device memory that PyTorch keeps cached (freed, not returned) is unavailable to the HIP runtime's own
 # allocations -- e.g. the scratch memory a queue needs when it first runs a kernel with a private
-# segment.  Finished jobs' inputs are returned to the device once the cache holds more than this.
+# segment.  Finished jobs' inputs are returned to the device once the cache holds more than
+# CACHE_SLACK_BYTES, at a moment when no step is in flight: returning memory (hipFree) waits for the
+# whole device, so a release while a step of a large job runs stalls the scheduler and every thread
+# that allocates for seconds.  Past CACHE_SLACK_MAX_BYTES it is returned at once.
 CACHE_SLACK_BYTES = 8e9
+CACHE_SLACK_MAX_BYTES = 48e9
 
 
 def _q_aux(plan, J2c):
@@ -204,6 +208,7 @@ class IntegralServer:
         self.prepared = deque()       # host-prepared jobs awaiting device setup and admission
         self.pending_units = deque()  # units of admitted jobs not yet launched
         self.in_flight = deque()      # (event, [units])
+        self._release_pending = False  # finished jobs freed inputs; the cache may need returning
         self.outputs_used = 0.0       # bytes of unit outputs in flight (or kept)
         # unit outputs live in one arena: the budget minus the share reserved for resident inputs
         self.arena = OutputArena(torch, (1.0 - INPUT_FRACTION) * self.budget // 8, self.device)
@@ -452,9 +457,14 @@ class IntegralServer:
                             job.J2c = None
                             self.gate.adjust(-job.input_bytes)
                             released = True
-            if released and torch.cuda.memory_reserved(self.device) - torch.cuda.memory_allocated(self.device) \
-                    > CACHE_SLACK_BYTES:
+        self._release_pending |= released
+        if self._release_pending:
+            slack = torch.cuda.memory_reserved(self.device) - torch.cuda.memory_allocated(self.device)
+            if slack <= CACHE_SLACK_BYTES:
+                self._release_pending = False
+            elif not self.in_flight or slack > CACHE_SLACK_MAX_BYTES:
                 torch.cuda.empty_cache()
+                self._release_pending = False
 
     def poll(self):
         """One scheduling iteration: retire finished steps, admit prepared jobs, launch a step."""

@@ -548,3 +548,35 @@ def test_setup_mixed_tables_match_a_host_construction():
             assert np.array_equal(m["it" + k].numpy(), np.stack([sp, k0], 1)[sel]), (gran, k)
             counts = np.bincount(sp[sel], minlength=n_sp)
             assert np.array_equal(m["off" + k], np.concatenate([[0], np.cumsum(counts)])), (gran, k)
+
+
+def test_serving_returns_cached_memory_only_between_steps():
+    """Freed inputs go back to the device (a call that waits for the whole device) when no step is in
+    flight, or at once past the hard limit; small slack is kept."""
+    from collections import deque
+    from types import SimpleNamespace
+    from electron_integral_playground_amd import serving
+    calls = []
+    state = {"reserved": 0.0}
+    cuda = SimpleNamespace(memory_reserved=lambda d: state["reserved"], memory_allocated=lambda d: 0.0,
+                           empty_cache=lambda: calls.append(state["reserved"]))
+    server = object.__new__(serving.IntegralServer)
+    server.torch, server.device, server._release_pending = SimpleNamespace(cuda=cuda), "cpu", False
+    busy = SimpleNamespace(query=lambda: False)          # a step still running
+    server.in_flight = deque([(busy, [], None)])
+    server._release_pending = True
+    state["reserved"] = serving.CACHE_SLACK_BYTES + 1
+    server._retire()
+    assert calls == [] and server._release_pending            # a step is in flight: wait
+    state["reserved"] = serving.CACHE_SLACK_MAX_BYTES + 1
+    server._retire()
+    assert len(calls) == 1 and not server._release_pending    # past the hard limit: now
+    server._release_pending = True
+    state["reserved"] = serving.CACHE_SLACK_BYTES + 1
+    server.in_flight = deque()
+    server._retire()
+    assert len(calls) == 2 and not server._release_pending    # idle: now
+    server._release_pending = True
+    state["reserved"] = serving.CACHE_SLACK_BYTES / 2
+    server._retire()
+    assert len(calls) == 2 and not server._release_pending    # little slack: kept
