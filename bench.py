@@ -26,6 +26,13 @@ import time
 
 import numpy as np
 
+# Hardware queues per process, read once when the HIP runtime starts: with HIP's default of 4, the
+# serving loop's setup streams share in-order queues with the compute stream and a job's device setup
+# waits behind a multi-second step of a large job (measured: small jobs 14-20 s in setup with 4 queues,
+# none with 16).  Raised here, before anything initialises the runtime.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
+
 ROOT = pathlib.Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 import __graft_entry__  # noqa: E402

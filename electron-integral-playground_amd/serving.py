@@ -16,8 +16,10 @@ does not depend on which other jobs share its launches.
 """
 
 import concurrent.futures as cf
+import os
 import threading
 import time
+import warnings
 from collections import deque
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional
@@ -41,6 +43,11 @@ LARGE_PREP_WORKERS = 4        # pool, so a burst of long preparations never hold
 # that allocates for seconds.  Past CACHE_SLACK_MAX_BYTES it is returned at once.
 CACHE_SLACK_BYTES = 8e9
 CACHE_SLACK_MAX_BYTES = 48e9
+# The server uses a compute stream and one setup stream per setup thread.  The HIP runtime maps streams
+# onto GPU_MAX_HW_QUEUES in-order hardware queues (default 4, fixed when the runtime starts); a setup
+# stream that shares the compute stream's queue waits behind whole steps (measured 14-20 s for small
+# jobs next to a 250-atom one).  Processes that serve should start with at least this many.
+MIN_HW_QUEUES = 8
 
 
 def _q_aux(plan, J2c):
@@ -205,6 +212,9 @@ class IntegralServer:
         self.pool = cf.ThreadPoolExecutor(prep_workers)
         self.large_pool = cf.ThreadPoolExecutor(LARGE_PREP_WORKERS)
         self.log = log
+        if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < MIN_HW_QUEUES:
+            warnings.warn(f"GPU_MAX_HW_QUEUES below {MIN_HW_QUEUES}: device setups may wait behind whole "
+                          "steps; start the serving process with a larger value", RuntimeWarning)
         self.prepared = deque()       # host-prepared jobs awaiting device setup and admission
         self.pending_units = deque()  # units of admitted jobs not yet launched
         self.in_flight = deque()      # (event, [units])
