@@ -39,8 +39,8 @@ LARGE_PREP_WORKERS = 4        # pool, so a burst of long preparations never hold
 # allocations -- e.g. the scratch memory a queue needs when it first runs a kernel with a private
 # segment.  Finished jobs' inputs are returned to the device once the cache holds more than
 # CACHE_SLACK_BYTES, at a moment when no step is in flight: returning memory (hipFree) waits for the
-# whole device, so a release while a step of a large job runs stalls the scheduler and every thread
-# that allocates for seconds.  Past CACHE_SLACK_MAX_BYTES it is returned at once.
+# whole device, so a release while a step of a large job runs holds the scheduler, and every thread
+# that allocates, until that step ends.  Past CACHE_SLACK_MAX_BYTES it is returned at once.
 CACHE_SLACK_BYTES = 8e9
 CACHE_SLACK_MAX_BYTES = 48e9
 # The server uses a compute stream and one setup stream per setup thread.  The HIP runtime maps streams
