@@ -29,8 +29,9 @@
  *     reference's interpolation table (boys_function.hpp:102-176);
  *   - the spherical transform is built at start-up from the solid-harmonic generating formulas
  *     (the reference's gen/cartesian_spherical_transformation.py:8-67), not from stored tables.
- * The three-centre loop is parallelised over auxiliary shells (one thread owns all rows of an
- * auxiliary shell), so no atomics are needed and results do not depend on the thread count.
+ * The three-centre triplets are evaluated in parallel into a buffer, round by round, and each output
+ * row then adds its contributions in a fixed (auxiliary primitive, pair) order on one thread, so no
+ * atomics are needed and results do not depend on the thread count.
  */
 #include <stdbool.h>
 #include <tgmath.h>
@@ -72,12 +73,19 @@ static void cart_xyz(int l, int c, int *x, int *y, int *z)
 }
 
 /* sqrt(ix! iy! iz! (2L)! / ((2ix)! (2iy)! (2iz)! L!)) -- kernel_cartesian_normalization.hpp:22-26 */
-static real_t cart_norm(int l, int c)
+static real_t cart_norm_eval(int l, int c)
 {
     int x, y, z;
     cart_xyz(l, c, &x, &y, &z);
     return sqrt(fact(x) * fact(y) * fact(z) * fact(2 * l) / (fact(2 * x) * fact(2 * y) * fact(2 * z) * fact(l)));
 }
+
+/* Tables filled once by ensure_tables(): normalisation constant and exponents of every Cartesian
+ * component (the inner loops read these instead of re-deriving them per element). */
+static real_t cnorm_tab[OR_MAX_AUX_L + 1][OR_MAX_CART];
+static int cxyz_tab[OR_MAX_AUX_L + 1][OR_MAX_CART][3];
+
+static inline real_t cart_norm(int l, int c) { return cnorm_tab[l][c]; }
 
 /* ------------------------------------------------------------------------------------------ */
 /* Cartesian -> real solid harmonics, built from the generating formulas                        */
@@ -130,6 +138,11 @@ static void build_solid_harmonics(void)
             }
         }
     }
+    for (int l = 0; l <= OR_MAX_AUX_L; l++)
+        for (int c = 0; c < ncart(l); c++) {
+            cnorm_tab[l][c] = cart_norm_eval(l, c);
+            cart_xyz(l, c, &cxyz_tab[l][c][0], &cxyz_tab[l][c][1], &cxyz_tab[l][c][2]);
+        }
     tables_ready = 1;
 }
 
@@ -320,7 +333,7 @@ static void transform_ij(int li, int lj, int nlead, const real_t *cart, real_t *
     const int ni = ncart(li), nj = ncart(lj);
     if (!spherical) { memcpy(out, cart, sizeof(real_t) * nlead * ni * nj); return; }
     const int si = 2 * li + 1, sj = 2 * lj + 1;
-    real_t *tmp = (real_t *) malloc(sizeof(real_t) * ni * sj);
+    real_t tmp[OR_MAX_CART * (2 * OR_MAX_AUX_L + 1)];
     for (int a = 0; a < nlead; a++) {
         const real_t *blk = cart + (size_t) a * ni * nj;
         for (int i = 0; i < ni; i++)
@@ -328,7 +341,6 @@ static void transform_ij(int li, int lj, int nlead, const real_t *cart, real_t *
         for (int i = 0; i < si; i++)
             for (int j = 0; j < sj; j++) out[((size_t) a * si + i) * sj + j] = sph_row_dot(li, i, tmp + j, sj);
     }
-    free(tmp);
 }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -525,6 +537,88 @@ int oracle_two_center(int i_L, int j_L, const double *aux_A_a, const double *aux
 /* Three-centre (P|ij), optionally erf-attenuated                                               */
 /* ------------------------------------------------------------------------------------------ */
 
+/* Per-thread work space of one three-centre triplet evaluation. */
+typedef struct {
+    herm_R R;
+    real_t *W;        /* W[ck][t]: sum_s (-1)^|s| E^k_s R_{t+s} for every pair Hermite index t */
+    real_t *cart;     /* (nk, ni, nj) Cartesian block, normalised */
+    real_t *ijsph;    /* (nk, oi, oj) after the ij transform */
+} j3c_ws;
+
+/* One primitive triplet (pair ip, auxiliary primitive ka) into blk[(k * oi + i) * oj + j]
+ * (reference three_center.cpp:27-173 for the triplet, :216-248 for the transforms).  The
+ * auxiliary-side sum over s is formed once per (Cartesian k component, pair Hermite index t) and
+ * then contracted with E^ij_t for every (i, j) -- the reference's nest :93-160 with its two inner
+ * sums in the other order. */
+static void three_center_triplet(int i_L, int j_L, int k_L, const double *P, const double *A, const double *B,
+                                 double pair_coef, const double *C, double aux_coef, double omega, bool spherical,
+                                 j3c_ws *ws, real_t *blk)
+{
+    const int ni = ncart(i_L), nj = ncart(j_L), nk = ncart(k_L);
+    const int oi = spherical ? 2 * i_L + 1 : ni, oj = spherical ? 2 * j_L + 1 : nj, ok = spherical ? 2 * k_L + 1 : nk;
+    const int Lab = i_L + j_L, S2 = Lab + 1, NT = S2 * S2 * S2;
+    real_t Ex[OR_MAX_L + 1][OR_MAX_L + 1][ESTR], Ey[OR_MAX_L + 1][OR_MAX_L + 1][ESTR], Ez[OR_MAX_L + 1][OR_MAX_L + 1][ESTR];
+    real_t Ek[OR_MAX_AUX_L + 1][OR_MAX_AUX_L + 1];
+    const real_t q = C[3], p = P[3];
+    hermite_E_single(k_L, 0.5 / q, Ek);
+    const real_t coef = (real_t) pair_coef * (real_t) aux_coef;
+    /* reference three_center.cpp:46-49 */
+    const real_t pref = (omega == 0.0) ? coef * 2.0 * pow(OR_PI, 2.5) / (p * q * sqrt(p + q))
+                                       : coef * 2.0 * pow(OR_PI, 2.5) * omega / (p * q * sqrt(p * q + (p + q) * omega * omega));
+    const real_t zeta = (omega == 0.0) ? p * q / (p + q) : 1.0 / (1.0 / p + 1.0 / q + 1.0 / (omega * omega));
+    hermite_E_pair(i_L, j_L, P[0] - A[0], A[0] - B[0], 0.5 / p, Ex);
+    hermite_E_pair(i_L, j_L, P[1] - A[1], A[1] - B[1], 0.5 / p, Ey);
+    hermite_E_pair(i_L, j_L, P[2] - A[2], A[2] - B[2], 0.5 / p, Ez);
+    herm_R_compute(&ws->R, zeta, P[0] - C[0], P[1] - C[1], P[2] - C[2]);
+    for (int ck = 0; ck < nk; ck++) {
+        const int kx = cxyz_tab[k_L][ck][0], ky = cxyz_tab[k_L][ck][1], kz = cxyz_tab[k_L][ck][2];
+        real_t *Wk = ws->W + (size_t) ck * NT;
+        for (int tx = 0; tx <= Lab; tx++)
+            for (int ty = 0; tx + ty <= Lab; ty++)
+                for (int tz = 0; tx + ty + tz <= Lab; tz++) {
+                    real_t w = 0.0;
+                    for (int sx = kx % 2; sx <= kx; sx += 2)
+                        for (int sy = ky % 2; sy <= ky; sy += 2)
+                            for (int sz = kz % 2; sz <= kz; sz += 2)
+                                w += (((sx + sy + sz) % 2) ? -1.0 : 1.0) * Ek[kx][sx] * Ek[ky][sy] * Ek[kz][sz]
+                                     * herm_R_get(&ws->R, tx + sx, ty + sy, tz + sz);
+                    Wk[(tx * S2 + ty) * S2 + tz] = w;
+                }
+    }
+    for (int ck = 0; ck < nk; ck++) {
+        const real_t *Wk = ws->W + (size_t) ck * NT;
+        for (int ci = 0; ci < ni; ci++) {
+            const int ix = cxyz_tab[i_L][ci][0], iy = cxyz_tab[i_L][ci][1], iz = cxyz_tab[i_L][ci][2];
+            for (int cj = 0; cj < nj; cj++) {
+                const int jx = cxyz_tab[j_L][cj][0], jy = cxyz_tab[j_L][cj][1], jz = cxyz_tab[j_L][cj][2];
+                real_t v = 0.0;
+                for (int tx = 0; tx <= ix + jx; tx++)
+                    for (int ty = 0; ty <= iy + jy; ty++) {
+                        const real_t exy = Ex[ix][jx][tx] * Ey[iy][jy][ty];
+                        for (int tz = 0; tz <= iz + jz; tz++)
+                            v += exy * Ez[iz][jz][tz] * Wk[(tx * S2 + ty) * S2 + tz];
+                    }
+                ws->cart[((size_t) ck * ni + ci) * nj + cj] = pref * v * cart_norm(i_L, ci) * cart_norm(j_L, cj) * cart_norm(k_L, ck);
+            }
+        }
+    }
+    /* ij transform per k component, then k transform per ij element */
+    transform_ij(i_L, j_L, nk, ws->cart, ws->ijsph, spherical);
+    for (int i = 0; i < oi; i++)
+        for (int j = 0; j < oj; j++)
+            for (int k = 0; k < ok; k++)
+                blk[((size_t) k * oi + i) * oj + j] = spherical
+                    ? sph_row_dot(k_L, k, ws->ijsph + (size_t) i * oj + j, oi * oj)
+                    : ws->ijsph[((size_t) k * oi + i) * oj + j];
+}
+
+/* Triplets are taken in (auxiliary shell, auxiliary primitive, pair) order, in rounds of at most
+ * kRoundReals output values: the triplets of a round are evaluated in parallel into a buffer, then
+ * each output row (auxiliary function) adds its contributions in that fixed order, one thread per
+ * row.  Results therefore do not depend on the thread count, and a single auxiliary shell over
+ * millions of pairs runs on every core. */
+#define J3C_ROUND_REALS (1L << 25)
+
 int oracle_three_center(int i_L, int j_L, int k_L, const double *pair_P_p, const double *pair_A_a, const double *pair_B_b,
                         const double *pair_coefficient, const int *pair_i_ao_start, const int *pair_j_ao_start,
                         int n_pair_ij, const double *aux_A_a, const double *aux_coefficient, const int *aux_i_aux_start,
@@ -537,92 +631,84 @@ int oracle_three_center(int i_L, int j_L, int k_L, const double *pair_P_p, const
         return 1;
     }
     (void) n_aux;
+    if (n_pair_ij <= 0 || n_aux_primitive_k <= 0) return 0;
     const int ni = ncart(i_L), nj = ncart(j_L), nk = ncart(k_L);
     const int oi = spherical ? 2 * i_L + 1 : ni, oj = spherical ? 2 * j_L + 1 : nj, ok = spherical ? 2 * k_L + 1 : nk;
-    /* distinct auxiliary shell starts: one OpenMP task per auxiliary shell */
-    int *starts = (int *) malloc(sizeof(int) * (n_aux_primitive_k > 0 ? n_aux_primitive_k : 1));
+    const size_t bsz = (size_t) ok * oi * oj;
+    const int S2 = i_L + j_L + 1;
+    /* auxiliary primitives grouped by shell start, first-appearance order of the starts */
+    int *starts = (int *) malloc(sizeof(int) * n_aux_primitive_k);
+    int *kord = (int *) malloc(sizeof(int) * n_aux_primitive_k);     /* primitives in (start, index) order */
+    int *kgrp = (int *) malloc(sizeof(int) * (n_aux_primitive_k + 1)); /* CSR over starts */
     int n_start = 0;
     for (int ka = 0; ka < n_aux_primitive_k; ka++) {
         int seen = 0;
         for (int s = 0; s < n_start; s++) if (starts[s] == aux_i_aux_start[ka]) { seen = 1; break; }
         if (!seen) starts[n_start++] = aux_i_aux_start[ka];
     }
+    int nk_ord = 0;
+    for (int s = 0; s < n_start; s++) {
+        kgrp[s] = nk_ord;
+        for (int ka = 0; ka < n_aux_primitive_k; ka++)
+            if (aux_i_aux_start[ka] == starts[s]) kord[nk_ord++] = ka;
+    }
+    kgrp[n_start] = nk_ord;
+    const long long n_trip = (long long) n_aux_primitive_k * n_pair_ij;
+    long long round = J3C_ROUND_REALS / (long long) bsz;
+    if (round < 64) round = 64;
+    if (round > n_trip) round = n_trip;
+    real_t *buf = (real_t *) malloc(sizeof(real_t) * bsz * (size_t) round);
+    for (long long t0 = 0; t0 < n_trip; t0 += round) {
+        const long long t1 = (t0 + round < n_trip) ? t0 + round : n_trip;
 #pragma omp parallel
-    {
-        real_t *cart = (real_t *) malloc(sizeof(real_t) * nk * ni * nj);
-        real_t *ijsph = (real_t *) malloc(sizeof(real_t) * nk * ni * nj);
-        real_t *blk = (real_t *) malloc(sizeof(real_t) * nk * ni * nj);
-        real_t Ex[OR_MAX_L + 1][OR_MAX_L + 1][ESTR], Ey[OR_MAX_L + 1][OR_MAX_L + 1][ESTR], Ez[OR_MAX_L + 1][OR_MAX_L + 1][ESTR];
-        real_t Ek[OR_MAX_AUX_L + 1][OR_MAX_AUX_L + 1];
-        herm_R R;
-        herm_R_alloc(&R, i_L + j_L + k_L);
-#pragma omp for schedule(dynamic)
-        for (int s = 0; s < n_start; s++) {
-            for (int ka = 0; ka < n_aux_primitive_k; ka++) {
-                if (aux_i_aux_start[ka] != starts[s]) continue;
-                const double *C = aux_A_a + 4 * ka;
-                const real_t q = C[3];
-                hermite_E_single(k_L, 0.5 / q, Ek);
-                for (int ip = 0; ip < n_pair_ij; ip++) {
-                    const double *P = pair_P_p + 4 * ip, *A = pair_A_a + 4 * ip, *B = pair_B_b + 4 * ip;
-                    const real_t p = P[3];
-                    const real_t coef = pair_coefficient[ip] * aux_coefficient[ka];
-                    /* reference three_center.cpp:46-49 */
-                    const real_t pref = (omega == 0.0) ? coef * 2.0 * pow(OR_PI, 2.5) / (p * q * sqrt(p + q))
-                                                       : coef * 2.0 * pow(OR_PI, 2.5) * omega / (p * q * sqrt(p * q + (p + q) * omega * omega));
-                    const real_t zeta = (omega == 0.0) ? p * q / (p + q) : 1.0 / (1.0 / p + 1.0 / q + 1.0 / (omega * omega));
-                    hermite_E_pair(i_L, j_L, P[0] - A[0], A[0] - B[0], 0.5 / p, Ex);
-                    hermite_E_pair(i_L, j_L, P[1] - A[1], A[1] - B[1], 0.5 / p, Ey);
-                    hermite_E_pair(i_L, j_L, P[2] - A[2], A[2] - B[2], 0.5 / p, Ez);
-                    herm_R_compute(&R, zeta, P[0] - C[0], P[1] - C[1], P[2] - C[2]);
-                    for (int ck = 0; ck < nk; ck++) {
-                        int kx, ky, kz; cart_xyz(k_L, ck, &kx, &ky, &kz);
-                        for (int ci = 0; ci < ni; ci++) {
-                            int ix, iy, iz; cart_xyz(i_L, ci, &ix, &iy, &iz);
-                            for (int cj = 0; cj < nj; cj++) {
-                                int jx, jy, jz; cart_xyz(j_L, cj, &jx, &jy, &jz);
-                                real_t v = 0.0;
-                                for (int tx = 0; tx <= ix + jx; tx++)
-                                    for (int ty = 0; ty <= iy + jy; ty++)
-                                        for (int tz = 0; tz <= iz + jz; tz++) {
-                                            const real_t Et = Ex[ix][jx][tx] * Ey[iy][jy][ty] * Ez[iz][jz][tz];
-                                            real_t w = 0.0;
-                                            for (int sx = kx % 2; sx <= kx; sx += 2)
-                                                for (int sy = ky % 2; sy <= ky; sy += 2)
-                                                    for (int sz = kz % 2; sz <= kz; sz += 2)
-                                                        w += (((sx + sy + sz) % 2) ? -1.0 : 1.0) * Ek[kx][sx] * Ek[ky][sy] * Ek[kz][sz]
-                                                             * herm_R_get(&R, tx + sx, ty + sy, tz + sz);
-                                            v += Et * w;
-                                        }
-                                cart[((size_t) ck * ni + ci) * nj + cj] = pref * v * cart_norm(i_L, ci) * cart_norm(j_L, cj) * cart_norm(k_L, ck);
-                            }
-                        }
+        {
+            j3c_ws ws;
+            herm_R_alloc(&ws.R, i_L + j_L + k_L);
+            ws.W = (real_t *) malloc(sizeof(real_t) * (size_t) nk * S2 * S2 * S2);
+            ws.cart = (real_t *) malloc(sizeof(real_t) * (size_t) nk * ni * nj);
+            ws.ijsph = (real_t *) malloc(sizeof(real_t) * (size_t) nk * ni * nj);
+#pragma omp for schedule(dynamic, 16)
+            for (long long t = t0; t < t1; t++) {
+                const int ka = kord[t / n_pair_ij], ip = (int) (t % n_pair_ij);
+                three_center_triplet(i_L, j_L, k_L, pair_P_p + 4 * ip, pair_A_a + 4 * ip, pair_B_b + 4 * ip,
+                                     pair_coefficient[ip], aux_A_a + 4 * ka, aux_coefficient[ka], omega, spherical,
+                                     &ws, buf + (size_t) (t - t0) * bsz);
+            }
+            herm_R_free(&ws.R);
+            free(ws.W);
+            free(ws.cart);
+            free(ws.ijsph);
+        }
+        /* shells touched by this round, and their triplet ranges inside it */
+        const int s_lo_idx = (int) (t0 / n_pair_ij), s_hi_idx = (int) ((t1 - 1) / n_pair_ij);
+        int s_first = 0;
+        while (kgrp[s_first + 1] <= s_lo_idx) s_first++;
+        int s_last = s_first;
+        while (kgrp[s_last + 1] <= s_hi_idx) s_last++;
+        const int n_rows = (s_last - s_first + 1) * ok;
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int r = 0; r < n_rows; r++) {
+            const int s = s_first + r / ok, k = r % ok;
+            long long a = (long long) kgrp[s] * n_pair_ij, b = (long long) kgrp[s + 1] * n_pair_ij;
+            if (a < t0) a = t0;
+            if (b > t1) b = t1;
+            double *row = J3c_matrix + (size_t) (starts[s] + k) * n_ao * n_ao;
+            for (long long t = a; t < b; t++) {
+                const int ip = (int) (t % n_pair_ij);
+                const int is = pair_i_ao_start[ip], js = pair_j_ao_start[ip];
+                const real_t *blk = buf + (size_t) (t - t0) * bsz + (size_t) k * oi * oj;
+                for (int i = 0; i < oi; i++)
+                    for (int j = 0; j < oj; j++) {
+                        const real_t val = blk[i * oj + j];
+                        row[(size_t) (is + i) * n_ao + (js + j)] += val;
+                        if (is != js) row[(size_t) (is + i) + (size_t) (js + j) * n_ao] += val;
                     }
-                    /* ij transform per k component, then k transform per ij element */
-                    transform_ij(i_L, j_L, nk, cart, ijsph, spherical);
-                    for (int i = 0; i < oi; i++)
-                        for (int j = 0; j < oj; j++)
-                            for (int k = 0; k < ok; k++)
-                                blk[((size_t) k * oi + i) * oj + j] = spherical
-                                    ? sph_row_dot(k_L, k, ijsph + (size_t) i * oj + j, oi * oj)
-                                    : ijsph[((size_t) k * oi + i) * oj + j];
-                    const int is = pair_i_ao_start[ip], js = pair_j_ao_start[ip], ks = aux_i_aux_start[ka];
-                    for (int k = 0; k < ok; k++)
-                        for (int i = 0; i < oi; i++)
-                            for (int j = 0; j < oj; j++) {
-                                const real_t val = blk[((size_t) k * oi + i) * oj + j];
-                                double *row = J3c_matrix + (size_t) (ks + k) * n_ao * n_ao;
-                                row[(size_t) (is + i) * n_ao + (js + j)] += val;
-                                if (is != js) row[(size_t) (is + i) + (size_t) (js + j) * n_ao] += val;
-                            }
-                }
             }
         }
-        herm_R_free(&R);
-        free(cart);
-        free(ijsph);
-        free(blk);
     }
+    free(buf);
     free(starts);
+    free(kord);
+    free(kgrp);
     return 0;
 }
