@@ -194,7 +194,11 @@ class IntegralServer:
     float64 metric (P|Q) of every job."""
 
     def __init__(self, budget_bytes=200e9, mixed=True, threshold=1e-6, keep_outputs=False, prep_workers=12,
-                 max_steps_in_flight=2, unit_cap_bytes=None, granularity="item", device="cuda", log=None):
+                 max_steps_in_flight=2, unit_cap_bytes=None, granularity="item", device="cuda", log=None,
+                 on_unit=None):
+        """``on_unit(job, unit)``, if given, consumes every finished unit: it is called when the unit's
+        step has retired, with the compute stream current and ``unit.out`` still valid (rows of
+        ``unit.blocks`` from ``unit.row_off``), before the unit's output memory is reused."""
         import torch
         self.torch = torch
         self.budget = float(budget_bytes)
@@ -212,6 +216,7 @@ class IntegralServer:
         self.pool = cf.ThreadPoolExecutor(prep_workers)
         self.large_pool = cf.ThreadPoolExecutor(LARGE_PREP_WORKERS)
         self.log = log
+        self.on_unit = on_unit
         if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < MIN_HW_QUEUES:
             warnings.warn(f"GPU_MAX_HW_QUEUES below {MIN_HW_QUEUES}: device setups may wait behind whole "
                           "steps; start the serving process with a larger value", RuntimeWarning)
@@ -452,6 +457,8 @@ class IntegralServer:
                 for u in units:
                     # consumed on the device: sum and 2-norm (reductions, no full-size temporaries)
                     u.checksum = torch.stack([u.out.sum(), torch.linalg.vector_norm(u.out)])
+                    if self.on_unit is not None:
+                        self.on_unit(u.job, u)
                     if not self.keep_outputs:
                         u.out = None
                         # stream-ordered reuse: later launches on this stream run after the checksum

@@ -119,3 +119,46 @@ def run_nuclear_oracle(lib, pair_data, mol, charges, omega=0.0, fn="oracle_nucle
 # Parity bar of the floating-point integrals: |HIP - oracle| <= 1e-12 + 1e-10 |oracle|.
 ATOL = 1e-12
 RTOL = 1e-10
+
+
+def oracle_aux_shell_block(lib, pair_data, pk, k_L, start, n_ao, omega=0.0, fn="oracle_three_center"):
+    """Dense (2 k_L + 1, n_ao, n_ao) oracle (ij|P) of ONE auxiliary shell (the primitives of class k_L
+    whose i_ao_start equals ``start``, rebased to function 0) against every orbital pair: the reference's
+    per-class loop restricted to one auxiliary shell, so samples of workloads whose full output exceeds
+    host memory can be checked column by column."""
+    sel = pk.i_ao_start == start
+    A = np.ascontiguousarray(pk.A_a[sel])
+    c = np.ascontiguousarray(pk.coefficient[sel])
+    st = np.zeros(int(sel.sum()), dtype=np.int32)
+    nk = 2 * k_L + 1
+    J = np.zeros((nk, n_ao, n_ao))
+    for (i_L, j_L), pd in pair_data.items():
+        rc = getattr(lib, fn)(i_L, j_L, k_L, dp(pd.P_p), dp(pd.A_a), dp(pd.B_b), dp(pd.coefficient),
+                              ip(pd.i_ao_start), ip(pd.j_ao_start), pd.P_p.shape[0], dp(A), dp(c), ip(st),
+                              A.shape[0], dp(J), n_ao, nk, ctypes.c_bool(True), ctypes.c_double(omega))
+        assert rc == 0
+    return J
+
+
+def sample_aux_shells(aux_data, per_class, seed):
+    """{k_L: sorted array of sampled auxiliary shell starts}: ``per_class`` (an int, or {k_L: int})
+    distinct shells per angular class, drawn with a seeded generator."""
+    rng = np.random.default_rng(seed)
+    out = {}
+    for k_L, pk in aux_data.items():
+        starts = np.unique(pk.i_ao_start)
+        n = per_class if isinstance(per_class, int) else per_class.get(k_L, 0)
+        out[k_L] = np.sort(rng.choice(starts, size=min(n, len(starts)), replace=False))
+    return out
+
+
+def assert_parity(got, ref, what, atol=ATOL, rtol=RTOL):
+    """|got - ref| <= atol + rtol |ref| elementwise, with the worst element named on failure."""
+    got, ref = np.asarray(got), np.asarray(ref)
+    err = np.abs(got - ref)
+    bad = err > atol + rtol * np.abs(ref)
+    if bad.any():
+        k = int(np.argmax(np.where(bad, err, -1.0)))
+        raise AssertionError(f"{what}: {int(bad.sum())} of {bad.size} elements outside {atol:g} + {rtol:g}|ref|; "
+                             f"worst at flat index {k}: got {got.flat[k]!r}, oracle {ref.flat[k]!r}")
+    return float(err.max()) if err.size else 0.0
