@@ -1,19 +1,27 @@
 """Benchmark of the McMurchie-Davidson three-centre path on one MI355X; prints one JSON line.
 
-value: the plain FP64 workload -- every (ij|P) of 80 atoms in a 14 A box, orbital basis 3s2p1d with 1-6
-primitives per shell, auxiliary s-f with 8 even-tempered exponents per l (n_ao = 1120, n_aux = 10240),
-written into the device-resident packed output J3c_packed[i >= j, P] (51.4 GB) with inputs resident in
-HBM before timing.  value = n_aux * n_ao (n_ao + 1) / 2 / (seconds per step).
+value (headline): plain FP64 3c integrals/s of the 200-atom workload (20 A box, orbital 4s3p2d1f with
+1-6 primitives per shell, auxiliary s-g with 8 even-tempered exponents per l; n_ao 6000, n_aux 40000,
+7 840 675 primitive pairs) on its fixed auxiliary-block share: every primitive pair against the
+auxiliary shells of the first 24 atoms (192 shells of each angular class l = 0..4, 4800 auxiliary
+functions, 12 % of n_aux -- the share one of eight GPUs computes when the tensor is sharded by
+auxiliary blocks).  One step writes the slice's compact (ij|P) -- 12.2 million significant-shell-pair
+rows x 4800 columns, 469 GB -- through a 40 GB device buffer in blocks of shell pairs, inputs resident
+in HBM before timing.  value = 4800 * n_ao (n_ao + 1) / 2 / (seconds per step).
 
 Beside it, in the same line:
-* roofline of the dominant kernel class (HIP-event time on the launch stream, algorithmic flops from
-  the fixed per-triplet model, HBM traffic from the committed rocprofv3 counter summary);
-* cpu_baseline: the reference's OpenMP C++ (oracle/_ref) on a bounded sample, with parity of the sample;
-* mixed_precision: the 200-atom workload (n_ao 6000, n_aux 40000, 3.9 TB of output streamed in blocks),
-  plain FP64 against the FP32/FP64 split, max abs and RMS error on the device, metric Cholesky;
+* mixed_precision: the same slice with the float32 tier (triplets whose bound is below 1e-6) --
+  integrals/s, ratio to plain, max abs / RMS difference from plain over the whole slice, and max abs
+  difference from the CPU reference on the sampled shells;
+* roofline of the dominant kernel class (HIP events on the launch stream, algorithmic flops of the fixed
+  per-triplet model, HBM traffic from the committed rocprofv3 counter summary);
+* cpu_baseline: the reference's OpenMP C++ (oracle/_ref; the oracle's restatement where that build is
+  absent) on a bounded sample of the same slice (whole auxiliary shells x all pairs), with its parity;
+* plain80: the 80-atom workload's full step (n_ao 1120, n_aux 10240: every (ij|P) into the packed
+  51.4 GB device output plus (P|Q)), with its own roofline;
 * batched: 256 molecules (lognormal sizes) in waves of HIP-graph-captured batch plans;
 * serving: a short Poisson stream through the continuous-batching server (p50/p99 latency);
-  `--serving --serving-jobs N` runs the long stream alone.
+  ``--serving --serving-jobs N`` runs a long stream alone, ``--mixed-full`` the full 3.9 TB comparison.
 """
 
 import argparse
@@ -37,76 +45,34 @@ ROOT = pathlib.Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 import __graft_entry__  # noqa: E402
 
-PEAK_FP64 = 78.6e12          # MI355X FP64 vector (and matrix) peak, FLOP/s
+PEAK_FP64 = 78.6e12          # MI355X FP64 vector peak, FLOP/s
 PEAK_HBM = 8.0e12            # bytes/s
-SAMPLE_SHELLS = 96           # auxiliary shells per angular class in the CPU-baseline sample
-PMC_SUMMARY = ROOT / "profiles" / "j3c_plain_pmc.md"
+SLICE_ATOMS = 24             # auxiliary shells of the first 24 atoms: 192 per angular class
+BLOCK_BYTES = 40e9           # device buffer of one streamed block
+CPU_SECONDS = 25.0           # bound of the CPU-baseline sample
+PMC_SUMMARY = ROOT / "profiles" / "j3c_mixed_slice_pmc.md"
+PMC_SUMMARY_PLAIN80 = ROOT / "profiles" / "j3c_plain80_pmc.md"
+METRIC = "3c integrals/s, mixed vs plain FP64 at <=1e-10 err; p50/p99 job latency"
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(mol, pairs, aux, gpu_packed, max_seconds=20.0):
-    """Time the reference C++ (or the oracle's restatement if the reference build is absent) on the
-    first auxiliary shells of every class, all orbital pairs; return throughput and parity."""
-    from tests.helpers import REF_SO, ORACLE_SO, dp, ip
-    kind, path, fn = ("reference", REF_SO, "three_center") if REF_SO.exists() else ("port", ORACLE_SO, "oracle_three_center")
-    cores = os.cpu_count() or 1
-    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
-    lib = ctypes.CDLL(str(path))
-    n_ao = mol.n_ao
-    # sample: aux shells whose functions start in [0, n_blk) of each class (rebased block output)
-    n_shell_per_class = SAMPLE_SHELLS
-    total_t, total_int = 0.0, 0
-    max_err = 0.0
-    for k_L, pk in aux.items():
-        starts = np.unique(pk.i_ao_start)[:n_shell_per_class]
-        sel = np.isin(pk.i_ao_start, starts)
-        A = np.ascontiguousarray(pk.A_a[sel]); c = np.ascontiguousarray(pk.coefficient[sel])
-        base = {int(s): k * (2 * k_L + 1) for k, s in enumerate(starts)}
-        st = np.array([base[int(s)] for s in pk.i_ao_start[sel]], dtype=np.int32)
-        nblk = len(starts) * (2 * k_L + 1)
-        J = np.zeros((nblk, n_ao, n_ao))
-        t0 = time.perf_counter()
-        for (i_L, j_L), pd in pairs.items():
-            rc = getattr(lib, fn)(i_L, j_L, k_L, dp(pd.P_p), dp(pd.A_a), dp(pd.B_b), dp(pd.coefficient),
-                                  ip(pd.i_ao_start), ip(pd.j_ao_start), pd.P_p.shape[0], dp(A), dp(c), ip(st),
-                                  A.shape[0], dp(J), n_ao, nblk, ctypes.c_bool(True), ctypes.c_double(0.0))
-            assert rc == 0
-        total_t += time.perf_counter() - t0
-        total_int += nblk * n_ao * (n_ao + 1) // 2
-        # parity of the sample against the GPU packed output
-        tri_i, tri_j = np.tril_indices(n_ao)
-        for k, s in enumerate(starts if gpu_packed is not None else []):
-            for m in range(2 * k_L + 1):
-                gpu_col = gpu_packed[:, int(s) + m]
-                ref_col = J[k * (2 * k_L + 1) + m][tri_i, tri_j]
-                max_err = max(max_err, float(np.max(np.abs(gpu_col - ref_col))))
-        if total_t > max_seconds:
-            break
-    sample = f"{n_shell_per_class} auxiliary shells per angular class x all {sum(len(p.P_p) for p in pairs.values())} primitive pairs"
-    return {"value": total_int / total_t, "unit": "3c integrals/s", "cores": int(os.environ["OMP_NUM_THREADS"]),
-            "kind": kind, "sample": sample, "seconds": round(total_t, 3)}, max_err
-
-
-def measured_traffic(la, lb, lc):
-    """HBM bytes of one pass of class (la lb | lc) (all its component kernels), from the committed
-    rocprofv3 WRITE_SIZE / FETCH_SIZE summary (calibrated against a device copy); None if absent."""
-    if not PMC_SUMMARY.exists():
+def measured_traffic(summary, cls):
+    """HBM bytes of class (la, lb, lc) per step from a committed rocprofv3 WRITE_SIZE / FETCH_SIZE
+    summary (rows ``| j3c_<la><lb><lc> | ... | calibrated bytes |``); None if absent."""
+    if not summary.exists():
         return None
-    total, found = 0.0, False
-    for line in PMC_SUMMARY.read_text().splitlines():
+    key = "j3c_%d%d%d" % cls
+    for line in summary.read_text().splitlines():
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        # rows of the traffic table: | kernel | WRITE_SIZE KiB | FETCH_SIZE KiB | calibrated bytes (x.xxxxe+yy) |
-        if (len(cells) == 4 and cells[0].startswith(f"j3c_{la}{lb}{lc}_") and not cells[0].endswith("_f32")
-                and "e+" in cells[3]):
+        if len(cells) >= 5 and cells[0] == key:
             try:
-                total += float(cells[3])
-                found = True
+                return float(cells[4])
             except ValueError:
-                pass
-    return total if found else None
+                return None
+    return None
 
 
 def q_aux_from_metric(plan, J2c):
@@ -117,6 +83,282 @@ def q_aux_from_metric(plan, J2c):
         idx = ac.ks_start[:, None] + np.arange(2 * ac.lc + 1)[None, :]
         out.append(np.sqrt(np.abs(diag[idx]).max(axis=1)))
     return out
+
+
+def slice_workload(n_slice_atoms=SLICE_ATOMS):
+    """The 200-atom workload with its auxiliary basis cut to the shells of the first atoms (their
+    functions are the first columns, since auxiliary offsets follow the atom order)."""
+    import copy
+    from electron_integral_playground_amd import generators, pair_utility
+    t0 = time.perf_counter()
+    mol = generators.make_molecule("mixed")
+    arrays = pair_utility.form_pair_arrays(mol, 1e-14)
+    pairs = pair_utility.form_primitive_pair_data(mol, arrays)
+    aux_full = pair_utility.form_auxiliary_pair_data(mol)
+    t_pairs = time.perf_counter() - t0
+    aux = {}
+    n_aux = 0
+    for lc, pk in aux_full.items():
+        sel = pk.i_atom < n_slice_atoms
+        sub = copy.copy(pk)
+        sub.A_a = np.ascontiguousarray(pk.A_a[sel])
+        sub.coefficient = np.ascontiguousarray(pk.coefficient[sel])
+        sub.i_ao_start = np.ascontiguousarray(pk.i_ao_start[sel])
+        sub.i_atom = np.ascontiguousarray(pk.i_atom[sel])
+        aux[lc] = sub
+        n_aux = max(n_aux, int(sub.i_ao_start.max()) + 2 * lc + 1)
+    mol_s = copy.copy(mol)
+    mol_s.n_aux = n_aux
+    return mol, mol_s, arrays, pairs, aux, t_pairs
+
+
+def class_times(plan, blocks, buf, mixed, stream):
+    """Per (la, lb, lc): summed HIP-event time (s) of its launches over one step, every launch on the
+    launch stream (one class at a time, so durations do not overlap)."""
+    import torch
+    ev = []
+    for b in blocks:
+        for ai, ac in enumerate(plan.aux):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            plan.run_block(*b, buf, mixed=mixed, aux_classes=[ai])
+            e1.record(stream)
+            pc = plan.pairs[b[0]]
+            ev.append(((pc.la, pc.lb, ac.lc), e0, e1))
+    torch.cuda.synchronize()
+    out = {}
+    for key, e0, e1 in ev:
+        out[key] = out.get(key, 0.0) + e0.elapsed_time(e1) * 1e-3
+    return out
+
+
+def cpu_baseline(mol, pairs, aux, gpu_cols, blocks_ao, max_seconds=CPU_SECONDS):
+    """The reference C++ (or the oracle's restatement if the reference build is absent) on whole
+    auxiliary shells of the slice against every pair, one shell per angular class in a fixed order until
+    max_seconds have passed.  Returns the baseline record and the parity of the GPU columns
+    (gpu_cols[name][lc][start] -> list of per-block (rows, nk) arrays) against it."""
+    from tests.helpers import REF_SO, ORACLE_SO, dp, ip
+    kind, path, fn = ("reference", REF_SO, "three_center") if REF_SO.exists() else ("port", ORACLE_SO, "oracle_three_center")
+    cores = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    lib = ctypes.CDLL(str(path))
+    n_ao = mol.n_ao
+    n_packed = n_ao * (n_ao + 1) // 2
+    total_t, total_int, shells = 0.0, 0, []
+    err = {name: 0.0 for name in gpu_cols}
+    for lc, start in sample_shells(aux):
+        pk = aux[lc]
+        sel = pk.i_ao_start == start
+        A = np.ascontiguousarray(pk.A_a[sel])
+        c = np.ascontiguousarray(pk.coefficient[sel])
+        st = np.zeros(int(sel.sum()), dtype=np.int32)
+        nk = 2 * lc + 1
+        J = np.zeros((nk, n_ao, n_ao))
+        t0 = time.perf_counter()
+        for (i_L, j_L), pd in pairs.items():
+            rc = getattr(lib, fn)(i_L, j_L, lc, dp(pd.P_p), dp(pd.A_a), dp(pd.B_b), dp(pd.coefficient),
+                                  ip(pd.i_ao_start), ip(pd.j_ao_start), pd.P_p.shape[0], dp(A), dp(c), ip(st),
+                                  A.shape[0], dp(J), n_ao, nk, ctypes.c_bool(True), ctypes.c_double(0.0))
+            assert rc == 0
+        total_t += time.perf_counter() - t0
+        total_int += nk * n_packed
+        shells.append((int(lc), int(start)))
+        for name, cols in gpu_cols.items():
+            for bi, (I, Jx) in enumerate(blocks_ao):
+                err[name] = max(err[name], float(np.abs(cols[(lc, start)][bi] - J[:, I, Jx].T).max()))
+        if total_t > max_seconds:
+            break
+    n_pp = sum(len(p.P_p) for p in pairs.values())
+    rec = {"value": total_int / total_t, "unit": "3c integrals/s", "cores": cores, "kind": kind,
+           "sample": f"{len(shells)} whole auxiliary shells (l, first function) {shells} of the slice x all {n_pp} "
+                     "primitive pairs, dense-equivalent integrals", "seconds": round(total_t, 3)}
+    return rec, err
+
+
+def sample_shells(aux):
+    """(l, first function) of the CPU-baseline sample: the first shell of atom 1 for l = 4, 2, 0, 3, 1."""
+    out = []
+    for lc in (4, 2, 0, 3, 1):
+        if lc in aux:
+            pk = aux[lc]
+            starts = np.unique(pk.i_ao_start[pk.i_atom == 1])
+            if len(starts):
+                out.append((lc, int(starts[0])))
+    return out
+
+
+def headline_leg(args):
+    """Plain and mixed-precision steps over the 200-atom workload's auxiliary-block share."""
+    import torch
+    from electron_integral_playground_amd.engine import ThreeCenterPlan, three_center_flops_per_triplet
+    mol, mol_s, arrays, pairs, aux, t_pairs = slice_workload(args.slice_atoms)
+    plan = ThreeCenterPlan(mol_s, pairs, aux)
+    if args.tune:
+        from electron_integral_playground_amd.engine import autotune
+        autotune(plan, save=args.tune, log=log, block_bytes=4e9)
+        autotune(plan, save=args.tune, log=log, block_bytes=4e9, use_f32=True)
+    J2c = plan.two_center()
+    q = q_aux_from_metric(plan, J2c)
+    del J2c
+    stats = plan.setup_mixed(arrays, q, threshold=args.threshold)
+    blocks = plan.row_blocks(BLOCK_BYTES)
+    plan.prepare_blocks(blocks, mixed=True)
+    rows = max(plan.block_rows(*b) for b in blocks)
+    A = torch.empty((rows, mol_s.n_aux), dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream()
+    n_pp = sum(len(a) for a in arrays.values())
+    log(f"headline: n_ao={mol.n_ao} n_aux(slice)={mol_s.n_aux} of {mol.n_aux} primitive pairs={n_pp} "
+        f"blocks={len(blocks)} host pair list {t_pairs:.1f}s; split {stats}")
+
+    def step(mixed):
+        for b in blocks:
+            plan.run_block(*b, A, mixed=mixed)
+
+    for _ in range(max(1, args.warmup)):
+        step(False)
+        step(True)
+    torch.cuda.synchronize()
+    log("headline warmup done")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(False)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(True)
+    torch.cuda.synchronize()
+    dt_mixed = (time.perf_counter() - t0) / args.steps
+    n_int = mol_s.n_aux * mol.n_ao * (mol.n_ao + 1) // 2
+    log(f"headline step plain {dt * 1e3:.1f} ms -> {n_int / dt:.4e} integrals/s; mixed {dt_mixed * 1e3:.1f} ms "
+        f"-> {n_int / dt_mixed:.4e} ({dt / dt_mixed:.3f}x)")
+
+    # per-class kernel time (one class at a time on the launch stream) and model flops
+    trip = plan.primitive_triplets()
+    times = class_times(plan, blocks, A, False, stream)
+    times32 = class_times(plan, blocks, A, True, stream)
+    per_class = sorted(((k, t, trip[k] * three_center_flops_per_triplet(*k)) for k, t in times.items()),
+                       key=lambda r: -r[1])
+    for k, t, fl in per_class[:10]:
+        log(f"  class {k}: {t * 1e3:.2f} ms/step  {fl / t / 1e12:.2f} TFLOP/s (model)  mixed {times32[k] * 1e3:.2f} ms")
+    key, t, fl = per_class[0]
+    roofline = {"bound": "fp64-valu", "kernel": "j3c_%d%d%d (%d component-chunk kernels, all blocks of one step)"
+                % (key + (plan.lib.eip_three_center_nchunks(*key),)),
+                "achieved": fl / t / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": fl / t / PEAK_FP64,
+                "traffic": measured_traffic(PMC_SUMMARY, key),
+                "traffic_unit": "HBM bytes per step for this class (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
+                "algorithmic_flops_per_step": fl, "class_seconds_per_step": t}
+
+    # error pass over the whole slice (mixed vs plain on the device), keeping the CPU sample's columns
+    B = torch.empty_like(A)
+    sample = sample_shells(aux)
+    cols = np.concatenate([np.arange(s, s + 2 * lc + 1) for lc, s in sample])
+    cols_t = torch.as_tensor(cols, device="cuda")
+    max_abs = torch.zeros((), dtype=torch.float64, device="cuda")
+    sum_sq = torch.zeros((), dtype=torch.float64, device="cuda")
+    max_ref = torch.zeros((), dtype=torch.float64, device="cuda")
+    got = {"plain": [], "mixed": []}
+    n_el = 0
+    for b in blocks:
+        n = plan.block_rows(*b)
+        plan.run_block(*b, A)
+        plan.run_block(*b, B, mixed=True)
+        for r in range(0, n, 8192):
+            r1 = min(n, r + 8192)
+            d = B[r:r1] - A[r:r1]
+            max_abs = torch.maximum(max_abs, d.abs().max())
+            sum_sq += (d * d).sum()
+            max_ref = torch.maximum(max_ref, A[r:r1].abs().max())
+        got["plain"].append(A[:n].index_select(1, cols_t).cpu().numpy())
+        got["mixed"].append(B[:n].index_select(1, cols_t).cpu().numpy())
+        n_el += n * mol_s.n_aux
+    torch.cuda.synchronize()
+    del A, B
+    torch.cuda.empty_cache()
+    gpu_cols = {name: {} for name in got}
+    c0 = 0
+    for lc, s in sample:
+        nk = 2 * lc + 1
+        for name in got:
+            gpu_cols[name][(lc, s)] = [g[:, c0:c0 + nk] for g in got[name]]
+        c0 += nk
+    blocks_ao = []
+    for pi, sp0, sp1 in blocks:
+        pc = plan.pairs[pi]
+        ni, nj = 2 * pc.la + 1, 2 * pc.lb + 1
+        sp = np.arange(sp0, sp1)
+        I = np.broadcast_to(pc.sp_i[sp].astype(np.int64)[:, None, None] + np.arange(ni)[None, :, None], (len(sp), ni, nj))
+        J = np.broadcast_to(pc.sp_j[sp].astype(np.int64)[:, None, None] + np.arange(nj)[None, None, :], (len(sp), ni, nj))
+        blocks_ao.append((I.ravel(), J.ravel()))
+    mixed = {"value": n_int / dt_mixed, "unit": "3c integrals/s", "ms_per_step": dt_mixed * 1e3,
+             "mixed_over_plain": dt / dt_mixed, "threshold": args.threshold,
+             "f32_triplet_fraction": stats["f32_triplets"] / (stats["f32_triplets"] + stats["f64_triplets"]),
+             "max_abs_err_vs_plain_fp64": max_abs.item(), "rms_err_vs_plain_fp64": (sum_sq.item() / n_el) ** 0.5,
+             "max_abs_value": max_ref.item(), "compared_elements": n_el,
+             "class_seconds_per_step": {"%d%d%d" % k: v for k, v in times32.items()}}
+    cpu = None
+    if not args.no_cpu:
+        cpu, err = cpu_baseline(mol_s, pairs, aux, gpu_cols, blocks_ao)
+        mixed["max_abs_err_vs_cpu_reference_sample"] = err["mixed"]
+        log(f"cpu baseline {cpu}; sample max |GPU - CPU| plain {err['plain']:.3e} mixed {err['mixed']:.3e}")
+        cpu_err = err["plain"]
+    else:
+        cpu_err = None
+    res = {"value": n_int / dt, "ms_per_step": dt * 1e3, "n_ao": mol.n_ao, "n_aux_slice": mol_s.n_aux,
+           "n_aux_full": mol.n_aux, "primitive_pairs": n_pp, "blocks_per_step": len(blocks),
+           "roofline": roofline, "cpu_baseline": cpu, "mixed": mixed,
+           "max_abs_err_vs_cpu_reference_sample": cpu_err,
+           "class_seconds_per_step": {"%d%d%d" % k: v for k, v, _ in per_class}}
+    del plan
+    torch.cuda.empty_cache()
+    return res
+
+
+def plain80_leg(args):
+    """The 80-atom workload: every (ij|P) into the packed device output plus (P|Q), per step."""
+    import torch
+    from electron_integral_playground_amd import generators, pair_utility
+    from electron_integral_playground_amd.engine import ThreeCenterPlan, three_center_flops_per_triplet
+    mol = generators.make_molecule("plain")
+    pairs = pair_utility.form_pair_data(mol, 1e-14)
+    aux = pair_utility.form_auxiliary_pair_data(mol)
+    plan = ThreeCenterPlan(mol, pairs, aux)
+    out = plan.new_output()
+    J2c = torch.zeros((mol.n_aux, mol.n_aux), dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream()
+    for _ in range(max(1, args.warmup)):
+        plan.run(out, concurrent=not args.single_stream, metric_out=J2c)
+    torch.cuda.synchronize()
+    ev = []
+    for pi, pc in enumerate(plan.pairs):
+        for ai, ac in enumerate(plan.aux):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            plan.launch_class(pi, ai, out)
+            e1.record(stream)
+            ev.append(((pc.la, pc.lb, ac.lc), pc.n_pp * ac.n_prim, e0, e1))
+    torch.cuda.synchronize()
+    per_class = sorted(((k, e0.elapsed_time(e1) * 1e-3, n * three_center_flops_per_triplet(*k)) for k, n, e0, e1 in ev),
+                       key=lambda r: -r[1])
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.run(out, concurrent=not args.single_stream, metric_out=J2c)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    n_int = plan.n_integrals()
+    key, t, fl = per_class[0]
+    log(f"plain80 step {dt * 1e3:.2f} ms -> {n_int / dt:.4e} integrals/s; dominant class {key} {t * 1e3:.3f} ms "
+        f"{fl / t / 1e12:.2f} TFLOP/s (model)")
+    res = {"workload": "80 atoms, 14 A box, 3s2p1d (1-6 prim), aux s-f 8 exp/l ratio 2.2",
+           "n_ao": mol.n_ao, "n_aux": mol.n_aux, "value": n_int / dt, "unit": "3c integrals/s",
+           "ms_per_step": dt * 1e3,
+           "output": "packed (n_ao(n_ao+1)/2, n_aux) float64 (ij|P) + dense (n_aux, n_aux) (P|Q) on the device, every step",
+           "roofline": {"bound": "fp64-valu", "kernel": "j3c_%d%d%d" % key, "achieved": fl / t / 1e12,
+                        "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": fl / t / PEAK_FP64,
+                        "traffic": measured_traffic(PMC_SUMMARY_PLAIN80, key)},
+           "top_classes_ms": {"%d%d%d" % k: t * 1e3 for k, t, _ in per_class[:8]}}
+    del out, plan, J2c
+    torch.cuda.empty_cache()
+    return res
 
 
 def mixed_leg(threshold=1e-6, block_bytes=40e9, n_atom=None, tune=None, aux_order="natural"):
@@ -341,12 +583,18 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tune", metavar="JSON", help="autotune chunk-plan variants first and save them to JSON")
-    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision workload")
-    ap.add_argument("--mixed-only", action="store_true", help="run only the mixed-precision workload")
-    ap.add_argument("--mixed-atoms", type=int, default=None, help="atoms of the mixed workload (default 200)")
-    ap.add_argument("--mixed-aux-order", choices=("qp", "natural"), default="natural",
-                    help="auxiliary-shell grouping of the mixed split (see ThreeCenterPlan.setup_mixed)")
+    ap.add_argument("--threshold", type=float, default=1e-6, help="float32-tier bound of the mixed split")
+    ap.add_argument("--slice-atoms", type=int, default=SLICE_ATOMS,
+                    help="auxiliary shells of this many atoms form the headline slice")
+    ap.add_argument("--headline-only", action="store_true", help="only the headline (profiling runs)")
+    ap.add_argument("--no-plain80", action="store_true", help="skip the 80-atom block")
+    ap.add_argument("--plain80-only", action="store_true")
+    ap.add_argument("--mixed-full", action="store_true",
+                    help="run only the full 200-atom comparison (all 3.9 TB streamed by both paths)")
+    ap.add_argument("--mixed-aux-order", choices=("qp", "natural"), default="natural")
+    ap.add_argument("--tune", metavar="JSON",
+                    help="autotune the chunk-plan variants (vector and matrix-core) on the headline slice first "
+                         "(or, with --mixed-full, on the full workload) and save the table to JSON")
     ap.add_argument("--serving", action="store_true", help="run only the serving workload")
     ap.add_argument("--batched-only", action="store_true", help="run only the batched workload")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched workload")
@@ -359,8 +607,7 @@ def main():
     ap.add_argument("--serving-part", type=int, default=0, help="part index of a stream run in several calls")
     ap.add_argument("--serving-latencies", metavar="JSON", help="write every job's latency to this file")
     ap.add_argument("--single-stream", action="store_true",
-                    help="time the steps with every launch on one stream (profiling runs: kernel durations "
-                         "then do not overlap)")
+                    help="plain80: every launch on one stream (profiling runs: kernel durations do not overlap)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 4 GiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -371,11 +618,16 @@ def main():
 
     import torch
     __graft_entry__.load_package()
-    from electron_integral_playground_amd import generators, pair_utility
-    from electron_integral_playground_amd.engine import (N_SIDE_STREAMS, ThreeCenterPlan, apply_tuning, autotune,
-                                                          three_center_flops_per_triplet)
+    from electron_integral_playground_amd.engine import apply_tuning
 
     torch.cuda.set_device(0)
+    variants = apply_tuning().get("variants", {})
+    if args.copy_probe:
+        a = torch.empty(1 << 29, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)                                   # 4 GiB read + 4 GiB written
+        torch.cuda.synchronize()
+        del a, b
     if args.serving:
         print(json.dumps({"serving": serving_leg(args.serving_jobs, args.serving_rate, mixed=not args.serving_plain,
                                                  part=args.serving_part, latency_file=args.serving_latencies)}),
@@ -384,125 +636,40 @@ def main():
     if args.batched_only:
         print(json.dumps({"batched": batched_leg()}), flush=True)
         return
-    if args.mixed_only:
-        print(json.dumps({"mixed_precision": mixed_leg(n_atom=args.mixed_atoms, tune=args.tune,
-                                                       aux_order=args.mixed_aux_order)}), flush=True)
+    if args.mixed_full:
+        print(json.dumps({"mixed_full": mixed_leg(tune=args.tune, aux_order=args.mixed_aux_order)}), flush=True)
         return
-    t0 = time.perf_counter()
-    mol = generators.make_molecule("plain")
-    pairs = pair_utility.form_pair_data(mol, 1e-14)
-    aux = pair_utility.form_auxiliary_pair_data(mol)
-    t_host = time.perf_counter() - t0
-    log(f"workload plain: n_ao={mol.n_ao} n_aux={mol.n_aux} pairs={sum(len(p.P_p) for p in pairs.values())} "
-        f"host pair list {t_host:.1f}s")
-    plan = ThreeCenterPlan(mol, pairs, aux)
-    out = plan.new_output()
-    J2c = torch.zeros((mol.n_aux, mol.n_aux), dtype=torch.float64, device="cuda")   # (P|Q), part of each step
-    stream = torch.cuda.current_stream()
-    if args.copy_probe:
-        a = torch.empty(1 << 29, dtype=torch.float64, device="cuda")
-        b = torch.empty_like(a)
-        b.copy_(a)                                   # 4 GiB read + 4 GiB written
-        torch.cuda.synchronize()
-        del a, b
-    if args.tune:
-        variants = autotune(plan, save=args.tune, log=log)
-    else:
-        variants = apply_tuning().get("variants", {})
-
-    for _ in range(args.warmup):
-        plan.run(out, concurrent=not args.single_stream, metric_out=J2c)
-    torch.cuda.synchronize()
-    log("warmup done")
-
-    # per-class kernel times with HIP events on the launch stream (one extra pass)
-    ev = []
-    for pi, pc in enumerate(plan.pairs):
-        for ai, ac in enumerate(plan.aux):
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            plan.launch_class(pi, ai, out)
-            e1.record(stream)
-            ev.append(((pc.la, pc.lb, ac.lc), pc.n_pp * ac.n_prim, e0, e1))
-    torch.cuda.synchronize()
-    per_class = []
-    for key, ntrip, e0, e1 in ev:
-        ms = e0.elapsed_time(e1)
-        fl = ntrip * three_center_flops_per_triplet(*key)
-        per_class.append((key, ms, fl))
-    per_class.sort(key=lambda r: -r[1])
-    for key, ms, fl in per_class[:8]:
-        log(f"  class {key}: {ms:.3f} ms  {fl / ms / 1e9:.2f} TFLOP/s (model)")
-
-    # the same step with every launch on one stream, for comparison (not the reported value)
-    dt_serial = None
-    if not args.single_stream:
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(max(1, args.steps // 2)):
-            plan.run(out, concurrent=False, metric_out=J2c)
-        torch.cuda.synchronize()
-        dt_serial = (time.perf_counter() - t0) / max(1, args.steps // 2)
-        log(f"single-stream step {dt_serial * 1e3:.2f} ms")
-
-    # timed region
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        plan.run(out, concurrent=not args.single_stream, metric_out=J2c)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / args.steps
-    n_int = plan.n_integrals()
-    value = n_int / dt
-    log(f"step {dt * 1e3:.2f} ms -> {value:.4e} integrals/s; model flops/step {plan.model_flops():.3e}")
-
-    key, ms, fl = per_class[0]
-    nchunks = plan.lib.eip_three_center_nchunks(*key)
-    roofline = {"bound": "mfma", "pipe": "fp64-valu", "kernel": "j3c_%d%d%d (all %d component chunks)" % (key + (nchunks,)),
-                "achieved": fl / (ms * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-                "frac": fl / (ms * 1e-3) / PEAK_FP64, "traffic": measured_traffic(*key),
-                "traffic_unit": "bytes per class pass (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)"}
-
-    cpu, err = (None, None)
-    if not args.no_cpu:
-        gpu_packed = out.cpu().numpy() if out.numel() * 8 < 8e9 else None
-        if gpu_packed is None:
-            # copy only the columns the sample needs
-            cols = sorted({int(s) + m for k_L, pk in aux.items() for s in np.unique(pk.i_ao_start)[:SAMPLE_SHELLS]
-                           for m in range(2 * k_L + 1)})
-            sub = out[:, cols].cpu().numpy()
-            gpu_packed = np.zeros((out.shape[0], max(cols) + 1))
-            gpu_packed[:, cols] = sub
-        cpu, err = cpu_baseline(mol, pairs, aux, gpu_packed)
-        log(f"cpu baseline {cpu}; sample max |GPU - reference| = {err:.3e}")
-
-    mixed = batched = None
-    del out, plan, J2c
-    torch.cuda.empty_cache()
-    if not args.no_mixed:
-        mixed = mixed_leg(n_atom=args.mixed_atoms, aux_order=args.mixed_aux_order)
-    if not args.no_batched:
-        batched = batched_leg()
-    serving = None
-    if not args.no_serving:
-        torch.cuda.empty_cache()
-        serving = serving_leg(args.line_serving_jobs, args.serving_rate)
-    line = {"metric": "3c integrals/s, mixed vs plain FP64 at <=1e-10 err; p50/p99 job latency", "value": value,
-            "unit": "3c integrals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": dt * 1e3, "single_stream_ms_per_step": dt_serial * 1e3 if dt_serial else None,
-            "streams": 1 if args.single_stream else N_SIDE_STREAMS + 2, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-            "dtype": "f64", "data": "synthetic (seeded generators, plain workload)",
-            "config": {"workload": "plain: 80 atoms, 14 A box, 3s2p1d (1-6 prim), aux s-f 8 exp/l ratio 2.2",
-                       "n_ao": mol.n_ao, "n_aux": mol.n_aux,
-                       "primitive_pairs": int(sum(len(p.P_p) for p in pairs.values())),
-                       "output": "packed (n_ao(n_ao+1)/2, n_aux) float64 (ij|P) plus dense (n_aux, n_aux) float64 "
-                                 "(P|Q), on device, every step"},
-            "roofline": roofline, "cpu_baseline": cpu,
-            "three_center_variants": variants,
-            "mixed_precision": mixed,
-            "batched": batched,
-            "serving": serving,
-            "max_abs_err_vs_reference_sample": err}
+    if args.plain80_only:
+        print(json.dumps({"plain80": plain80_leg(args)}), flush=True)
+        return
+    head = headline_leg(args)
+    plain80 = batched = serving = None
+    if not args.headline_only:
+        if not args.no_plain80:
+            plain80 = plain80_leg(args)
+        if not args.no_batched:
+            batched = batched_leg()
+        if not args.no_serving:
+            torch.cuda.empty_cache()
+            serving = serving_leg(args.line_serving_jobs, args.serving_rate)
+    line = {"metric": METRIC, "value": head["value"], "unit": "3c integrals/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded generators)",
+            "config": {"workload": "mixed-precision workload (200 atoms, 20 A box, 4s3p2d1f 1-6 prim, aux s-g 8 exp/l "
+                                   "ratio 2.2; n_ao 6000, n_aux 40000), auxiliary-block share of the first "
+                                   f"{args.slice_atoms} atoms (n_aux {head['n_aux_slice']}) x all "
+                                   f"{head['primitive_pairs']} primitive pairs; plain FP64",
+                       "n_ao": head["n_ao"], "n_aux_slice": head["n_aux_slice"], "n_aux_full": head["n_aux_full"],
+                       "primitive_pairs": head["primitive_pairs"],
+                       "output": "compact (ij|P) rows of significant shell pairs x slice columns, float64, streamed "
+                                 f"through a {BLOCK_BYTES / 1e9:.0f} GB device buffer in {head['blocks_per_step']} "
+                                 "blocks per step; inputs resident before timing"},
+            "roofline": head["roofline"], "cpu_baseline": head["cpu_baseline"],
+            "mixed_precision": head["mixed"],
+            "max_abs_err_vs_cpu_reference_sample": head["max_abs_err_vs_cpu_reference_sample"],
+            "class_seconds_per_step": head["class_seconds_per_step"],
+            "plain80": plain80, "batched": batched, "serving": serving,
+            "three_center_variants": variants}
     print(json.dumps(line), flush=True)
 
 

@@ -4,8 +4,8 @@ In the style of the reference's offline generator (gen/equation_mcmurchie_davids
 turns the MD recursions into straight-line C, but targeting gfx950 and restructured for it:
 
 * ``pairE_<la><lb>``: one thread per primitive pair.  Builds the pair's Hermite expansion E^{ij}_t by
-  the vertical recurrence and the binomial horizontal transfer (reference
-  mcmurchie_davidson_E_term.hpp:22-45, 166-199), contracts it straight into the *output* basis of the
+  the two-index Hermite recurrence (the coefficients of reference mcmurchie_davidson_E_term.hpp:22-45,
+  166-199, without its transfer between centres), contracts it straight into the *output* basis of the
   pair (real solid harmonics, or normalised Cartesians) and folds the pair coefficient in.  Entries are
   stored t-major, in exactly the order the three-centre kernels consume them, so those kernels read
   them as wave-uniform scalar loads.
@@ -84,41 +84,46 @@ PREFETCH_MAX_DOUBLES = 0
 
 
 # ---------------------------------------------------------------------------------------------
-# Hermite E (pair) code: VRR + HRR per axis
+# Hermite E (pair) code: two-index recurrence per axis
 # ---------------------------------------------------------------------------------------------
 
 def _emit_E_axis(lines, ax, la, lb, ty="double"):
-    """Emit E{ax}_{i}_{j}_{t} for i <= la, j <= lb, t <= i + j."""
-    L = la + lb
-    lines.append(f"    const {ty} E0{ax}_0_0 = 1.0;")
-    for i in range(L):
+    """Emit E{ax}_{i}_{j}_{t} for i <= la, j <= lb, t <= i + j by the two-index Hermite recurrence
+
+        E^{i+1,j}_t = 1/(2p) E^{ij}_{t-1} + PA E^{ij}_t + (t+1) E^{ij}_{t+1}   (first index), then
+        E^{i,j+1}_t = 1/(2p) E^{ij}_{t-1} + PB E^{ij}_t + (t+1) E^{ij}_{t+1}   (second index).
+
+    The reference builds E^{i+j,0} and transfers to the second centre, E^{ij} = sum_m C(j, m) AB^{j-m}
+    E^{i+m,0} (mcmurchie_davidson_E_term.hpp:166-199): the same coefficients, but for a diffuse A and a
+    tight B (P near B, |AB| of several Bohr) the transfer's terms of order AB^{i+j} cancel to a value
+    smaller by many orders (seen as 1e-4 relative error on a (df|s) integral in float64).  The
+    recurrence in j uses PB, which is small there, and cancels nothing."""
+    _emit_E_axis_two_index(lines, ax, la, lb, ty)
+
+
+def _emit_E_axis_two_index(lines, ax, la, lb, ty="double"):
+    """E{ax}_{i}_{j}_{t} by the recurrences in i (with PA) and j (with PB); see _emit_E_axis."""
+    def name(i, j, t):
+        return f"E{ax}_{i}_{j}_{t}"
+
+    def step(i, j, t, X, src):
+        terms = []
+        if t >= 1:
+            terms.append(f"o2p * {name(*src, t - 1)}")
+        if t <= sum(src):
+            terms.append(f"{X}{ax} * {name(*src, t)}")
+        if t + 1 <= sum(src):
+            terms.append(f"{fmt(t + 1)} * {name(*src, t + 1)}")
+        lines.append(f"    const {ty} {name(i, j, t)} = {' + '.join(terms)};")
+
+    lines.append(f"    const {ty} {name(0, 0, 0)} = 1.0;")
+    for i in range(la):
         for t in range(i + 2):
-            terms = []
-            if t >= 1:
-                terms.append(f"o2p * E0{ax}_{i}_{t - 1}")
-            if t <= i:
-                terms.append(f"PA{ax} * E0{ax}_{i}_{t}")
-            if t + 1 <= i:
-                terms.append(f"{fmt(t + 1)} * E0{ax}_{i}_{t + 1}")
-            lines.append(f"    const {ty} E0{ax}_{i + 1}_{t} = {' + '.join(terms)};")
-    if lb > 0:
-        for k in range(1, lb + 1):
-            lines.append(f"    const {ty} AB{ax}_p{k} = " + (f"AB{ax};" if k == 1 else f"AB{ax}_p{k - 1} * AB{ax};"))
+            step(i + 1, 0, t, "PA", (i, 0))
     for i in range(la + 1):
-        for j in range(lb + 1):
-            for t in range(i + j + 1):
-                terms = []
-                for m in range(j, -1, -1):
-                    if t > i + m:
-                        continue
-                    c = comb(j, j - m)
-                    pw = j - m
-                    base = f"E0{ax}_{i + m}_{t}"
-                    if pw == 0:
-                        terms.append(base if c == 1 else f"{fmt(c)} * {base}")
-                    else:
-                        terms.append(f"{fmt(c)} * AB{ax}_p{pw} * {base}")
-                lines.append(f"    const {ty} E{ax}_{i}_{j}_{t} = {' + '.join(terms)};")
+        for j in range(lb):
+            for t in range(i + j + 2):
+                step(i, j + 1, t, "PB", (i, j))
 
 
 def emit_pairE(la, lb, spherical):
@@ -145,8 +150,8 @@ def emit_pairE(la, lb, spherical):
     L.append("    } else { Px = Pp[0]; Py = Pp[1]; Pz = Pp[2]; p = Pp[3]; }")
     L.append("    const double o2p = 0.5 / p;")
     L.append("    const double PAx = Px - Aa[0], PAy = Py - Aa[1], PAz = Pz - Aa[2];")
-    L.append("    const double ABx = Aa[0] - Bb[0], ABy = Aa[1] - Bb[1], ABz = Aa[2] - Bb[2];")
-    L.append("    (void) PAx; (void) PAy; (void) PAz; (void) ABx; (void) ABy; (void) ABz; (void) o2p;")
+    L.append("    const double PBx = Px - Bb[0], PBy = Py - Bb[1], PBz = Pz - Bb[2];")
+    L.append("    (void) PAx; (void) PAy; (void) PAz; (void) PBx; (void) PBy; (void) PBz; (void) o2p;")
     for ax in "xyz":
         _emit_E_axis(L, ax, la, lb)
     stride = pair_stride(la, lb, spherical) if spherical else nnz
@@ -290,8 +295,8 @@ def _prologue_lines(P, indent):
         f"{indent}{{ const double dx = {P}0 - Cx, dy = {P}1 - Cy, dz = {P}2 - Cz, pe = {P}3;",
         f"{indent}  double z_, f_;",
         f"{indent}  if (a.omega == 0.0) {{ const double rs = eip::rsq_nr(pe + q); z_ = pe * q * (rs * rs); f_ = ckq * eip::rcp_nr(pe) * rs; }}",
-        f"{indent}  else {{ const double w2 = a.omega * a.omega; z_ = 1.0 / (1.0 / pe + 1.0 / q + 1.0 / w2);",
-        f"{indent}         f_ = ck * eip::kTwoPi25 * hl * a.omega / (pe * q * sqrt(pe * q + (pe + q) * w2)); }}",
+        f"{indent}  else {{ const double w2 = a.omega * a.omega; z_ = pe * q * eip::rcp_nr(q + pe + pe * q * eip::rcp_nr(w2));",
+        f"{indent}         f_ = ckq * a.omega * eip::rcp_nr(pe) * eip::rsq_nr(pe * q + (pe + q) * w2); }}",
         f"{indent}  nzeta = z_; npf = f_; ndx = dx; ndy = dy; ndz = dz; nT = z_ * (dx * dx + dy * dy + dz * dz); }}",
     ]
 
@@ -497,9 +502,11 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64", stage=None):
         out.append("            zeta = p * q * (rs * rs);")
         out.append("            pf = ckq * eip::rcp_nr(p) * rs;")
         out.append("        } else {")
+        # zeta = 1 / (1/p + 1/q + 1/w^2), pf = ck 2 pi^(5/2) hl w / (p q sqrt(p q + (p + q) w^2)) by the same
+        # refined reciprocal and reciprocal square root (reference three_center.cpp:46-49)
         out.append("            const double w2 = a.omega * a.omega;")
-        out.append("            zeta = 1.0 / (1.0 / p + 1.0 / q + 1.0 / w2);")
-        out.append("            pf = ck * eip::kTwoPi25 * hl * a.omega / (p * q * sqrt(p * q + (p + q) * w2));")
+        out.append("            zeta = p * q * eip::rcp_nr(q + p + p * q * eip::rcp_nr(w2));")
+        out.append("            pf = ckq * a.omega * eip::rcp_nr(p) * eip::rsq_nr(p * q + (p + q) * w2);")
         out.append("        }")
         out.append(f"        const {ty} PQx = ({ty}) dPQx, PQy = ({ty}) dPQy, PQz = ({ty}) dPQz;")
         out.append(f"        const {ty} T = ({ty}) (zeta * (dPQx * dPQx + dPQy * dPQy + dPQz * dPQz));")
@@ -849,3 +856,227 @@ def m_chunks(la, lb, lc, shrink=0, budget=None):
     n = -(-nk // per)
     per = -(-nk // n)          # balance
     return [(m, min(nk, m + per)) for m in range(0, nk, per)]
+
+
+# ---------------------------------------------------------------------------------------------
+# Three-centre kernel on the FP64 matrix cores (v_mfma_f64_16x16x4f64)
+# ---------------------------------------------------------------------------------------------
+#
+# For one auxiliary component m and one block b of 16 pair-output rows, the contraction over a group
+# of four primitive pairs is a 16x16x4 matrix product
+#     acc_m,b[row, aux] += sum_{k < 4} E^{pp_k}[16 b + row, t] * V^{pp_k, aux}[t, m]
+# issued once per pair Hermite index t.  Lane l of a wave holds A[row = l % 16][k = l / 16] (its own
+# primitive pair's pair-matrix entry, read from the dense layout below), B[k = l / 16][aux = l % 16]
+# (V of its own triplet: primitive pair pp_{l/16} with auxiliary shell aux_{l%16}, computed in the
+# lane's registers like the vector kernels do) and, in accumulator register r < 4, row 4 r + l / 16
+# of column l % 16 (lane layout checked on the device by a probe of the instruction).  A workgroup of four waves takes one work item (shell pair, 64 auxiliary shells) of
+# the vector kernels' item tables, 16 auxiliary shells per wave.  The accumulators live in the matrix
+# cores' registers, so a chunk of auxiliary components is bounded by the live Hermite integrals only.
+
+MFMA_WAVES = 4                   # waves per workgroup (16 auxiliary shells each)
+# Pair-matrix operands of a group loaded at the top of its iteration, ahead of the Boys function and
+# the Hermite integrals, so their latency is hidden behind that work (the rest are loaded in the
+# contraction, a few ahead of use): as many as fit beside the live Hermite integrals in this many
+# vector registers, at most MFMA_E_PREFETCH_MAX.
+MFMA_E_PREFETCH_VGPRS = 216
+MFMA_E_PREFETCH_MAX = 24
+
+
+def mfma_blocks(la, lb):
+    """Blocks of 16 pair-output rows."""
+    return -(-(n_out(la, True) * n_out(lb, True)) // 16)
+
+
+def mfma_hermite(la, lb):
+    """Pair Hermite indices t in the order of the packed layout (and of the dense layout's t axis)."""
+    ts = []
+    for t, _ in pair_layout(la, lb, True):
+        if t not in ts:
+            ts.append(t)
+    return ts
+
+
+def mfma_dense_width(la, lb):
+    """Doubles per primitive pair in the dense layout: [t][16 * blocks] (structural zeros stored)."""
+    return len(mfma_hermite(la, lb)) * 16 * mfma_blocks(la, lb)
+
+
+def emit_pair_dense(la, lb):
+    """pairD_<la><lb>: packed pair-matrix rows -> the dense [pp][t][16 * blocks] layout the matrix-core
+    kernels read (one thread per primitive pair)."""
+    name = f"pairD_{la}{lb}"
+    ts = mfma_hermite(la, lb)
+    tix = {t: i for i, t in enumerate(ts)}
+    W = 16 * mfma_blocks(la, lb)
+    layout = pair_layout(la, lb, True)
+    stride = pair_stride(la, lb, True)
+    where = {(tix[t], row): k for k, (t, row) in enumerate(layout)}
+    L = [f"// dense pair matrix of class ({la},{lb}): {len(ts)} Hermite indices x {W} rows"]
+    L.append(f'extern "C" __global__ void __launch_bounds__(256) {name}(const double* __restrict__ E, '
+             f'double* __restrict__ D, int n_pp)')
+    L.append("{")
+    L.append("    const int pp = blockIdx.x * blockDim.x + threadIdx.x;")
+    L.append("    if (pp >= n_pp) return;")
+    L.append(f"    const double* __restrict__ e = E + (size_t) pp * {stride};")
+    L.append(f"    double2* __restrict__ d = reinterpret_cast<double2*>(D + (size_t) pp * {len(ts) * W});")
+    for ti in range(len(ts)):
+        for r in range(0, W, 2):
+            v = [f"e[{where[(ti, rr)]}]" if (ti, rr) in where else "0.0" for rr in (r, r + 1)]
+            L.append(f"    d[{(ti * W + r) // 2}] = make_double2({v[0]}, {v[1]});")
+    L.append("}")
+    return name, "\n".join(L)
+
+
+def mfma_chunks(la, lb, lc, r_budget, acc_budget=224):
+    """Split the auxiliary components over matrix-core kernels: each chunk keeps at most r_budget
+    Hermite integrals live (vector registers) and 8 * components * blocks accumulator registers."""
+    nk = 2 * lc + 1
+    nb = mfma_blocks(la, lb)
+    per = nk
+    while per > 1:
+        n = -(-nk // per)
+        p = -(-nk // n)
+        ok = all(_needed_R(la, lb, lc, m, min(nk, m + p)) <= r_budget and 8 * (min(nk, m + p) - m) * nb <= acc_budget
+                 for m in range(0, nk, p))
+        if ok:
+            break
+        per -= 1
+    n = -(-nk // per)
+    per = -(-nk // n)
+    return [(m, min(nk, m + per)) for m in range(0, nk, per)]
+
+
+def emit_j3c_mfma(la, lb, lc, m0, m1):
+    """Float64 three-centre kernel on the matrix cores for auxiliary components [m0, m1) of class
+    (la lb | lc); reads the dense pair matrix (J3cData.E points to it) and the vector kernels' items."""
+    spherical = True
+    name = f"j3cm_{la}{lb}{lc}_{m0}_{m1}"
+    L = la + lb + lc
+    nij = n_out(la, spherical) * n_out(lb, spherical)
+    nj = n_out(lb, spherical)
+    nb = mfma_blocks(la, lb)
+    W = 16 * nb
+    ts = mfma_hermite(la, lb)
+    DS = len(ts) * W
+    aux_rows = aux_transform(lc, spherical)
+    ms = list(range(m0, m1))
+    needed = set()
+    for t in ts:
+        for m in ms:
+            for s in aux_rows[m][1]:
+                needed.add((t[0] + s[0], t[1] + s[1], t[2] + s[2]))
+    out = [f"// ({la}{lb}|{lc}) aux components [{m0},{m1}) f64 matrix cores: L = {L}, {len(ts)} Hermite indices, "
+           f"{nb} row blocks"]
+    out.append("template <bool kJobs>")
+    out.append(f"__device__ __forceinline__ void {name}_body(const J3cArgs a)")
+    out.append("{")
+    out.append("    const int item = a.item_begin + eip::xcd_remap(blockIdx.x, gridDim.x);")
+    out.append("    if (item >= a.item_end) return;")
+    out.append("    const eip::cptr<J3cData> jt = kJobs ? j3c_item_job(a, item) : (eip::cptr<J3cData>) nullptr;")
+    out.append("#define JD(f) (kJobs ? (decltype(a.d.f)) jt->f : a.d.f)")
+    out.append("    const int wave = (int) threadIdx.x >> 6, lane = (int) threadIdx.x & 63;")
+    out.append("    const int kslot = lane >> 4, jcol = lane & 15;")
+    out.append("    const int sp = eip::ro(a.items)[2 * item];")
+    out.append("    const int ks_first = eip::ro(a.items)[2 * item + 1] + 16 * wave;")
+    out.append("    const int ks_last = a.items_ks_end ? eip::ro(a.items_ks_end)[item] : (int) JD(n_ks);")
+    out.append("    if (ks_first >= ks_last) return;                 // no auxiliary shell for this wave")
+    out.append("    const int ks_raw = ks_first + jcol;")
+    out.append("    const bool active = ks_raw < ks_last;")
+    out.append("    const int ks = active ? ks_raw : ks_last - 1;")
+    out.append("    const int ksh = JD(ks_perm) ? eip::ro(JD(ks_perm))[ks] : ks;")
+    out.append("    const int kp0 = eip::ro(JD(ks_prim_off))[ksh], kp1 = eip::ro(JD(ks_prim_off))[ksh + 1];")
+    out.append("    const int pp0 = a.items_pp ? eip::ro(a.items_pp)[2 * item] : eip::ro(JD(sp_pp_off))[sp];")
+    out.append("    const int pp1 = a.items_pp ? eip::ro(a.items_pp)[2 * item + 1] : eip::ro(JD(sp_pp_off))[sp + 1];")
+    out.append("    const double* __restrict__ ED = JD(E);")
+    out.append("    const double* __restrict__ PP = JD(pp_P);")
+    for m in ms:
+        for b in range(nb):
+            out.append(f"    eip::v4d acc_{m}_{b} = {{0.0, 0.0, 0.0, 0.0}};")
+    out.append("    const int nkp_wave = eip::wave_max(kp1 - kp0);")
+    out.append("    for (int t_kp = 0; t_kp < nkp_wave; ++t_kp) {")
+    out.append("      const bool kp_on = kp0 + t_kp < kp1;")
+    out.append("      const int kp = kp_on ? kp0 + t_kp : kp1 - 1;")
+    out.append("      const double Cx = JD(ka_C)[4 * kp + 0], Cy = JD(ka_C)[4 * kp + 1], Cz = JD(ka_C)[4 * kp + 2];")
+    out.append("      const double q = JD(ka_C)[4 * kp + 3];")
+    out.append("      const double ck = kp_on ? JD(ka_coef)[kp] : 0.0;")
+    out.extend(_aux_prefactor_lines(lc))
+    # centre and exponent of the lane's primitive pair in the first group, then one group ahead
+    out.append("      double Pn0 = 0, Pn1 = 0, Pn2 = 0, Pn3 = 1;")
+    out.append("      if (pp0 < pp1) { const int q0 = min(pp0 + kslot, pp1 - 1);")
+    out.append("        Pn0 = PP[4 * (size_t) q0 + 0]; Pn1 = PP[4 * (size_t) q0 + 1]; Pn2 = PP[4 * (size_t) q0 + 2]; Pn3 = PP[4 * (size_t) q0 + 3]; }")
+    out.append("      for (int g0 = pp0; g0 < pp1; g0 += 4) {")
+    out.append("        const int pq = g0 + kslot;")
+    out.append("        const bool pp_on = pq < pp1;")
+    out.append("        const int pp = pp_on ? pq : pp1 - 1;")
+    out.append("        const double Px = Pn0, Py = Pn1, Pz = Pn2, p = Pn3;")
+    out.append(f"        const double* __restrict__ Ed = ED + (size_t) pp * {DS} + jcol;")
+    tb = [(ti, b) for ti in range(len(ts)) for b in range(nb)]
+    n_pre = max(0, min(MFMA_E_PREFETCH_MAX, (MFMA_E_PREFETCH_VGPRS - 2 * len(needed)) // 2))
+    pre = tb[:n_pre]
+    for ti, b in pre:
+        out.append(f"        const double ea_{ti}_{b} = Ed[{ti * W + 16 * b}];")
+    out.append("        { const int qn = min(g0 + 4 + kslot, pp1 - 1);      // the next group's pair (the last reloads one)")
+    out.append("          Pn0 = PP[4 * (size_t) qn + 0]; Pn1 = PP[4 * (size_t) qn + 1]; Pn2 = PP[4 * (size_t) qn + 2]; Pn3 = PP[4 * (size_t) qn + 3]; }")
+    out.append("        const double dPQx = Px - Cx, dPQy = Py - Cy, dPQz = Pz - Cz;")
+    out.append("        double zeta, pf;")
+    out.append("        if (a.omega == 0.0) {")
+    out.append("            const double rs = eip::rsq_nr(p + q);")
+    out.append("            zeta = p * q * (rs * rs);")
+    out.append("            pf = ckq * eip::rcp_nr(p) * rs;")
+    out.append("        } else {")
+    out.append("            const double w2 = a.omega * a.omega;")
+    out.append("            zeta = p * q * eip::rcp_nr(q + p + p * q * eip::rcp_nr(w2));")
+    out.append("            pf = ckq * a.omega * eip::rcp_nr(p) * eip::rsq_nr(p * q + (p + q) * w2);")
+    out.append("        }")
+    out.append("        if (!pp_on) pf = 0.0;                        // an empty slot of the group adds nothing")
+    out.append("        const double PQx = dPQx, PQy = dPQy, PQz = dPQz;")
+    out.append("        const double T = zeta * (dPQx * dPQx + dPQy * dPQy + dPQz * dPQz);")
+    out.append(f"        double F[{L + 1}];")
+    out.append(f"        eip::boys64<{L}>(T, F);")
+    out.append("        const double m2z = -2.0 * zeta;")
+    out.append("        const double R000_0 = pf * F[0];")
+    for n in range(1, L + 1):
+        out.append(f"        const double pf_{n} = {'pf' if n == 1 else f'pf_{n - 1}'} * m2z;")
+        out.append(f"        const double R000_{n} = pf_{n} * F[{n}];")
+    _emit_R(out, L, needed, "double")
+    for ti, t in enumerate(ts):
+        for m in ms:
+            r, terms = aux_rows[m]
+            pieces = []
+            for s, (qf, j) in terms.items():
+                assert j == 0
+                coef = SqrtRational(r, qf).to_float()
+                rn = _r_name(0, (t[0] + s[0], t[1] + s[1], t[2] + s[2]))
+                pieces.append(rn if coef == 1.0 else f"{fmt(coef)} * {rn}")
+            out.append(f"        const double V_{ti}_{m} = {' + '.join(pieces)};")
+        for b in range(nb):
+            if (ti, b) in pre:
+                out.append(f"        {{ const double ea = ea_{ti}_{b};")
+            else:
+                out.append(f"        {{ const double ea = Ed[{ti * W + 16 * b}];")
+            for m in ms:
+                out.append(f"          acc_{m}_{b} = __builtin_amdgcn_mfma_f64_16x16x4f64(ea, V_{ti}_{m}, acc_{m}_{b}, 0, 0, 0);")
+            out.append("        }")
+    out.append("      }")
+    out.append("    }")
+    out.append("    if (!active) return;")
+    out.append("    const int col = eip::ro(JD(ks_col))[ksh];")
+    out.append("    const int I = eip::ro(JD(sp_i))[sp], J = eip::ro(JD(sp_j))[sp];")
+    out.append("    __attribute__((address_space(1))) double* __restrict__ o = (__attribute__((address_space(1))) double*) JD(out) + (JD(sp_out_base) ? eip::ro(JD(sp_out_base))[sp] : 0LL);")
+    out.append("    const long long ld = JD(sp_ld) ? (long long) eip::ro(JD(sp_ld))[sp] : JD(ld);")
+    out.append(f"    const long long base = (long long) sp * {nij};")
+    for b in range(nb):
+        for r_ in range(4):
+            out.append(f"    {{ const int ij = {16 * b + 4 * r_} + kslot;")
+            out.append(f"      const int mi = ij / {nj}, mj = ij - mi * {nj};")
+            out.append(f"      if (ij < {nij} && (a.layout != 0 || I != J || mi >= mj)) {{")
+            out.append("        const long long row = (a.layout == 0 ? eip::packed_row(I + mi, J + mj) : base + ij) * ld + col;")
+            for m in ms:
+                out.append(f"        if (a.accumulate) o[row + {m}] += acc_{m}_{b}[{r_}]; else o[row + {m}] = acc_{m}_{b}[{r_}];")
+            out.append("      }")
+            out.append("    }")
+    out.append("#undef JD")
+    out.append("}")
+    out.append(f'extern "C" __global__ void __launch_bounds__({64 * MFMA_WAVES}) {name}(const J3cArgs a) {{ {name}_body<false>(a); }}')
+    out.append(f'extern "C" __global__ void __launch_bounds__({64 * MFMA_WAVES}) {name}_jobs(const J3cArgs a) {{ {name}_body<true>(a); }}')
+    return name, "\n".join(out)

@@ -12,12 +12,17 @@ import os
 import sys
 
 from . import boys_table
-from .gen_kernels import (emit_j2c, emit_j3c, emit_nuc, emit_pairE, m_chunks, pair_layout)
+from .gen_kernels import (emit_j2c, emit_j3c, emit_j3c_mfma, emit_nuc, emit_pair_dense, emit_pairE, m_chunks,
+                          mfma_chunks, mfma_dense_width, pair_layout)
 
 SPEC_MAX_L = 3
 SPEC_MAX_AUX_L = 4
 # Register budgets (doubles per lane) of the chunk-plan variants; variant 0 is the default.
 VARIANT_BUDGETS = (200, 240, 160, 120, 320)
+# Float64 matrix-core variants (appended after the vector ones) for pair classes with at least 9 output
+# rows: budgets of live Hermite integrals per chunk (the accumulators sit in the matrix-core registers).
+MFMA_PAIR_CLASSES = ((1, 1), (1, 2), (2, 2), (1, 3), (2, 3), (3, 3))
+MFMA_R_BUDGETS = (84, 112, 160, 220)
 
 HEADER = """// Generated by codegen/generate.py -- do not edit.
 #include "../md_device.hpp"
@@ -82,12 +87,30 @@ def generate(out_dir, shrink=None):
     disp.append("    }\n    return -1;\n}\n")
     reg.append("int eip_pair_nnz(int la, int lb);")
 
+    # ---- dense pair matrices for the matrix-core kernels ----
+    src = [HEADER]
+    cases = []
+    for la, lb in MFMA_PAIR_CLASSES:
+        name, code = emit_pair_dense(la, lb)
+        src.append(code)
+        cases.append(f"        case {la * 10 + lb}: hipLaunchKernelGGL({name}, dim3((n_pp + 255) / 256), dim3(256), 0, s, E, D, n_pp); return 0;")
+    src.append("int eip_launch_pairD(int la, int lb, const double* E, double* D, int n_pp, hipStream_t s)\n{\n"
+               "    switch (la * 10 + lb) {\n" + "\n".join(cases) + "\n    }\n    return -1;\n}\n")
+    _write(os.path.join(out_dir, "pairD.hip"), "\n".join(src))
+    reg.append("int eip_launch_pairD(int la, int lb, const double* E, double* D, int n_pp, hipStream_t s);")
+    disp.append("int eip_pair_dense_width(int la, int lb)\n{\n    switch (la * 10 + lb) {")
+    for la, lb in MFMA_PAIR_CLASSES:
+        disp.append(f"        case {la * 10 + lb}: return {mfma_dense_width(la, lb)};")
+    disp.append("    }\n    return -1;\n}\n")
+    reg.append("int eip_pair_dense_width(int la, int lb);")
+
     # ---- three-centre ----
     # Each class gets up to len(VARIANT_BUDGETS) chunk plans (register budgets); variant 0 is the
     # default, the others are candidates for the autotuner.  Identical chunks are emitted once.
-    plan_rows = []          # (la, lb, lc, variant, chunk, m0, m1)
+    plan_rows = []          # (la, lb, lc, variant, chunk, m0, m1, matrix cores)
     for la, lb in pair_classes:
         plans = {}
+        mplans = {}
         for lc in range(SPEC_MAX_AUX_L + 1):
             cls_plans = []
             for budget in VARIANT_BUDGETS:
@@ -97,7 +120,30 @@ def generate(out_dir, shrink=None):
             plans[lc] = cls_plans
             for v, plan in enumerate(cls_plans):
                 for ci, (m0, m1) in enumerate(plan):
-                    plan_rows.append((la, lb, lc, v, ci, m0, m1))
+                    plan_rows.append((la, lb, lc, v, ci, m0, m1, 0))
+            mplans[lc] = []
+            if (la, lb) in MFMA_PAIR_CLASSES:
+                for budget in MFMA_R_BUDGETS:
+                    plan = mfma_chunks(la, lb, lc, budget)
+                    if plan not in mplans[lc]:
+                        mplans[lc].append(plan)
+                for k, plan in enumerate(mplans[lc]):
+                    for ci, (m0, m1) in enumerate(plan):
+                        plan_rows.append((la, lb, lc, len(cls_plans) + k, ci, m0, m1, 1))
+        if (la, lb) in MFMA_PAIR_CLASSES:
+            src = [HEADER]
+            cases = []
+            for lc, cls_plans in mplans.items():
+                for m0, m1 in sorted({c for plan in cls_plans for c in plan}):
+                    name, code = emit_j3c_mfma(la, lb, lc, m0, m1)
+                    src.append(code)
+                    cases.append(f"        case {lc * 10000 + m0 * 100 + m1}: if (a.jobs) hipLaunchKernelGGL({name}_jobs, dim3(nblocks), dim3(256), 0, s, a);"
+                                 f" else hipLaunchKernelGGL({name}, dim3(nblocks), dim3(256), 0, s, a); return 0;")
+            fn = f"eip_launch_j3cm_{la}{lb}"
+            src.append(f"int {fn}(int lc, int m0, int m1, const J3cArgs& a, int nblocks, hipStream_t s)\n{{\n"
+                       "    switch (lc * 10000 + m0 * 100 + m1) {\n" + "\n".join(cases) + "\n    }\n    return -1;\n}\n")
+            _write(os.path.join(out_dir, f"j3cm_{la}{lb}.hip"), "\n".join(src))
+            reg.append(f"int {fn}(int lc, int m0, int m1, const J3cArgs& a, int nblocks, hipStream_t s);")
         for prec in ("f64", "f32"):
             src = [HEADER]
             cases = []
@@ -116,9 +162,9 @@ def generate(out_dir, shrink=None):
     reg.append("int eip_j3c_nvariants(int la, int lb, int lc);")
     reg.append("int eip_j3c_nchunks(int la, int lb, int lc, int variant);")
     reg.append("int eip_j3c_chunk(int la, int lb, int lc, int variant, int chunk, int* m0, int* m1);")
-    disp.append("static const short kChunks[][7] = {")
+    disp.append("static const short kChunks[][8] = {")
     for row in plan_rows:
-        disp.append("    {%d, %d, %d, %d, %d, %d, %d}," % row)
+        disp.append("    {%d, %d, %d, %d, %d, %d, %d, %d}," % row)
     disp.append("};")
     disp.append("int eip_j3c_chunk(int la, int lb, int lc, int variant, int chunk, int* m0, int* m1)\n{\n"
                 "    for (const auto& r : kChunks)\n"
@@ -128,9 +174,21 @@ def generate(out_dir, shrink=None):
                 "    for (const auto& r : kChunks) n += (r[0] == la && r[1] == lb && r[2] == lc && r[3] == variant);\n    return n;\n}\n")
     disp.append("int eip_j3c_nvariants(int la, int lb, int lc)\n{\n    int n = 0;\n"
                 "    for (const auto& r : kChunks) if (r[0] == la && r[1] == lb && r[2] == lc && r[3] + 1 > n) n = r[3] + 1;\n    return n;\n}\n")
+    disp.append("int eip_j3c_variant_mfma(int la, int lb, int lc, int variant)\n{\n"
+                "    for (const auto& r : kChunks)\n"
+                "        if (r[0] == la && r[1] == lb && r[2] == lc && r[3] == variant) return r[7];\n    return -1;\n}\n")
+    reg.append("int eip_j3c_variant_mfma(int la, int lb, int lc, int variant);")
     disp.append("int eip_launch_j3c(int la, int lb, int lc, int variant, int chunk, int f32, const J3cArgs& a, int nblocks, hipStream_t s)\n{")
     disp.append("    int m0, m1;")
     disp.append("    if (eip_j3c_chunk(la, lb, lc, variant, chunk, &m0, &m1) != 0) return -1;")
+    disp.append("    if (eip_j3c_variant_mfma(la, lb, lc, variant) == 1) {")
+    disp.append("        if (f32) return -1;                    // matrix-core kernels are float64 only")
+    disp.append("        switch (la * 10 + lb) {")
+    for la, lb in MFMA_PAIR_CLASSES:
+        disp.append(f"            case {la * 10 + lb}: return eip_launch_j3cm_{la}{lb}(lc, m0, m1, a, nblocks, s);")
+    disp.append("        }")
+    disp.append("        return -1;")
+    disp.append("    }")
     disp.append("    switch (la * 10 + lb) {")
     for la, lb in pair_classes:
         disp.append(f"        case {la * 10 + lb}: return f32 ? eip_launch_j3c_{la}{lb}_f32(lc, m0, m1, a, nblocks, s)"

@@ -20,6 +20,8 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <memory>
+#include <mutex>
 #include <numeric>
 #include <thread>
 #include <vector>
@@ -55,21 +57,91 @@ hipStream_t thread_stream()
     return s;
 }
 
-// Device buffer owned by one call, released on scope exit.
-struct DevBuf {
-    void* p = nullptr;
-    ~DevBuf() { if (p) (void) hipFree(p); }
-    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 8); }
-    template <class T> T* as() const { return static_cast<T*>(p); }
+// Device memory of one C-ABI call.  By default every buffer is a hipMalloc allocation released when
+// the call returns.  Over a fixed region (the no-allocation entry point) buffers are consecutive
+// 256-byte-aligned pieces of that region and nothing is allocated; a request that does not fit returns
+// hipErrorOutOfMemory.  Host arrays uploaded with put() are kept alive until the call returns -- after
+// its last stream synchronisation -- so an asynchronous copy never reads released host memory.
+class CallMem {
+  public:
+    CallMem() = default;
+    CallMem(char* region, size_t bytes) : region_(region), cap_(bytes) {}
+    ~CallMem()
+    {
+        // an early return may leave copies from kept host arrays in flight: wait for them first
+        if (!keep_.empty()) (void) hipStreamSynchronize(stream_);
+        for (void* p : owned_) (void) hipFree(p);
+    }
+    CallMem(const CallMem&) = delete;
+    CallMem& operator=(const CallMem&) = delete;
+
+    bool fixed() const { return region_ != nullptr; }
+    size_t capacity() const { return cap_; }
+    // bytes still available in the fixed region (unbounded otherwise)
+    size_t left() const { return fixed() ? cap_ - std::min(cap_, round_up(used_)) : SIZE_MAX; }
+
+    template <class T>
+    hipError_t get(T** p, size_t n)
+    {
+        const size_t bytes = std::max<size_t>(n * sizeof(T), 8);
+        if (fixed()) {
+            const size_t off = round_up(used_);
+            if (off > cap_ || bytes > cap_ - off) return hipErrorOutOfMemory;
+            *p = reinterpret_cast<T*>(region_ + off);
+            used_ = off + bytes;
+            return hipSuccess;
+        }
+        void* q = nullptr;
+        const hipError_t e = hipMalloc(&q, bytes);
+        if (e == hipSuccess) {
+            owned_.push_back(q);
+            *p = static_cast<T*>(q);
+        }
+        return e;
+    }
+
+    // Device copy of a host array (the host array is moved into the call's keep list first).
+    template <class T>
+    hipError_t put(T** p, std::vector<T>&& host, hipStream_t s)
+    {
+        const hipError_t e = get(p, host.size());
+        if (e != hipSuccess) return e;
+        auto h = std::make_shared<std::vector<T>>(std::move(host));
+        keep_.push_back(h);
+        stream_ = s;
+        return h->empty() ? hipSuccess : hipMemcpyAsync(*p, h->data(), h->size() * sizeof(T), hipMemcpyHostToDevice, s);
+    }
+
+    template <class T>
+    hipError_t put(T** p, const std::vector<T>& host, hipStream_t s)
+    {
+        return put(p, std::vector<T>(host), s);
+    }
+
+  private:
+    static size_t round_up(size_t x) { return (x + 255) & ~size_t(255); }
+    char* region_ = nullptr;
+    size_t cap_ = 0, used_ = 0;
+    std::vector<void*> owned_;
+    std::vector<std::shared_ptr<void>> keep_;
+    hipStream_t stream_ = nullptr;        // every put() of one call uses the call's one stream
 };
 
-template <class T>
-hipError_t upload(DevBuf& b, const std::vector<T>& v, hipStream_t s)
-{
-    hipError_t e = b.alloc(v.size() * sizeof(T));
-    if (e != hipSuccess) return e;
-    return hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
-}
+// Memory request of a call: rc 5 when a fixed region is too small for this call's inputs, else rc 2.
+#define EIP_MEM(mem, expr)                                                                 \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            if ((mem).fixed() && e_ == hipErrorOutOfMemory) {                              \
+                fprintf(stderr, "libmolecular_integral: %s: this call needs more than the "  \
+                        "fixed %zu-byte scratch region\n", __func__, (mem).capacity());    \
+                return 5;                                                                  \
+            }                                                                              \
+            fprintf(stderr, "libmolecular_integral: %s failed: %s (%s:%d)\n", #expr,       \
+                    hipGetErrorString(e_), __FILE__, __LINE__);                            \
+            return 2;                                                                      \
+        }                                                                                  \
+    } while (0)
 
 // Stable grouping of items by an integer key pair.  Returns the permutation (items in group order),
 // group offsets (CSR) and the key pair of each group.
@@ -131,44 +203,40 @@ bool unsupported(const char* fn, int i_L, int j_L, int k_L = -1)
     return true;
 }
 
-// Grouped primitive pairs of one orbital class, resident on the device.
+// Grouped primitive pairs of one orbital class, resident on the device for one call.
 struct DevPairs {
     Grouping g;
-    DevBuf P, A, B, coef, off, E;
+    double *P = nullptr, *A = nullptr, *B = nullptr, *coef = nullptr, *E = nullptr;
+    int* off = nullptr;
     int n_pp = 0, n_sp = 0;
     bool spec = false;     // packed specialised pair matrix (else dense generic)
     int nnz = 0;           // entries per primitive pair
 };
 
 // allow_spec: the consumer has a specialised kernel for this class (else the dense generic layout).
-int prepare_pairs(DevPairs& d, int la, int lb, bool sph, const double* P_p, const double* A_a, const double* B_b,
-                  const double* coef, const int* i_start, const int* j_start, int n, bool recompute_P, bool allow_spec,
-                  hipStream_t s)
+int prepare_pairs(DevPairs& d, CallMem& mem, int la, int lb, bool sph, const double* P_p, const double* A_a,
+                  const double* B_b, const double* coef, const int* i_start, const int* j_start, int n,
+                  bool recompute_P, bool allow_spec, hipStream_t s)
 {
     d.g = group_by(i_start, j_start, n);
     d.n_pp = n;
     d.n_sp = (int) d.g.key_a.size();
-    std::vector<double> hP = P_p ? gather_rows(P_p, d.g.order, 4) : std::vector<double>(4 * (size_t) n, 0.0);
-    std::vector<double> hA = gather_rows(A_a, d.g.order, 4), hB = gather_rows(B_b, d.g.order, 4);
-    std::vector<double> hc = gather_rows(coef, d.g.order, 1);
-    EIP_CHECK(upload(d.P, hP, s));
-    EIP_CHECK(upload(d.A, hA, s));
-    EIP_CHECK(upload(d.B, hB, s));
-    EIP_CHECK(upload(d.coef, hc, s));
-    EIP_CHECK(upload(d.off, d.g.off, s));
+    EIP_MEM(mem, mem.put(&d.P, P_p ? gather_rows(P_p, d.g.order, 4) : std::vector<double>(4 * (size_t) n, 0.0), s));
+    EIP_MEM(mem, mem.put(&d.A, gather_rows(A_a, d.g.order, 4), s));
+    EIP_MEM(mem, mem.put(&d.B, gather_rows(B_b, d.g.order, 4), s));
+    EIP_MEM(mem, mem.put(&d.coef, gather_rows(coef, d.g.order, 1), s));
+    EIP_MEM(mem, mem.put(&d.off, d.g.off, s));
     d.spec = allow_spec && sph && la <= EIP_SPEC_MAX_L && lb <= EIP_SPEC_MAX_L;
     if (d.spec) {
         d.nnz = eip_pair_nnz(la, lb);
-        EIP_CHECK(d.E.alloc(sizeof(double) * (size_t) n * d.nnz));
-        PairPrepArgs a{d.P.as<double>(), d.A.as<double>(), d.B.as<double>(), d.coef.as<double>(),
-                       d.E.as<double>(), nullptr, n, recompute_P ? 1 : 0};
+        EIP_MEM(mem, mem.get(&d.E, (size_t) n * d.nnz));
+        PairPrepArgs a{d.P, d.A, d.B, d.coef, d.E, nullptr, n, recompute_P ? 1 : 0};
         if (eip_launch_pairE(la, lb, a, (n + 255) / 256, s) != 0) return 3;
     } else {
         const int L = la + lb;
         d.nnz = nout(la, sph) * nout(lb, sph) * (L + 1) * (L + 2) * (L + 3) / 6;
-        EIP_CHECK(d.E.alloc(sizeof(double) * (size_t) n * d.nnz));
-        GenPairArgs a{la, lb, sph ? 1 : 0, recompute_P ? 1 : 0, d.P.as<double>(), d.A.as<double>(),
-                      d.B.as<double>(), d.coef.as<double>(), n, d.E.as<double>()};
+        EIP_MEM(mem, mem.get(&d.E, (size_t) n * d.nnz));
+        GenPairArgs a{la, lb, sph ? 1 : 0, recompute_P ? 1 : 0, d.P, d.A, d.B, d.coef, n, d.E};
         EIP_CHECK((hipError_t) eip_gen_pair_matrix(a, s));
     }
     EIP_CHECK(hipGetLastError());
@@ -257,18 +325,19 @@ int overlap(int i_L, int j_L, const double* pair_A_a, const double* pair_B_b, co
     if (!(i_L >= 0 && j_L >= 0 && i_L <= j_L && j_L <= kMaxL) && unsupported(__func__, i_L, j_L)) return 1;
     if (n_pair <= 0) return 0;
     hipStream_t s = thread_stream();
+    CallMem mem;
     DevPairs d;
-    if (int rc = prepare_pairs(d, i_L, j_L, spherical, nullptr, pair_A_a, pair_B_b, pair_coefficient,
+    if (int rc = prepare_pairs(d, mem, i_L, j_L, spherical, nullptr, pair_A_a, pair_B_b, pair_coefficient,
                                pair_i_ao_start, pair_j_ao_start, n_pair, true, true, s)) return rc;
     const int ni = nout(i_L, spherical), nj = nout(j_L, spherical), nrow = ni * nj;
-    DevBuf out;
-    EIP_CHECK(out.alloc(sizeof(double) * (size_t) d.n_sp * nrow));
+    double* out = nullptr;
+    EIP_MEM(mem, mem.get(&out, (size_t) d.n_sp * nrow));
     const int L = i_L + j_L;
-    GenOverlapArgs a{d.off.as<int>(), d.A.as<double>(), d.B.as<double>(), d.E.as<double>(),
-                     d.nnz, d.spec ? 1 : (long long) (L + 1) * (L + 2) * (L + 3) / 6, d.n_sp, nrow, out.as<double>()};
+    GenOverlapArgs a{d.off, d.A, d.B, d.E, d.nnz, d.spec ? 1 : (long long) (L + 1) * (L + 2) * (L + 3) / 6, d.n_sp,
+                     nrow, out};
     EIP_CHECK((hipError_t) eip_gen_overlap(a, s));
     std::vector<double> h((size_t) d.n_sp * nrow);
-    EIP_CHECK(hipMemcpyAsync(h.data(), out.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    EIP_CHECK(hipMemcpyAsync(h.data(), out, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
     EIP_CHECK(hipStreamSynchronize(s));
     for (int sp = 0; sp < d.n_sp; ++sp) {
         const int I = d.g.key_a[sp], J = d.g.key_b[sp];
@@ -290,27 +359,25 @@ int nuclear_attraction(int i_L, int j_L, const double* pair_P_p, const double* p
     if (!(i_L >= 0 && j_L >= 0 && i_L <= j_L && j_L <= kMaxL) && unsupported(__func__, i_L, j_L)) return 1;
     if (n_pair <= 0 || n_charge <= 0) return 0;
     hipStream_t s = thread_stream();
+    CallMem mem;
     DevPairs d;
-    if (int rc = prepare_pairs(d, i_L, j_L, spherical, pair_P_p, pair_A_a, pair_B_b, pair_coefficient,
+    if (int rc = prepare_pairs(d, mem, i_L, j_L, spherical, pair_P_p, pair_A_a, pair_B_b, pair_coefficient,
                                pair_i_ao_start, pair_j_ao_start, n_pair, false, true, s)) return rc;
     const int ni = nout(i_L, spherical), nj = nout(j_L, spherical), nrow = ni * nj;
-    DevBuf charge, out;
-    EIP_CHECK(charge.alloc(sizeof(double) * 3 * (size_t) n_charge));
-    EIP_CHECK(hipMemcpyAsync(charge.p, charge_position, sizeof(double) * 3 * (size_t) n_charge, hipMemcpyHostToDevice, s));
-    EIP_CHECK(out.alloc(sizeof(double) * (size_t) d.n_sp * nrow * n_charge));
+    double *charge = nullptr, *out = nullptr;
+    EIP_MEM(mem, mem.put(&charge, std::vector<double>(charge_position, charge_position + 3 * (size_t) n_charge), s));
+    EIP_MEM(mem, mem.get(&out, (size_t) d.n_sp * nrow * n_charge));
     if (d.spec) {
-        NucArgs a{d.off.as<int>(), d.P.as<double>(), d.E.as<double>(), charge.as<double>(), d.n_sp, n_charge,
-                  out.as<double>(), n_charge, omega};
+        NucArgs a{d.off, d.P, d.E, charge, d.n_sp, n_charge, out, n_charge, omega};
         const int nchunk = (n_charge + 63) / 64;
         if (eip_launch_nuc(i_L, j_L, a, d.n_sp * nchunk, s) != 0) return 3;
     } else {
-        GenNucArgs a{i_L, j_L, spherical ? 1 : 0, d.off.as<int>(), d.P.as<double>(), d.E.as<double>(), d.n_sp,
-                     charge.as<double>(), n_charge, out.as<double>(), n_charge, omega};
+        GenNucArgs a{i_L, j_L, spherical ? 1 : 0, d.off, d.P, d.E, d.n_sp, charge, n_charge, out, n_charge, omega};
         EIP_CHECK((hipError_t) eip_gen_nuclear(a, s));
     }
     EIP_CHECK(hipGetLastError());
     std::vector<double> h((size_t) d.n_sp * nrow * n_charge);
-    EIP_CHECK(hipMemcpyAsync(h.data(), out.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    EIP_CHECK(hipMemcpyAsync(h.data(), out, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
     EIP_CHECK(hipStreamSynchronize(s));
     for (int sp = 0; sp < d.n_sp; ++sp) {
         const int I = d.g.key_a[sp], J = d.g.key_b[sp];
@@ -336,36 +403,36 @@ int two_center(int i_L, int j_L, const double* aux_A_a, const double* aux_i_coef
     if (!(i_L >= 0 && j_L >= 0 && i_L <= j_L && j_L <= kMaxAuxL) && unsupported(__func__, i_L, j_L)) return 1;
     if (n_aux_primitive_i <= 0 || n_aux_primitive_j <= 0) return 0;
     hipStream_t s = thread_stream();
+    CallMem mem;
     Grouping gi = group_by(aux_i_aux_start, nullptr, n_aux_primitive_i);
     Grouping gj = group_by(aux_j_aux_start, nullptr, n_aux_primitive_j);
     const int n_i = (int) gi.key_a.size(), n_j = (int) gj.key_a.size();
     const int ni = nout(i_L, spherical), nj = nout(j_L, spherical);
-    DevBuf iA, ic, io, jA, jc, jo, icol, jcol, out;
-    EIP_CHECK(upload(iA, gather_rows(aux_A_a, gi.order, 4), s));
-    EIP_CHECK(upload(ic, gather_rows(aux_i_coefficient, gi.order, 1), s));
-    EIP_CHECK(upload(io, gi.off, s));
-    EIP_CHECK(upload(jA, gather_rows(aux_B_b, gj.order, 4), s));
-    EIP_CHECK(upload(jc, gather_rows(aux_j_coefficient, gj.order, 1), s));
-    EIP_CHECK(upload(jo, gj.off, s));
+    double *iA, *ic, *jA, *jc, *out;
+    int *io, *jo, *icol, *jcol;
+    EIP_MEM(mem, mem.put(&iA, gather_rows(aux_A_a, gi.order, 4), s));
+    EIP_MEM(mem, mem.put(&ic, gather_rows(aux_i_coefficient, gi.order, 1), s));
+    EIP_MEM(mem, mem.put(&io, gi.off, s));
+    EIP_MEM(mem, mem.put(&jA, gather_rows(aux_B_b, gj.order, 4), s));
+    EIP_MEM(mem, mem.put(&jc, gather_rows(aux_j_coefficient, gj.order, 1), s));
+    EIP_MEM(mem, mem.put(&jo, gj.off, s));
     std::vector<int> hic(n_i), hjc(n_j);
     for (int k = 0; k < n_i; ++k) hic[k] = k * ni;
     for (int k = 0; k < n_j; ++k) hjc[k] = k * nj;
-    EIP_CHECK(upload(icol, hic, s));
-    EIP_CHECK(upload(jcol, hjc, s));
+    EIP_MEM(mem, mem.put(&icol, std::move(hic), s));
+    EIP_MEM(mem, mem.put(&jcol, std::move(hjc), s));
     const long long ld = (long long) n_j * nj;
-    EIP_CHECK(out.alloc(sizeof(double) * (size_t) n_i * ni * ld));
+    EIP_MEM(mem, mem.get(&out, (size_t) n_i * ni * ld));
     if (spherical && j_L <= EIP_SPEC_MAX_AUX_L) {
-        J2cArgs a{io.as<int>(), iA.as<double>(), ic.as<double>(), icol.as<int>(), jo.as<int>(), jA.as<double>(),
-                  jc.as<double>(), jcol.as<int>(), n_i, n_j, 0, 0, out.as<double>(), ld, omega};
+        J2cArgs a{io, iA, ic, icol, jo, jA, jc, jcol, n_i, n_j, 0, 0, out, ld, omega};
         if (eip_launch_j2c(i_L, j_L, a, n_i * ((n_j + 63) / 64), s) != 0) return 3;
     } else {
-        GenJ2cArgs a{i_L, j_L, spherical ? 1 : 0, io.as<int>(), iA.as<double>(), ic.as<double>(), n_i,
-                     jo.as<int>(), jA.as<double>(), jc.as<double>(), n_j, out.as<double>(), ld, omega};
+        GenJ2cArgs a{i_L, j_L, spherical ? 1 : 0, io, iA, ic, n_i, jo, jA, jc, n_j, out, ld, omega};
         EIP_CHECK((hipError_t) eip_gen_two_center(a, s));
     }
     EIP_CHECK(hipGetLastError());
     std::vector<double> h((size_t) n_i * ni * ld);
-    EIP_CHECK(hipMemcpyAsync(h.data(), out.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    EIP_CHECK(hipMemcpyAsync(h.data(), out, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
     EIP_CHECK(hipStreamSynchronize(s));
     for (int si = 0; si < n_i; ++si)
         for (int sj = 0; sj < n_j; ++sj) {
@@ -381,43 +448,64 @@ int two_center(int i_L, int j_L, const double* aux_A_a, const double* aux_i_coef
     return 0;
 }
 
-int three_center(int i_L, int j_L, int k_L, const double* pair_P_p, const double* pair_A_a, const double* pair_B_b,
-                 const double* pair_coefficient, const int* pair_i_ao_start, const int* pair_j_ao_start, int n_pair_ij,
-                 const double* aux_A_a, const double* aux_coefficient, const int* aux_i_aux_start,
-                 int n_aux_primitive_k, double* J3c_matrix, int n_ao, int n_aux, bool spherical, double omega)
+}  // extern "C"
+
+namespace {
+
+// Compact output rows kept per batch of shell pairs when device memory is allocated per call.
+constexpr long long kBlockBudgetBytes = 256LL << 20;
+
+// The three-centre C-ABI call over the given call memory and stream (shared by three_center and its
+// no-allocation variant; the kernels and their order are the same, so the results are identical).
+int three_center_impl(CallMem& mem, hipStream_t s, int i_L, int j_L, int k_L, const double* pair_P_p,
+                      const double* pair_A_a, const double* pair_B_b, const double* pair_coefficient,
+                      const int* pair_i_ao_start, const int* pair_j_ao_start, int n_pair_ij, const double* aux_A_a,
+                      const double* aux_coefficient, const int* aux_i_aux_start, int n_aux_primitive_k,
+                      double* J3c_matrix, int n_ao, bool spherical, double omega)
 {
-    if (!(i_L >= 0 && j_L >= 0 && i_L <= j_L && j_L <= kMaxL && k_L >= 0 && k_L <= kMaxAuxL)
-        && unsupported(__func__, i_L, j_L, k_L)) return 1;
-    (void) n_aux;
-    if (n_pair_ij <= 0 || n_aux_primitive_k <= 0) return 0;
-    hipStream_t s = thread_stream();
     DevPairs d;
-    if (int rc = prepare_pairs(d, i_L, j_L, spherical, pair_P_p, pair_A_a, pair_B_b, pair_coefficient,
+    if (int rc = prepare_pairs(d, mem, i_L, j_L, spherical, pair_P_p, pair_A_a, pair_B_b, pair_coefficient,
                                pair_i_ao_start, pair_j_ao_start, n_pair_ij, false,
                                k_L <= EIP_SPEC_MAX_AUX_L, s)) return rc;
     Grouping gk = group_by(aux_i_aux_start, nullptr, n_aux_primitive_k);
     const int n_ks = (int) gk.key_a.size();
     const int ni = nout(i_L, spherical), nj = nout(j_L, spherical), nk = nout(k_L, spherical), nrow = ni * nj;
-    DevBuf kC, kc, ko, kcol;
-    EIP_CHECK(upload(kC, gather_rows(aux_A_a, gk.order, 4), s));
-    EIP_CHECK(upload(kc, gather_rows(aux_coefficient, gk.order, 1), s));
-    EIP_CHECK(upload(ko, gk.off, s));
+    double *kC, *kc;
+    int *ko, *kcol;
+    EIP_MEM(mem, mem.put(&kC, gather_rows(aux_A_a, gk.order, 4), s));
+    EIP_MEM(mem, mem.put(&kc, gather_rows(aux_coefficient, gk.order, 1), s));
+    EIP_MEM(mem, mem.put(&ko, gk.off, s));
     std::vector<int> hkcol(n_ks);
     for (int k = 0; k < n_ks; ++k) hkcol[k] = k * nk;
-    EIP_CHECK(upload(kcol, hkcol, s));
-    const long long ld = (long long) n_ks * nk;
-    // Bound the compact buffer: process shell pairs in batches of at most ~256 MB of output.
-    const long long per_sp = (long long) nrow * ld;
-    const int batch = (int) std::max(1LL, std::min<long long>(d.n_sp, (256LL << 20) / 8 / std::max(1LL, per_sp)));
+    EIP_MEM(mem, mem.put(&kcol, std::move(hkcol), s));
     const bool spec = d.spec;
-    DevBuf out, items, spi, spj;
-    EIP_CHECK(out.alloc(sizeof(double) * (size_t) batch * per_sp));
-    std::vector<double> h((size_t) batch * per_sp);
+    int *spi = nullptr, *spj = nullptr;
+    const double* E_kernel = d.E;
     if (spec) {
-        EIP_CHECK(upload(spi, d.g.key_a, s));
-        EIP_CHECK(upload(spj, d.g.key_b, s));
+        EIP_MEM(mem, mem.put(&spi, d.g.key_a, s));
+        EIP_MEM(mem, mem.put(&spj, d.g.key_b, s));
+        if (eip_j3c_variant_mfma(i_L, j_L, k_L, current_variant(i_L, j_L, k_L, 0)) == 1) {
+            // the class's float64 kernels run on the matrix cores and read the dense pair matrix
+            double* ED = nullptr;
+            EIP_MEM(mem, mem.get(&ED, (size_t) d.n_pp * eip_pair_dense_width(i_L, j_L)));
+            if (eip_launch_pairD(i_L, j_L, d.E, ED, d.n_pp, s) != 0) return 3;
+            E_kernel = ED;
+        }
     }
+    const long long ld = (long long) n_ks * nk;
     const int kblocks = (n_ks + 63) / 64;
+    // Batches of shell pairs: compact output rows (and work items) of at most kBlockBudgetBytes, or --
+    // over a fixed region -- of what the region has left after the inputs.
+    const long long per_sp = (long long) nrow * ld;
+    const long long per_sp_bytes = per_sp * 8 + (spec ? 8LL * kblocks : 0);
+    const long long room = mem.fixed() ? (long long) mem.left() - 1024 : kBlockBudgetBytes;
+    const int batch = (int) std::min<long long>(d.n_sp, std::max(mem.fixed() ? 0LL : 1LL, room / std::max(1LL, per_sp_bytes)));
+    if (batch < 1) EIP_MEM(mem, hipErrorOutOfMemory);
+    double* out = nullptr;
+    int* items = nullptr;
+    EIP_MEM(mem, mem.get(&out, (size_t) batch * per_sp));
+    if (spec) EIP_MEM(mem, mem.get(&items, (size_t) batch * kblocks * 2));
+    std::vector<double> h((size_t) batch * per_sp);
     for (int sp0 = 0; sp0 < d.n_sp; sp0 += batch) {
         const int nb = std::min(batch, d.n_sp - sp0);
         if (spec) {
@@ -426,27 +514,26 @@ int three_center(int i_L, int j_L, int k_L, const double* pair_P_p, const double
             hit.reserve((size_t) nb * kblocks * 2);
             for (int sp = sp0; sp < sp0 + nb; ++sp)
                 for (int kb = 0; kb < kblocks; ++kb) { hit.push_back(sp); hit.push_back(kb * 64); }
-            DevBuf it;
-            EIP_CHECK(upload(it, hit, s));
+            EIP_CHECK(hipMemcpyAsync(items, hit.data(), hit.size() * sizeof(int), hipMemcpyHostToDevice, s));
             // out pointer shifted so that row (sp * nrow + ij) lands at (sp - sp0) * nrow + ij
             J3cArgs a{};
-            a.items = it.as<int>();
+            a.items = items;
             a.items_ks_end = nullptr;
             a.item_begin = 0;
             a.item_end = nb * kblocks;
-            a.d.sp_pp_off = d.off.as<int>();
-            a.d.sp_i = spi.as<int>();
-            a.d.sp_j = spj.as<int>();
-            a.d.pp_P = d.P.as<double>();
-            a.d.E = d.E.as<double>();
+            a.d.sp_pp_off = d.off;
+            a.d.sp_i = spi;
+            a.d.sp_j = spj;
+            a.d.pp_P = d.P;
+            a.d.E = E_kernel;
             a.d.E32 = nullptr;
             a.d.ks_perm = nullptr;
-            a.d.ks_prim_off = ko.as<int>();
-            a.d.ks_col = kcol.as<int>();
-            a.d.ka_C = kC.as<double>();
-            a.d.ka_coef = kc.as<double>();
+            a.d.ks_prim_off = ko;
+            a.d.ks_col = kcol;
+            a.d.ka_C = kC;
+            a.d.ka_coef = kc;
             a.d.n_ks = n_ks;
-            a.d.out = out.as<double>() - (long long) sp0 * nrow * ld;
+            a.d.out = out - (long long) sp0 * nrow * ld;
             a.d.ld = ld;
             a.layout = 1;
             a.omega = omega;
@@ -455,13 +542,12 @@ int three_center(int i_L, int j_L, int k_L, const double* pair_P_p, const double
             for (int c = 0; c < nchunks; ++c)
                 if (eip_launch_j3c(i_L, j_L, k_L, v, c, 0, a, nb * kblocks, s) != 0) return 3;
             EIP_CHECK(hipGetLastError());
-            EIP_CHECK(hipStreamSynchronize(s));        // 'it' is released at scope exit
+            EIP_CHECK(hipStreamSynchronize(s));        // 'hit' is read by the copy above until here
         } else {
-            GenJ3cArgs a{i_L, j_L, k_L, spherical ? 1 : 0, d.off.as<int>() + sp0, d.P.as<double>(), d.E.as<double>(), nb,
-                         ko.as<int>(), kC.as<double>(), kc.as<double>(), n_ks, out.as<double>(), ld, omega};
+            GenJ3cArgs a{i_L, j_L, k_L, spherical ? 1 : 0, d.off + sp0, d.P, d.E, nb, ko, kC, kc, n_ks, out, ld, omega};
             EIP_CHECK((hipError_t) eip_gen_three_center(a, s));
         }
-        EIP_CHECK(hipMemcpyAsync(h.data(), out.p, sizeof(double) * (size_t) nb * per_sp, hipMemcpyDeviceToHost, s));
+        EIP_CHECK(hipMemcpyAsync(h.data(), out, sizeof(double) * (size_t) nb * per_sp, hipMemcpyDeviceToHost, s));
         EIP_CHECK(hipStreamSynchronize(s));
         // Scatter-add: threads own disjoint auxiliary shells, hence disjoint rows of J3c.
         const size_t nao2 = (size_t) n_ao * n_ao;
@@ -484,6 +570,25 @@ int three_center(int i_L, int j_L, int k_L, const double* pair_P_p, const double
         });
     }
     return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int three_center(int i_L, int j_L, int k_L, const double* pair_P_p, const double* pair_A_a, const double* pair_B_b,
+                 const double* pair_coefficient, const int* pair_i_ao_start, const int* pair_j_ao_start, int n_pair_ij,
+                 const double* aux_A_a, const double* aux_coefficient, const int* aux_i_aux_start,
+                 int n_aux_primitive_k, double* J3c_matrix, int n_ao, int n_aux, bool spherical, double omega)
+{
+    if (!(i_L >= 0 && j_L >= 0 && i_L <= j_L && j_L <= kMaxL && k_L >= 0 && k_L <= kMaxAuxL)
+        && unsupported(__func__, i_L, j_L, k_L)) return 1;
+    (void) n_aux;
+    if (n_pair_ij <= 0 || n_aux_primitive_k <= 0) return 0;
+    CallMem mem;
+    return three_center_impl(mem, thread_stream(), i_L, j_L, k_L, pair_P_p, pair_A_a, pair_B_b, pair_coefficient,
+                             pair_i_ao_start, pair_j_ao_start, n_pair_ij, aux_A_a, aux_coefficient, aux_i_aux_start,
+                             n_aux_primitive_k, J3c_matrix, n_ao, spherical, omega);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -520,8 +625,33 @@ int eip_three_center_chunk(int i_L, int j_L, int k_L, int variant, int chunk, in
 int eip_set_three_center_variant(int i_L, int j_L, int k_L, int use_f32, int variant)
 {
     if (!spec_3c(i_L, j_L, k_L) || variant < 0 || variant >= eip_j3c_nvariants(i_L, j_L, k_L)) return 1;
+    if (use_f32 && eip_j3c_variant_mfma(i_L, j_L, k_L, variant) == 1) return 1;   // float64-only variant
     g_variant[use_f32 ? 1 : 0][i_L][j_L][k_L] = variant;
     return 0;
+}
+
+int eip_three_center_variant_mfma(int i_L, int j_L, int k_L, int variant)
+{
+    return spec_3c(i_L, j_L, k_L) ? eip_j3c_variant_mfma(i_L, j_L, k_L, variant) : -1;
+}
+
+int eip_three_center_uses_mfma(int i_L, int j_L, int k_L)
+{
+    return spec_3c(i_L, j_L, k_L) ? (eip_j3c_variant_mfma(i_L, j_L, k_L, current_variant(i_L, j_L, k_L, 0)) == 1) : 0;
+}
+
+int eip_pair_matrix_dense_width(int i_L, int j_L)
+{
+    if (!(i_L >= 0 && j_L >= 0 && i_L <= j_L && j_L <= EIP_SPEC_MAX_L)) return -1;
+    return eip_pair_dense_width(i_L, j_L);
+}
+
+int eip_pair_matrix_dense(int i_L, int j_L, const double* d_E, int n_pp, double* d_ED, void* stream)
+{
+    if (eip_pair_matrix_dense_width(i_L, j_L) < 0) return 1;
+    if (n_pp <= 0) return 0;
+    if (eip_launch_pairD(i_L, j_L, d_E, d_ED, n_pp, (hipStream_t) stream) != 0) return 3;
+    return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
 int eip_pair_matrix(int i_L, int j_L, const double* d_P_p, const double* d_A_a, const double* d_B_b,

@@ -15,6 +15,9 @@
 
 namespace eip {
 
+// Four doubles: the accumulator tile slice one lane holds for v_mfma_f64_16x16x4f64.
+typedef double v4d __attribute__((ext_vector_type(4)));
+
 constexpr double kTwoPi25 = 34.986836655249725;     // 2 pi^(5/2)
 constexpr double kTwoPi = 6.283185307179586;        // 2 pi
 constexpr double kPi = 3.141592653589793;

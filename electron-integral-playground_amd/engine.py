@@ -163,6 +163,21 @@ class AuxClass:
     dev: Dict[str, object] = field(default_factory=dict)
 
 
+def _dense_pair_matrix(lib, la, lb, E, device):
+    """The dense [n_pp][n_t][16 * blocks] float64 pair matrix from the packed one (eip_pair_matrix_dense),
+    on the current stream, which is then synchronised (launches may go to other streams)."""
+    import torch
+    w = lib.eip_pair_matrix_dense_width(la, lb)
+    if w < 0:
+        raise ValueError(f"pair class ({la},{lb}) has no matrix-core kernels")
+    D = torch.empty((E.shape[0], w), dtype=torch.float64, device=device)
+    rc = lib.eip_pair_matrix_dense(la, lb, _ptr(E), int(E.shape[0]), _ptr(D), _stream_handle(None))
+    if rc != 0:
+        raise RuntimeError(f"eip_pair_matrix_dense({la},{lb}) failed with {rc}")
+    torch.cuda.current_stream(device).synchronize()
+    return D
+
+
 # Primitive pairs with p = a + b below this exponent are "diffuse": the mixed-precision split always
 # evaluates them in float64.
 DIFFUSE_PAIR_EXPONENT = 0.2
@@ -370,7 +385,30 @@ class ThreeCenterPlan:
                 self.items[(pi, ai)] = torch.stack([sp, k0], 1)
         self._zero = None
         self._job_tables = {}
+        self._dense = {}
+        # dense pair matrices of the classes whose float64 kernels run on the matrix cores, built here
+        # (on the caller's stream) rather than at a first launch
+        for pi in range(len(self.pairs)):
+            for ai in range(len(self.aux)):
+                self.kernel_E(pi, ai)
         torch.cuda.current_stream().synchronize()
+
+    def dense_E(self, pi):
+        """Dense float64 pair matrix of pair class pi (the matrix-core kernels' layout), built on first
+        use and kept."""
+        d = self._dense.get(pi)
+        if d is None:
+            d = _dense_pair_matrix(self.lib, self.pairs[pi].la, self.pairs[pi].lb, self.pairs[pi].dev["E"], self.device)
+            self._dense[pi] = d
+        return d
+
+    def kernel_E(self, pi, ai):
+        """The float64 pair matrix the current kernels of class (pi, ai) read: dense when that class runs
+        on the matrix cores, packed otherwise."""
+        pc, ac = self.pairs[pi], self.aux[ai]
+        if self.lib.eip_three_center_uses_mfma(pc.la, pc.lb, ac.lc) == 1:
+            return self.dense_E(pi)
+        return pc.dev["E"]
 
     def _zero_tables(self):
         """(rows, spans) of the packed rows no kernel writes (shell pairs screened out entirely), which
@@ -422,7 +460,7 @@ class ThreeCenterPlan:
             if isinstance(v, (list, tuple)):
                 return sum(size(x) for x in v)
             return 0
-        total = sum(size(pc.dev) for pc in self.pairs) + sum(size(ac.dev) for ac in self.aux)
+        total = sum(size(pc.dev) for pc in self.pairs) + sum(size(ac.dev) for ac in self.aux) + size(self._dense)
         total += size(self.items) + size(self.items_lpt) + size(getattr(self, "mixed", {})) + size(self._zero or ())
         return int(total)
 
@@ -454,9 +492,10 @@ class ThreeCenterPlan:
         pc, ac = self.pairs[pi], self.aux[ai]
         it = self.class_items(pi, ai)
         dp, da = pc.dev, ac.dev
+        E = dp["E"] if use_f32 else self.kernel_E(pi, ai)
         rc = self.lib.eip_three_center_items(
             pc.la, pc.lb, ac.lc, int(use_f32), _ptr(it), _vp(0), it.shape[0], _ptr(dp["off"]), _ptr(dp["si"]),
-            _ptr(dp["sj"]), _ptr(dp["P"]), _ptr(dp["E"]), _ptr(dp["E32"]), _vp(0), _ptr(da["off"]),
+            _ptr(dp["sj"]), _ptr(dp["P"]), _ptr(E), _ptr(dp["E32"]), _vp(0), _ptr(da["off"]),
             _ptr(da["col"]), _ptr(da["C"]), _ptr(da["c"]), ac.n_ks, _ptr(out), _c_ll(self.n_aux), 0,
             _c_double(self.omega), _stream_handle(stream))
         if rc != 0:
@@ -533,8 +572,10 @@ class ThreeCenterPlan:
     def job_words(self, pi, ai, out_ptr, ld, perm=None):
         """The 16 words of one eip_j3c_job entry: pair class pi, auxiliary class ai, output."""
         dp, da = self.pairs[pi].dev, self.aux[ai].dev
+        # E: what the float64 kernels of the class read (dense for matrix-core variants); the float32
+        # kernels read E32
         return [dp["off"].data_ptr(), dp["si"].data_ptr(), dp["sj"].data_ptr(), dp["P"].data_ptr(),
-                dp["E"].data_ptr(), dp["E32"].data_ptr(), perm.data_ptr() if perm is not None else 0,
+                self.kernel_E(pi, ai).data_ptr(), dp["E32"].data_ptr(), perm.data_ptr() if perm is not None else 0,
                 da["off"].data_ptr(), da["col"].data_ptr(), da["C"].data_ptr(), da["c"].data_ptr(),
                 self.aux[ai].n_ks, int(out_ptr), int(ld), 0, 0]
 
@@ -685,9 +726,10 @@ class ThreeCenterPlan:
         pc = self.pairs[pi]
         return (sp1 - sp0) * (2 * pc.la + 1) * (2 * pc.lb + 1)
 
-    def run_block(self, pi, sp0, sp1, out, mixed=False, stream=None):
+    def run_block(self, pi, sp0, sp1, out, mixed=False, stream=None, aux_classes=None):
         """(ij|P) of shell pairs [sp0, sp1) of class pi, all auxiliary shells, into the compact block
-        out[(sp - sp0) * nij + ij, P] (every row written; no zeroing needed)."""
+        out[(sp - sp0) * nij + ij, P] (every row written; no zeroing needed).  ``aux_classes`` (indices
+        into self.aux) restricts the launches to those auxiliary classes' columns."""
         pc = self.pairs[pi]
         nij = (2 * pc.la + 1) * (2 * pc.lb + 1)
         base = out.data_ptr() - 8 * sp0 * nij * self.n_aux
@@ -695,6 +737,8 @@ class ThreeCenterPlan:
                    for ai in range(len(self.aux))]
         jobs = self.job_table(entries)
         for ai, ac in enumerate(self.aux):
+            if aux_classes is not None and ai not in aux_classes:
+                continue
             if not mixed:
                 self.launch_work(pi, ai, self._block_items(pi, ai, sp0, sp1)[0], jobs[ai], layout=1, stream=stream)
                 continue
@@ -895,6 +939,22 @@ class BatchPlan:
         self.out2c_base = np.concatenate([[0], np.cumsum(n2)]).astype(np.int64)
         torch.cuda.current_stream().synchronize()
 
+    def kernel_E(self, pi, ai):
+        """Float64 pair matrix of the current kernels of class (pi, ai): dense (built on first use, which
+        must happen outside graph capture -- BatchPlan.prepare does it) for matrix-core variants."""
+        (la, lb), _, _, dp = self.pairs[pi]
+        if self.lib.eip_three_center_uses_mfma(la, lb, self.aux[ai][0]) == 1:
+            if "ED" not in dp:
+                dp["ED"] = _dense_pair_matrix(self.lib, la, lb, dp["E"], self.device)
+            return dp["ED"]
+        return dp["E"]
+
+    def prepare(self):
+        """Build every table the launches read (dense pair matrices of matrix-core classes) ahead of a
+        captured sequence."""
+        for pi, ai in self.items:
+            self.kernel_E(pi, ai)
+
     def n_integrals(self):
         return int(sum(n_aux * n_ao * (n_ao + 1) // 2 for n_ao, n_aux in self.sizes))
 
@@ -945,9 +1005,10 @@ class BatchPlan:
             for (pi, ai), (it, end) in self.items.items():
                 (la, lb), _, _, dp = self.pairs[pi]
                 lc, n_ks, da = self.aux[ai]
+                E = self.kernel_E(pi, ai)
                 rc = self.lib.eip_three_center_items_batched(
                     la, lb, lc, 0, _ptr(it), _ptr(end), it.shape[0], _ptr(dp["off"]), _ptr(dp["si"]), _ptr(dp["sj"]),
-                    _ptr(dp["base"]), _ptr(dp["ld"]), _ptr(dp["P"]), _ptr(dp["E"]), _ptr(dp["E32"]), _vp(0),
+                    _ptr(dp["base"]), _ptr(dp["ld"]), _ptr(dp["P"]), _ptr(E), _ptr(dp["E32"]), _vp(0),
                     _ptr(da["off"]), _ptr(da["col"]), _ptr(da["C"]), _ptr(da["c"]), n_ks, _ptr(out),
                     _c_double(self.omega), _stream_handle(stream))
                 if rc != 0:
@@ -970,18 +1031,22 @@ RESOURCES_FILE = _pathlib.Path(__file__).resolve().parent / "lib" / "kernel_reso
 def _variant_kernels(lib, la, lb, lc, v, f32):
     m0, m1 = ctypes.c_int(), ctypes.c_int()
     names, c = [], 0
+    prefix = "j3cm" if lib.eip_three_center_variant_mfma(la, lb, lc, v) == 1 else "j3c"
     while lib.eip_three_center_chunk(la, lb, lc, v, c, ctypes.byref(m0), ctypes.byref(m1)) == 0:
-        names.append(f"j3c_{la}{lb}{lc}_{m0.value}_{m1.value}" + ("_f32" if f32 else ""))
+        names.append(f"{prefix}_{la}{lb}{lc}_{m0.value}_{m1.value}" + ("_f32" if f32 else ""))
         c += 1
     return names
 
 
 def usable_variants(lib, la, lb, lc, f32=False):
-    """Variants of a class none of whose kernels spill to scratch memory (variant 0 is always kept)."""
+    """Variants of a class none of whose kernels spill to scratch memory (variant 0 is always kept);
+    matrix-core variants are float64 only."""
     from .build import spills
     res = _json.loads(RESOURCES_FILE.read_text()) if RESOURCES_FILE.exists() else {}
     out = []
     for v in range(max(0, lib.eip_three_center_nvariants(la, lb, lc))):
+        if f32 and lib.eip_three_center_variant_mfma(la, lb, lc, v) == 1:
+            continue
         names = _variant_kernels(lib, la, lb, lc, v, f32)
         if v == 0 or all(n in res and not spills(res[n]) for n in names):
             out.append(v)

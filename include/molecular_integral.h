@@ -81,6 +81,22 @@ int eip_three_center_nvariants(int i_L, int j_L, int k_L);
 int eip_three_center_chunk(int i_L, int j_L, int k_L, int variant, int chunk, int* m0, int* m1);
 int eip_set_three_center_variant(int i_L, int j_L, int k_L, int use_f32, int variant);
 
+/* Variants after the vector ones run the float64 contraction on the matrix cores
+ * (v_mfma_f64_16x16x4f64: four primitive pairs x 16 auxiliary shells per instruction, one workgroup of
+ * four waves per work item) and read the dense pair matrix below instead of the packed one.
+ * eip_three_center_variant_mfma: 1 for such a variant, 0 for a vector variant, -1 for no such class;
+ * eip_three_center_uses_mfma: 1 when the class's current float64 variant is one (then every E pointer
+ * given for that class -- argument or eip_j3c_job -- must be the dense matrix).  Float32 variants never
+ * are: setting one for use_f32 = 1 returns 1. */
+int eip_three_center_variant_mfma(int i_L, int j_L, int k_L, int variant);
+int eip_three_center_uses_mfma(int i_L, int j_L, int k_L);
+
+/* Doubles per primitive pair of the dense pair matrix [n_pp][n_t][16 * ceil(nij / 16)] of class
+ * (i_L, j_L) (structural zeros stored, -1 when the class has no matrix-core kernels), and its
+ * construction from the packed float64 matrix d_E. */
+int eip_pair_matrix_dense_width(int i_L, int j_L);
+int eip_pair_matrix_dense(int i_L, int j_L, const double* d_E, int n_pp, double* d_ED, void* stream);
+
 /* Pair matrix (Hermite expansion contracted into real solid harmonics, pair coefficient folded in)
  * for n_pp primitive pairs: d_E[n_pp][nnz] float64 and, when d_E32 is not null, a float32 copy. */
 int eip_pair_matrix(int i_L, int j_L, const double* d_P_p, const double* d_A_a, const double* d_B_b,

@@ -162,3 +162,38 @@ def assert_parity(got, ref, what, atol=ATOL, rtol=RTOL):
         raise AssertionError(f"{what}: {int(bad.sum())} of {bad.size} elements outside {atol:g} + {rtol:g}|ref|; "
                              f"worst at flat index {k}: got {got.flat[k]!r}, oracle {ref.flat[k]!r}")
     return float(err.max()) if err.size else 0.0
+
+
+ORACLE_LD_SO = ROOT / "oracle" / "build" / "libmd_oracle_ld.so"
+_oracle_ld = None
+
+
+def oracle_ld_lib():
+    """The oracle built with long-double internal arithmetic (decides ill-conditioned elements)."""
+    global _oracle_ld
+    if _oracle_ld is None:
+        _oracle_ld = ctypes.CDLL(str(ORACLE_LD_SO))
+    return _oracle_ld
+
+
+def oracle_shell_pair_block(lib, pd, rows, la, lb, pk, k_L, start, omega=0.0, fn="oracle_three_center"):
+    """(2 k_L + 1, 2 la + 1, 2 lb + 1) oracle block of ONE contracted shell pair -- the primitive pairs
+    ``rows`` of the class's pair data ``pd`` (all with the same AO offsets) -- against ONE auxiliary shell
+    (primitives of ``pk`` starting at ``start``), on a rebased AO range so the output stays small."""
+    rows = np.asarray(rows)
+    I, J = int(pd.i_ao_start[rows[0]]), int(pd.j_ao_start[rows[0]])
+    ni, nj = 2 * la + 1, 2 * lb + 1
+    i0, j0 = 0, (0 if I == J else ni)
+    n = ni if I == J else ni + nj
+    take = lambda a: np.ascontiguousarray(a[rows])
+    P, A, B, c = take(pd.P_p), take(pd.A_a), take(pd.B_b), take(pd.coefficient)
+    si, sj = np.full(len(rows), i0, np.int32), np.full(len(rows), j0, np.int32)
+    sel = pk.i_ao_start == start
+    K, kc = np.ascontiguousarray(pk.A_a[sel]), np.ascontiguousarray(pk.coefficient[sel])
+    ks = np.zeros(int(sel.sum()), dtype=np.int32)
+    nk = 2 * k_L + 1
+    out = np.zeros((nk, n, n))
+    rc = getattr(lib, fn)(la, lb, k_L, dp(P), dp(A), dp(B), dp(c), ip(si), ip(sj), len(rows), dp(K), dp(kc), ip(ks),
+                          K.shape[0], dp(out), n, nk, ctypes.c_bool(True), ctypes.c_double(omega))
+    assert rc == 0
+    return out[:, i0:i0 + ni, j0:j0 + nj]

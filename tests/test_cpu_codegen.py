@@ -64,3 +64,68 @@ def test_choose_stage_needs_a_clear_win(tmp_path):
     table = choose_stage.choose([str(lds)], [str(dpp)])
     # 1.4 < 0.98 * 1.5: dpp; 0.99 is not 2 % faster than 1.0: lds; (ff) rows never leave LDS
     assert table == {"011_f64": "lds", "123_f64": "dpp", "333_f64": "lds"}
+
+
+def _eval_generated(lines, env):
+    """Evaluate generated ``const double name = expr;`` lines in exact rational arithmetic."""
+    from fractions import Fraction
+    for line in lines:
+        m = re.match(r"\s*const double (\w+) = (.*);", line)
+        expr = re.sub(r"(\d+\.\d+)", lambda mm: f"Fraction('{mm.group(1)}')", m.group(2))
+        env[m.group(1)] = eval(expr, {"Fraction": Fraction}, env)
+    return env
+
+
+def test_pair_hermite_recurrence_equals_reference_transfer_exactly():
+    """The generated two-index recurrence gives, in exact arithmetic, the coefficients of the
+    reference's vertical recurrence plus binomial transfer E^{ij}_t = sum_m C(j, m) AB^{j-m} E^{i+m,0}_t
+    (mcmurchie_davidson_E_term.hpp:22-45, 166-199) for every (i, j, t) up to l = 3."""
+    from fractions import Fraction as F
+    from math import comb
+    import random
+    rng = random.Random(5)
+    for la in range(4):
+        for lb in range(4):
+            PA, AB, o2p = F(rng.randint(-60, 60), 7), F(rng.randint(-60, 60), 3), F(1, rng.randint(1, 50))
+            lines = []
+            gen_kernels._emit_E_axis(lines, "x", la, lb)
+            env = _eval_generated(lines, {"PAx": PA, "PBx": PA + AB, "o2p": o2p})
+            E0 = {(0, 0): F(1)}
+            for i in range(la + lb):
+                for t in range(i + 2):
+                    E0[(i + 1, t)] = ((o2p * E0[(i, t - 1)] if t >= 1 else 0) + (PA * E0[(i, t)] if t <= i else 0)
+                                      + ((t + 1) * E0[(i, t + 1)] if t + 1 <= i else 0))
+            for i in range(la + 1):
+                for j in range(lb + 1):
+                    for t in range(i + j + 1):
+                        ref = sum((comb(j, j - m) * AB ** (j - m) * E0[(i + m, t)] for m in range(j + 1) if t <= i + m),
+                                  F(0))
+                        assert env[f"Ex_{i}_{j}_{t}"] == ref, (la, lb, i, j, t)
+
+
+def test_pair_hermite_recurrence_is_stable_for_a_tight_second_centre():
+    """Diffuse A (a = 0.087), tight B (b = 4790), |AB| = 5.3 Bohr, (d f): in float64 the transfer
+    formula loses most digits of E^{2,3}_0 while the generated recurrence keeps it to 1e-12 relative."""
+    from fractions import Fraction as F
+    from math import comb
+    a, b, AB = 0.0866, 4790.0, 5.28
+    p = a + b
+    PA, PB, o2p = b / p * AB, -a / p * AB, 0.5 / p         # P - A = b/p (B - A) up to sign convention
+    PA, PB = -PA, -PB
+    lines = []
+    gen_kernels._emit_E_axis(lines, "x", 2, 3)
+    exact = _eval_generated(lines, {"PAx": F(PA), "PBx": F(PB), "o2p": F(o2p)})["Ex_2_3_0"]
+    env = {"PAx": PA, "PBx": PB, "o2p": o2p}
+    for line in lines:
+        m = re.match(r"\s*const double (\w+) = (.*);", line)
+        env[m.group(1)] = eval(m.group(2), {}, env)
+    rec = env["Ex_2_3_0"]
+    E0 = {(0, 0): 1.0}
+    for i in range(5):
+        for t in range(i + 2):
+            E0[(i + 1, t)] = ((o2p * E0[(i, t - 1)] if t >= 1 else 0.0) + (PA * E0[(i, t)] if t <= i else 0.0)
+                              + ((t + 1) * E0[(i, t + 1)] if t + 1 <= i else 0.0))
+    ab = PB - PA                                         # A - B
+    hrr = sum(comb(3, 3 - m) * ab ** (3 - m) * E0[(2 + m, 0)] for m in range(4))
+    assert abs(rec - float(exact)) <= 1e-12 * abs(float(exact))
+    assert abs(hrr - float(exact)) > 1e3 * abs(rec - float(exact))

@@ -234,3 +234,66 @@ def test_zeroing_writes_exactly_the_screened_rows(max_blocks):
     assert bool((out[mask] == 0).all().item())
     assert bool((out[~mask] == 1.0).all().item())
 
+
+
+def test_matrix_core_variants_match_vector_kernels():
+    """Every float64 matrix-core variant (v_mfma_f64_16x16x4f64 kernels over the dense pair matrix) gives
+    the vector kernels' (ij|P) to 1e-12 + 1e-10 |x| on every class that has one, through the packed
+    whole-class launch and the compact block launch, and the drop-in entry point matches the oracle."""
+    import torch
+    from electron_integral_playground_amd.engine import ThreeCenterPlan, apply_tuning
+    from electron_integral_playground_amd.molecular_integral import three_center
+    mol = random_molecule(n_atom=4, seed=23, orbital_l=(0, 1, 2, 3), aux_l=(0, 1, 2, 3, 4), nprim=(1, 4))
+    pairs = pair_utility.form_pair_data(mol, 1e-14)
+    aux = pair_utility.form_auxiliary_pair_data(mol)
+    plan = ThreeCenterPlan(mol, pairs, aux)
+    lib = plan.lib
+    classes = [(pi, ai) for pi, pc in enumerate(plan.pairs) for ai, ac in enumerate(plan.aux)
+               if any(lib.eip_three_center_variant_mfma(pc.la, pc.lb, ac.lc, v) == 1
+                      for v in range(lib.eip_three_center_nvariants(pc.la, pc.lb, ac.lc)))]
+    assert len(classes) >= 20
+    try:
+        ref = plan.new_output()
+        for pi, ai in classes:
+            pc, ac = plan.pairs[pi], plan.aux[ai]
+            lib.eip_set_three_center_variant(pc.la, pc.lb, ac.lc, 0, 0)
+            plan.launch_class(pi, ai, ref)
+        torch.cuda.synchronize()
+        checked = 0
+        for pi, ai in classes:
+            pc, ac = plan.pairs[pi], plan.aux[ai]
+            nij = (2 * pc.la + 1) * (2 * pc.lb + 1)
+            for v in range(lib.eip_three_center_nvariants(pc.la, pc.lb, ac.lc)):
+                if lib.eip_three_center_variant_mfma(pc.la, pc.lb, ac.lc, v) != 1:
+                    continue
+                print(f"case class ({pc.la}{pc.lb}|{ac.lc}) matrix-core variant {v}", flush=True)
+                assert lib.eip_set_three_center_variant(pc.la, pc.lb, ac.lc, 0, v) == 0
+                assert lib.eip_set_three_center_variant(pc.la, pc.lb, ac.lc, 1, v) == 1   # float64 only
+                assert lib.eip_three_center_uses_mfma(pc.la, pc.lb, ac.lc) == 1
+                got = torch.full_like(ref, float("nan"))
+                plan.launch_class(pi, ai, got)
+                blk = torch.full((pc.n_sp * nij, mol.n_aux), float("nan"), dtype=torch.float64, device="cuda")
+                plan.run_block(pi, 0, pc.n_sp, blk, aux_classes=[ai])
+                torch.cuda.synchronize()
+                cols = torch.as_tensor((ac.ks_start[:, None] + np.arange(2 * ac.lc + 1)[None, :]).ravel(), device="cuda")
+                sp = np.arange(pc.n_sp)
+                I = (pc.sp_i[sp][:, None, None] + np.arange(2 * pc.la + 1)[None, :, None]).astype(np.int64)
+                J = (pc.sp_j[sp][:, None, None] + np.arange(2 * pc.lb + 1)[None, None, :]).astype(np.int64)
+                hi, lo = np.maximum(I, J), np.minimum(I, J)
+                packed = torch.as_tensor((hi * (hi + 1) // 2 + lo).ravel(), device="cuda")
+                r = ref[packed].index_select(1, cols).cpu().numpy()
+                np.testing.assert_allclose(got[packed].index_select(1, cols).cpu().numpy(), r, rtol=RTOL, atol=ATOL)
+                np.testing.assert_allclose(blk.index_select(1, cols).cpu().numpy(), r, rtol=RTOL, atol=ATOL)
+                checked += 1
+            lib.eip_set_three_center_variant(pc.la, pc.lb, ac.lc, 0, 0)
+        assert checked >= len(classes)
+        # the drop-in ABI with every matrix-core class on its first matrix-core variant
+        for pi, ai in classes:
+            pc, ac = plan.pairs[pi], plan.aux[ai]
+            v = next(v for v in range(lib.eip_three_center_nvariants(pc.la, pc.lb, ac.lc))
+                     if lib.eip_three_center_variant_mfma(pc.la, pc.lb, ac.lc, v) == 1)
+            lib.eip_set_three_center_variant(pc.la, pc.lb, ac.lc, 0, v)
+        np.testing.assert_allclose(three_center(pairs, aux, mol), run_three_center_oracle(oracle_lib(), pairs, aux, mol),
+                                   rtol=RTOL, atol=ATOL)
+    finally:
+        apply_tuning()

@@ -17,14 +17,19 @@ outputs (51 GB to 3.9 TB) exceed what a CPU check can produce:
    sampled outputs equal when each job is served alone.
 
 Tolerances: float64 paths |HIP - oracle| <= 1e-12 + 1e-10 |oracle|; the mixed-precision path max abs
-error <= 1e-10 against the oracle (its stated bound).
+error <= 1e-10 against the oracle (its stated bound).  The reference's Cartesian-then-spherical route
+cancels heavily for a few shell pairs of very unbalanced exponents (e.g. 0.087 with 4790 at 5 Bohr in a
+(df) pair: the float64 oracle and the compiled reference agree with each other and both differ from the
+long-double oracle by 1e-10); an element outside its bar is therefore decided by the long-double oracle,
+evaluated for that shell pair alone, within the same bar plus 10x the float64 oracle's own error there.
 """
 
 import numpy as np
 import pytest
 
-from tests.helpers import (ATOL, RTOL, assert_parity, oracle_aux_shell_block, oracle_lib, run_nuclear_oracle,
-                           run_overlap_oracle, run_three_center_oracle, run_two_center_oracle, sample_aux_shells)
+from tests.helpers import (ATOL, RTOL, assert_parity, oracle_aux_shell_block, oracle_ld_lib, oracle_lib,
+                           oracle_shell_pair_block, run_nuclear_oracle, run_overlap_oracle, run_three_center_oracle,
+                           run_two_center_oracle, sample_aux_shells)
 from electron_integral_playground_amd import generators, pair_utility
 
 pytestmark = pytest.mark.gpu
@@ -56,6 +61,45 @@ def _block_ao(pc, sp0, sp1):
     J = pc.sp_j[sp].astype(np.int64)[:, None, None] + np.arange(nj)[None, None, :]
     shape = (len(sp), ni, nj)
     return np.broadcast_to(I, shape).ravel(), np.broadcast_to(J, shape).ravel()
+
+
+def _compare(got, ref, what, bound_abs, bound_rel, arbitrate):
+    """got, ref: (rows, nk) against the float64 oracle.  Rows with an element outside bound_abs +
+    bound_rel |ref| are decided by ``arbitrate(rows) -> (long-double oracle, float64 oracle)`` values of
+    those rows: they pass within bound_abs + bound_rel |ld| + 10 |oracle64 - ld|.  Returns (max error
+    against the float64 oracle, rows decided by the long-double oracle)."""
+    got, ref = np.asarray(got), np.asarray(ref)
+    err = np.abs(got - ref)
+    bad = np.unique(np.nonzero(err > bound_abs + bound_rel * np.abs(ref))[0])
+    if len(bad):
+        ld, o64 = arbitrate(bad)
+        cond = np.abs(o64 - ld)
+        e2 = np.abs(got[bad] - ld)
+        ok = e2 <= bound_abs + bound_rel * np.abs(ld) + 10 * cond
+        if not ok.all():
+            k = np.unravel_index(int(np.argmax(np.where(ok, -1.0, e2))), e2.shape)
+            raise AssertionError(f"{what}: row {int(bad[k[0]])} component {int(k[1])}: got {got[bad][k]!r}, "
+                                 f"long-double oracle {ld[k]!r}, float64 oracle {o64[k]!r}")
+    return (float(err.max()) if err.size else 0.0), len(bad)
+
+
+def _arbiter(pd, order, sp_pp_off, pc, sp0, pk, lc, start):
+    """arbitrate(rows) for compact rows of shell pairs from sp0 of pair class pc: the oracle of each
+    involved shell pair alone, long double and float64."""
+    nij, nj = (2 * pc.la + 1) * (2 * pc.lb + 1), 2 * pc.lb + 1
+
+    def arbitrate(rows):
+        ld = np.empty((len(rows), 2 * lc + 1))
+        o64 = np.empty_like(ld)
+        for sp in np.unique(sp0 + rows // nij):
+            prims = order[sp_pp_off[sp]:sp_pp_off[sp + 1]]
+            b_ld = oracle_shell_pair_block(oracle_ld_lib(), pd, prims, pc.la, pc.lb, pk, lc, int(start))
+            b_64 = oracle_shell_pair_block(oracle_lib(), pd, prims, pc.la, pc.lb, pk, lc, int(start))
+            for k in np.flatnonzero(sp0 + rows // nij == sp):
+                mi, mj = divmod(int(rows[k]) % nij, nj)
+                ld[k], o64[k] = b_ld[:, mi, mj], b_64[:, mi, mj]
+        return ld, o64
+    return arbitrate
 
 
 def _q_aux(plan, J2c):
@@ -261,23 +305,31 @@ def test_mixed_200_atoms_float64_and_mixed_vs_oracle():
     c0 = 0
     worst64 = worst32 = 0.0
     sq32, n_el = 0.0, 0
+    n_arb = 0
     for lc, starts in sample.items():
         nk = 2 * lc + 1
+        ai = [a.lc for a in plan.aux].index(lc)
         for start in starts:
             ref = oracle_aux_shell_block(lib, pairs, aux[lc], lc, int(start), mol.n_ao)
             for bi, (I, J) in enumerate(ao):
+                pi, sp0, _ = blocks[bi]
+                pc = plan.pairs[pi]
+                arb = _arbiter(pairs[(pc.la, pc.lb)], plan._pair_orders[pi], pc.sp_pp_off, pc, sp0, aux[lc], lc, start)
                 r = ref[:, I, J].T
                 g64 = got64[bi][:, c0:c0 + nk].cpu().numpy()
                 g32 = got32[bi][:, c0:c0 + nk].cpu().numpy()
-                worst64 = max(worst64, assert_parity(g64, r, f"mixed workload float64 l={lc} shell {start} block {bi}"))
+                e64, n64 = _compare(g64, r, f"mixed workload float64 l={lc} shell {start} block {bi}", ATOL, RTOL, arb)
+                e32, n32 = _compare(g32, r, f"mixed workload mixed l={lc} shell {start} block {bi}", MIXED_BOUND, 0.0, arb)
+                worst64, worst32, n_arb = max(worst64, e64), max(worst32, e32), n_arb + n64 + n32
                 d = np.abs(g32 - r)
-                worst32 = max(worst32, float(d.max()))
                 sq32 += float((d * d).sum())
                 n_el += d.size
-            _say(f"  l={lc} shell {start}: worst float64 {worst64:.3e}, worst mixed {worst32:.3e}")
-            assert worst32 <= MIXED_BOUND, (lc, start, worst32)
+            _say(f"  l={lc} shell {start} (aux class {ai}): worst vs float64 oracle: float64 path {worst64:.3e}, "
+                 f"mixed {worst32:.3e}; rows decided by the long-double oracle so far {n_arb}")
             c0 += nk
-    _say(f"  mixed vs oracle: max abs {worst32:.3e}, RMS {(sq32 / n_el) ** 0.5:.3e} over {n_el} sampled elements")
+    _say(f"  mixed vs oracle: max abs {worst32:.3e}, RMS {(sq32 / n_el) ** 0.5:.3e} over {n_el} sampled elements, "
+         f"{n_arb} rows decided by the long-double oracle")
+    assert n_arb <= 1e-4 * n_el
     del got64, got32, plan
     _free()
 
