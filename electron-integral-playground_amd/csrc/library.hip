@@ -591,6 +591,77 @@ int three_center(int i_L, int j_L, int k_L, const double* pair_P_p, const double
                              n_aux_primitive_k, J3c_matrix, n_ao, spherical, omega);
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// No-allocation variant of three_center
+// ---------------------------------------------------------------------------------------------
+
+// Device memory of three_center_noalloc: a fixed region in the library's code object, one copy per
+// GPU the library runs on, held until the process ends.  In host code this name is a stand-in, not a
+// device address: set-up looks up each device's own address (scratch_region) and only that is used.
+constexpr size_t kNoAllocScratchBytes = size_t(256) << 20;        // 256 MiB per GPU
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch region, looked up once per device (with that device
+// current) and kept.
+int scratch_region(char** out)
+{
+    int dev = -1;
+    EIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        fprintf(stderr, "libmolecular_integral: three_center_noalloc: device %d beyond the %d supported\n", dev,
+                kMaxDevices);
+        return 2;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            fprintf(stderr, "libmolecular_integral: three_center_noalloc: hipGetSymbolAddress(scratch) failed: %s\n",
+                    hipGetErrorString(e));
+            return 2;
+        }
+        g_scratch_ptr[dev] = static_cast<char*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+long long three_center_noalloc_capacity(void) { return (long long) kNoAllocScratchBytes; }
+
+int three_center_noalloc(int i_L, int j_L, int k_L, const double* pair_P_p, const double* pair_A_a,
+                         const double* pair_B_b, const double* pair_coefficient, const int* pair_i_ao_start,
+                         const int* pair_j_ao_start, int n_pair_ij, const double* aux_A_a,
+                         const double* aux_coefficient, const int* aux_i_aux_start, int n_aux_primitive_k,
+                         double* J3c_matrix, int n_ao, int n_aux, bool spherical, double omega)
+{
+    if (!(i_L >= 0 && j_L >= 0 && i_L <= j_L && j_L <= kMaxL && k_L >= 0 && k_L <= kMaxAuxL)
+        && unsupported(__func__, i_L, j_L, k_L)) return 1;
+    (void) n_aux;
+    if (n_pair_ij <= 0 || n_aux_primitive_k <= 0) return 0;
+    char* region = nullptr;
+    if (int rc = scratch_region(&region)) return rc;
+    CallMem mem(region, kNoAllocScratchBytes);
+    // The null stream (no stream is created).  Every buffer of the call is a piece of the region and is
+    // fully written before it is read (pair matrices by their kernel, every compact output row by the
+    // class kernels), so nothing has to be cleared; calls that overlap on one GPU race on the region.
+    return three_center_impl(mem, nullptr, i_L, j_L, k_L, pair_P_p, pair_A_a, pair_B_b, pair_coefficient,
+                             pair_i_ao_start, pair_j_ao_start, n_pair_ij, aux_A_a, aux_coefficient, aux_i_aux_start,
+                             n_aux_primitive_k, J3c_matrix, n_ao, spherical, omega);
+}
+
 // ---------------------------------------------------------------------------------------------
 // Device-resident entry points
 // ---------------------------------------------------------------------------------------------

@@ -180,6 +180,23 @@ int eip_nuclear_device(int i_L, int j_L, const int* d_sp_pp_off, const double* d
                        int n_sp, const double* d_charge, int n_charge, double* d_out, long long ld, double omega,
                        void* stream);
 
+/* ---- Part 3: entry point that allocates no device memory ----------------------------------- */
+
+/* three_center without any device allocation: same arguments, accumulate (+=) and mirror semantics
+ * and return codes as three_center, and bit-identical results (the same kernels in the same order).
+ * No counterpart in the reference: it is for callers that cannot allocate device memory (harnesses
+ * that forbid hipMalloc, tight serving loops).  Every device buffer of the call is carved from a fixed
+ * region in the library (three_center_noalloc_capacity() bytes, 256 MiB, one per GPU, held until the
+ * process ends); a call whose inputs and smallest output batch do not fit returns 5 with a message on
+ * stderr.  Runs on the null stream of the current device.  Calls that overlap on one GPU share the
+ * region and race: serialise them. */
+int three_center_noalloc(int i_L, int j_L, int k_L, const double* pair_P_p, const double* pair_A_a,
+                         const double* pair_B_b, const double* pair_coefficient, const int* pair_i_ao_start,
+                         const int* pair_j_ao_start, int n_pair_ij, const double* aux_A_a,
+                         const double* aux_coefficient, const int* aux_i_aux_start, int n_aux_primitive_k,
+                         double* J3c_matrix, int n_ao, int n_aux, bool spherical, double omega);
+long long three_center_noalloc_capacity(void);
+
 #ifdef __cplusplus
 }
 #endif
