@@ -17,8 +17,11 @@ from .gen_kernels import (emit_j2c, emit_j3c, emit_j3c_mfma, emit_nuc, emit_pair
 
 SPEC_MAX_L = 3
 SPEC_MAX_AUX_L = 4
-# Register budgets (doubles per lane) of the chunk-plan variants; variant 0 is the default.
-VARIANT_BUDGETS = (200, 240, 160, 120, 320)
+# Register budgets (doubles per lane) of the chunk-plan variants; variant 0 is the default.  The build's
+# spill guard lowers only variant 0's budget (chunk_shrink.json); the others keep theirs, so classes whose
+# default is cut to one auxiliary component per kernel still offer wider chunks to the autotuner (which
+# skips variants that spill to scratch memory).
+VARIANT_BUDGETS = (200, 240, 160, 120, 320, 280)
 # Float64 matrix-core variants (appended after the vector ones) for pair classes with at least 9 output
 # rows: budgets of live Hermite integrals per chunk (the accumulators sit in the matrix-core registers).
 MFMA_PAIR_CLASSES = ((1, 1), (1, 2), (2, 2), (1, 3), (2, 3), (3, 3))
@@ -113,8 +116,8 @@ def generate(out_dir, shrink=None):
         mplans = {}
         for lc in range(SPEC_MAX_AUX_L + 1):
             cls_plans = []
-            for budget in VARIANT_BUDGETS:
-                plan = m_chunks(la, lb, lc, shrink.get((la, lb, lc), 0), budget)
+            for vi, budget in enumerate(VARIANT_BUDGETS):
+                plan = m_chunks(la, lb, lc, shrink.get((la, lb, lc), 0) if vi == 0 else 0, budget)
                 if plan not in cls_plans:
                     cls_plans.append(plan)
             plans[lc] = cls_plans
