@@ -25,10 +25,12 @@ Beside it, in the same line:
 """
 
 import argparse
+import csv
 import ctypes
 import json
 import os
 import pathlib
+import re
 import sys
 import time
 
@@ -50,7 +52,7 @@ PEAK_HBM = 8.0e12            # bytes/s
 SLICE_ATOMS = 24             # auxiliary shells of the first 24 atoms: 192 per angular class
 BLOCK_BYTES = 40e9           # device buffer of one streamed block
 CPU_SECONDS = 25.0           # bound of the CPU-baseline sample
-PMC_SUMMARY = ROOT / "profiles" / "j3c_mixed_slice_pmc.md"
+PMC_SUMMARY = ROOT / "profiles" / "j3c_slice200_plain_pmc.md"
 PMC_SUMMARY_PLAIN80 = ROOT / "profiles" / "j3c_plain80_pmc.md"
 METRIC = "3c integrals/s, mixed vs plain FP64 at <=1e-10 err; p50/p99 job latency"
 
@@ -59,17 +61,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def measured_traffic(summary, cls):
-    """HBM bytes of class (la, lb, lc) per step from a committed rocprofv3 WRITE_SIZE / FETCH_SIZE
-    summary (rows ``| j3c_<la><lb><lc> | ... | calibrated bytes |``); None if absent."""
+def measured_traffic(summary, cls, column=3):
+    """HBM bytes of class (la, lb, lc) per step from a committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary written by --pmc-summary (rows ``| j3c_<la><lb><lc> | fetch KiB | write KiB | calibrated
+    bytes | ...``); None if absent."""
     if not summary.exists():
         return None
     key = "j3c_%d%d%d" % cls
     for line in summary.read_text().splitlines():
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) >= 5 and cells[0] == key:
+        if len(cells) >= 4 and cells[0] == key:
             try:
-                return float(cells[4])
+                return float(cells[column])
             except ValueError:
                 return None
     return None
@@ -186,6 +189,159 @@ def sample_shells(aux):
     return out
 
 
+COPY_PROBE_BYTES = 256 << 20    # each of the copy kernel's source and destination
+COPY_PROBE_RUNS = 8
+# three-centre kernels of a plain FP64 pass: vector (j3c_) or matrix-core (j3cm_) chunk kernels, direct or
+# job-table forms; the float32 kernels carry an _f32 suffix and do not match
+J3C_F64_KERNEL = re.compile(r"^j3cm?_(\d)(\d)(\d)_\d+_\d+(_jobs)?$")
+
+
+def copy_probe():
+    """COPY_PROBE_RUNS launches of the library's plain copy kernel over COPY_PROBE_BYTES: the known-size
+    dispatches whose FETCH_SIZE / WRITE_SIZE readings calibrate the counter summaries."""
+    import torch
+    from electron_integral_playground_amd import library
+    from electron_integral_playground_amd.engine import _stream_handle
+    lib = library()
+    n = COPY_PROBE_BYTES // 8
+    a = torch.arange(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(COPY_PROBE_RUNS):
+        rc = lib.eip_copy_probe(ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()),
+                                ctypes.c_longlong(n), _stream_handle(None))
+        assert rc == 0, rc
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+    del a, b
+
+
+def profile_plain_leg(args):
+    """The headline's plain FP64 steps alone (W warmup + K timed), then one per-class HIP-event pass on
+    the launch stream: W + K + 1 plain passes over every class, the command the kernel-trace and counter
+    summaries are made from.  Returns per-class event times, model flops and algorithmic bytes."""
+    import torch
+    from electron_integral_playground_amd.engine import ThreeCenterPlan, three_center_flops_per_triplet
+    mol, mol_s, arrays, pairs, aux, _ = slice_workload(args.slice_atoms)
+    plan = ThreeCenterPlan(mol_s, pairs, aux)
+    blocks = plan.row_blocks(BLOCK_BYTES)
+    plan.prepare_blocks(blocks)
+    rows = max(plan.block_rows(*b) for b in blocks)
+    A = torch.empty((rows, mol_s.n_aux), dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream()
+    log(f"profile-plain: {len(blocks)} blocks per step, {max(1, args.warmup)} warmup + {args.steps} timed steps + "
+        "1 event pass")
+    for _ in range(max(1, args.warmup)):
+        for b in blocks:
+            plan.run_block(*b, A)
+    torch.cuda.synchronize()
+    log("profile-plain: warmup done")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for b in blocks:
+            plan.run_block(*b, A)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    log(f"profile-plain: {dt * 1e3:.1f} ms/step")
+    times = class_times(plan, blocks, A, False, stream)
+    trip = plan.primitive_triplets()
+    out_bytes, in_bytes = {}, {}
+    for pi, sp0, sp1 in blocks:
+        pc = plan.pairs[pi]
+        for ai, ac in enumerate(plan.aux):
+            k = (pc.la, pc.lb, ac.lc)
+            out_bytes[k] = out_bytes.get(k, 0) + plan.block_rows(pi, sp0, sp1) * ac.n_ks * (2 * ac.lc + 1) * 8
+            E = plan.kernel_E(pi, ai)
+            in_bytes[k] = in_bytes.get(k, 0) + E.numel() * E.element_size() * (sp1 - sp0) // max(1, pc.n_sp)
+    classes = {"%d%d%d" % k: {"event_s_per_pass": t, "model_flops": trip[k] * three_center_flops_per_triplet(*k),
+                              "store_bytes": out_bytes[k], "pair_matrix_bytes": in_bytes[k]}
+               for k, t in times.items()}
+    del A, plan
+    torch.cuda.empty_cache()
+    return {"profile_plain": {"passes": max(1, args.warmup) + args.steps + 1, "ms_per_step": dt * 1e3,
+                              "blocks_per_step": len(blocks), "classes": classes}}
+
+
+def _csv_rows(path):
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def _class_of(kernel):
+    m = J3C_F64_KERNEL.match(kernel.strip())
+    return None if m is None else "%s%s%s" % m.groups()[:3]
+
+
+def trace_summary(stats_csv, run_json):
+    """Markdown table: per class, dispatches, total and per-pass kernel time from a kernel_stats.csv of a
+    --profile-plain run, beside that run's own HIP-event time per pass."""
+    run = json.loads(pathlib.Path(run_json).read_text())["profile_plain"]
+    passes = run["passes"]
+    agg = {}
+    for r in _csv_rows(stats_csv):
+        cls = _class_of(r.get("Name") or r.get("Kernel_Name") or "")
+        if cls is None:
+            continue
+        a = agg.setdefault(cls, [0, 0.0])
+        a[0] += int(r["Calls"])
+        a[1] += float(r["TotalDurationNs"]) * 1e-9
+    lines = [f"plain passes per class: {passes}; bench ms/step {run['ms_per_step']:.1f}", "",
+             "| class | dispatches | kernel s (trace) | s per pass (trace) | s per pass (HIP events) | "
+             "trace / events | model TFLOP/s (trace) | frac of 78.6 |",
+             "|---|---|---|---|---|---|---|---|"]
+    for cls, (calls, tot) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        c = run["classes"].get(cls)
+        ev = c["event_s_per_pass"] if c else float("nan")
+        per = tot / passes
+        tf = c["model_flops"] / per / 1e12 if c else float("nan")
+        lines.append(f"| j3c_{cls} | {calls} | {tot:.4f} | {per:.5f} | {ev:.5f} | {per / ev:.3f} | {tf:.2f} | "
+                     f"{tf / 78.6:.3f} |")
+    return "\n".join(lines)
+
+
+def pmc_summary(fetch_csv, write_csv, run_json):
+    """Markdown table: per class, FETCH_SIZE and WRITE_SIZE per pass (KiB, summed over the class's
+    dispatches), each scaled by the factor the copy probe's dispatches give (known bytes / reading), beside
+    the algorithmic bytes; rows ``| j3c_<cls> | ... | calibrated bytes | ...``."""
+    run = json.loads(pathlib.Path(run_json).read_text())["profile_plain"]
+    passes = run["passes"]
+
+    def read(path, counter):
+        per_cls, probe = {}, []
+        for r in _csv_rows(path):
+            if r.get("Counter_Name", counter) != counter:
+                continue
+            name = r.get("Kernel_Name") or r.get("Name") or ""
+            v = float(r["Counter_Value"])
+            if "copy_probe_kernel" in name:
+                probe.append(v)
+            cls = _class_of(name)
+            if cls is not None:
+                per_cls[cls] = per_cls.get(cls, 0.0) + v
+        return per_cls, probe
+
+    F, pf = read(fetch_csv, "FETCH_SIZE")
+    W, pw = read(write_csv, "WRITE_SIZE")
+    known_kib = COPY_PROBE_BYTES / 1024
+    fF = known_kib / (sum(pf) / len(pf))
+    fW = known_kib / (sum(pw) / len(pw))
+    lines = [f"calibration: copy_probe_kernel ({len(pf)} + {len(pw)} dispatches, {COPY_PROBE_BYTES >> 20} MiB read "
+             f"+ {COPY_PROBE_BYTES >> 20} MiB written each): FETCH_SIZE {sum(pf) / len(pf):.1f} KiB -> factor "
+             f"{fF:.4f}; WRITE_SIZE {sum(pw) / len(pw):.1f} KiB -> factor {fW:.4f}",
+             f"plain passes per class: {passes}; values are per pass (one step's launches of the class)", "",
+             "| class | FETCH_SIZE KiB/pass | WRITE_SIZE KiB/pass | HBM bytes/pass (calibrated) | algorithmic bytes/pass "
+             "(stores + pair matrix once) | traffic / algorithmic |",
+             "|---|---|---|---|---|---|"]
+    rows = []
+    for cls in set(F) | set(W):
+        fk, wk = F.get(cls, 0.0) / passes, W.get(cls, 0.0) / passes
+        hbm = (fk * fF + wk * fW) * 1024
+        c = run["classes"].get(cls)
+        alg = c["store_bytes"] + c["pair_matrix_bytes"] if c else float("nan")
+        rows.append((hbm, f"| j3c_{cls} | {fk:.1f} | {wk:.1f} | {hbm:.4e} | {alg:.4e} | {hbm / alg:.3f} |"))
+    lines += [r for _, r in sorted(rows, reverse=True)]
+    return "\n".join(lines)
+
+
 def headline_leg(args):
     """Plain and mixed-precision steps over the 200-atom workload's auxiliary-block share."""
     import torch
@@ -245,7 +401,9 @@ def headline_leg(args):
                 % (key + (plan.lib.eip_three_center_nchunks(*key),)),
                 "achieved": fl / t / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": fl / t / PEAK_FP64,
                 "traffic": measured_traffic(PMC_SUMMARY, key),
-                "traffic_unit": "HBM bytes per step for this class (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
+                "traffic_unit": "HBM bytes per step for this class (rocprofv3 FETCH_SIZE x 1.9998 + WRITE_SIZE x 1.0000, "
+                                "factors from the library's plain copy kernel)",
+                "traffic_algorithmic": measured_traffic(PMC_SUMMARY, key, column=4),
                 "algorithmic_flops_per_step": fl, "class_seconds_per_step": t}
 
     # error pass over the whole slice (mixed vs plain on the device), keeping the CPU sample's columns
@@ -345,6 +503,23 @@ def plain80_leg(args):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
     n_int = plan.n_integrals()
+    if args.profile_plain:
+        classes = {}
+        for pi, pc in enumerate(plan.pairs):
+            ni, nj = 2 * pc.la + 1, 2 * pc.lb + 1
+            diag = int((np.asarray(pc.sp_i) == np.asarray(pc.sp_j)).sum())
+            rows = (pc.n_sp - diag) * ni * nj + diag * ni * (ni + 1) // 2
+            for ai, ac in enumerate(plan.aux):
+                k = (pc.la, pc.lb, ac.lc)
+                E = plan.kernel_E(pi, ai)
+                classes["%d%d%d" % k] = {"store_bytes": rows * ac.n_ks * (2 * ac.lc + 1) * 8,
+                                         "pair_matrix_bytes": E.numel() * E.element_size()}
+        for k, t, fl in per_class:
+            classes["%d%d%d" % k].update(event_s_per_pass=t, model_flops=fl)
+        del out, plan, J2c
+        torch.cuda.empty_cache()
+        return {"profile_plain": {"passes": max(1, args.warmup) + 1 + args.steps, "ms_per_step": dt * 1e3,
+                                  "classes": classes}}
     key, t, fl = per_class[0]
     log(f"plain80 step {dt * 1e3:.2f} ms -> {n_int / dt:.4e} integrals/s; dominant class {key} {t * 1e3:.3f} ms "
         f"{fl / t / 1e12:.2f} TFLOP/s (model)")
@@ -609,8 +784,25 @@ def main():
     ap.add_argument("--single-stream", action="store_true",
                     help="plain80: every launch on one stream (profiling runs: kernel durations do not overlap)")
     ap.add_argument("--copy-probe", action="store_true",
-                    help="also run one 4 GiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+                    help="first run the library's plain copy kernel %d times over %d MiB (calibrates rocprofv3 "
+                         "FETCH_SIZE/WRITE_SIZE)" % (COPY_PROBE_RUNS, COPY_PROBE_BYTES >> 20))
+    ap.add_argument("--profile-plain", action="store_true",
+                    help="headline (or, with --plain80-only, the 80-atom block): only the plain FP64 passes and "
+                         "one per-class event pass, printing the per-class figures the --trace-summary and "
+                         "--pmc-summary tables use (the command profiled for them)")
+    ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "RUN_JSON"),
+                    help="write the per-class HBM traffic table from two rocprofv3 --pmc counter_collection.csv "
+                         "files of a --profile-plain run whose printed line is in RUN_JSON, then exit")
+    ap.add_argument("--trace-summary", nargs=2, metavar=("KERNEL_STATS_CSV", "RUN_JSON"),
+                    help="write the per-class kernel-time table from a rocprofv3 --kernel-trace --stats "
+                         "kernel_stats.csv of a --profile-plain run, then exit")
     args = ap.parse_args()
+    if args.pmc_summary:
+        print(pmc_summary(*args.pmc_summary))
+        return
+    if args.trace_summary:
+        print(trace_summary(*args.trace_summary))
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the integral path here is a single-GPU build "
               "(multi-GPU sharding by auxiliary blocks is a design note, not implemented).", file=sys.stderr)
@@ -623,11 +815,7 @@ def main():
     torch.cuda.set_device(0)
     variants = apply_tuning().get("variants", {})
     if args.copy_probe:
-        a = torch.empty(1 << 29, dtype=torch.float64, device="cuda")
-        b = torch.empty_like(a)
-        b.copy_(a)                                   # 4 GiB read + 4 GiB written
-        torch.cuda.synchronize()
-        del a, b
+        copy_probe()
     if args.serving:
         print(json.dumps({"serving": serving_leg(args.serving_jobs, args.serving_rate, mixed=not args.serving_plain,
                                                  part=args.serving_part, latency_file=args.serving_latencies)}),
@@ -635,6 +823,9 @@ def main():
         return
     if args.batched_only:
         print(json.dumps({"batched": batched_leg()}), flush=True)
+        return
+    if args.profile_plain:
+        print(json.dumps(plain80_leg(args) if args.plain80_only else profile_plain_leg(args)), flush=True)
         return
     if args.mixed_full:
         print(json.dumps({"mixed_full": mixed_leg(tune=args.tune, aux_order=args.mixed_aux_order)}), flush=True)

@@ -288,6 +288,18 @@ __global__ void __launch_bounds__(256) zero_spans_paced_kernel(const long long* 
     }
 }
 
+// Plain copy of n doubles, one 8-byte load and one 8-byte store per lane (the store width of the
+// three-centre kernels), grid-stride over a fixed grid: the known-size kernel that calibrates the HBM
+// byte counters of a profiler.
+constexpr int kCopyProbeBlocks = 8192;
+
+__global__ void __launch_bounds__(256) copy_probe_kernel(const double* __restrict__ src, double* __restrict__ dst,
+                                                         long long n)
+{
+    const long long stride = (long long) gridDim.x * blockDim.x;
+    for (long long i = (long long) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 __global__ void __launch_bounds__(256) zero_rows_kernel(const int* __restrict__ rows, int n_rows, long long ld,
                                                         double* __restrict__ out)
 {
@@ -804,6 +816,13 @@ int eip_three_center_work(int i_L, int j_L, int k_L, int use_f32, const int* d_i
     const int v = current_variant(i_L, j_L, k_L, use_f32);
     for (int c = 0; c < eip_j3c_nchunks(i_L, j_L, k_L, v); ++c)
         if (eip_launch_j3c(i_L, j_L, k_L, v, c, use_f32, a, n_items, (hipStream_t) stream) != 0) return 3;
+    return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int eip_copy_probe(const double* d_src, double* d_dst, long long n, void* stream)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(copy_probe_kernel, dim3(kCopyProbeBlocks), dim3(256), 0, (hipStream_t) stream, d_src, d_dst, n);
     return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 

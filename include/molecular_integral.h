@@ -157,6 +157,11 @@ int eip_three_center_work(int i_L, int j_L, int k_L, int use_f32, const int* d_i
                           const int* d_items_pp, const int* d_items_job, int n_items, const void* d_jobs, int layout,
                           int accumulate, double omega, void* stream);
 
+/* d_dst[i] = d_src[i] for i < n (device pointers): a plain copy kernel of known size, 8 n bytes read and
+ * 8 n bytes written, with one 8-byte load and store per lane.  No counterpart in the reference: it
+ * calibrates the profiler's HBM byte counters (FETCH_SIZE, WRITE_SIZE) for the traffic figures. */
+int eip_copy_probe(const double* d_src, double* d_dst, long long n, void* stream);
+
 /* Zero rows d_rows[0..n_rows) (each ld doubles long) of d_out: the packed rows of screened-out shell
  * pairs, which no three-centre kernel writes. */
 int eip_zero_rows(const int* d_rows, int n_rows, long long ld, double* d_out, void* stream);
