@@ -350,12 +350,14 @@ def headline_leg(args):
     plan = ThreeCenterPlan(mol_s, pairs, aux)
     if args.tune:
         from electron_integral_playground_amd.engine import autotune
-        autotune(plan, save=args.tune, log=log, block_bytes=4e9)
+        if not args.tune_f32_only:
+            autotune(plan, save=args.tune, log=log, block_bytes=4e9)
         autotune(plan, save=args.tune, log=log, block_bytes=4e9, use_f32=True)
     J2c = plan.two_center()
     q = q_aux_from_metric(plan, J2c)
     del J2c
-    stats = plan.setup_mixed(arrays, q, threshold=args.threshold)
+    stats = plan.setup_mixed(arrays, q, threshold=args.threshold, granularity=args.mixed_granularity,
+                             estimate=args.mixed_estimate if args.mixed_granularity == "item" else "schwarz")
     blocks = plan.row_blocks(BLOCK_BYTES)
     plan.prepare_blocks(blocks, mixed=True)
     rows = max(plan.block_rows(*b) for b in blocks)
@@ -416,6 +418,8 @@ def headline_leg(args):
     max_ref = torch.zeros((), dtype=torch.float64, device="cuda")
     got = {"plain": [], "mixed": []}
     n_el = 0
+    if plan.checked:
+        plan.checked_tally = {}
     for b in blocks:
         n = plan.block_rows(*b)
         plan.run_block(*b, A)
@@ -430,6 +434,13 @@ def headline_leg(args):
         got["mixed"].append(B[:n].index_select(1, cols_t).cpu().numpy())
         n_el += n * mol_s.n_aux
     torch.cuda.synchronize()
+    if plan.checked:
+        tally = {k: (int(a.item()), int(b.item())) for k, (a, b) in plan.checked_tally.items()}
+        plan.checked_tally = None
+        stats["f64_triplets"] = sum(b for _, b in tally.values())
+        stats["f32_triplets"] = sum(a - b for a, b in tally.values())
+        log("checked split: float64 share per class " +
+            ", ".join(f"{''.join(map(str, k))} {b / max(a, 1):.3f}" for k, (a, b) in sorted(tally.items())))
     del A, B
     torch.cuda.empty_cache()
     gpu_cols = {name: {} for name in got}
@@ -448,7 +459,8 @@ def headline_leg(args):
         J = np.broadcast_to(pc.sp_j[sp].astype(np.int64)[:, None, None] + np.arange(nj)[None, None, :], (len(sp), ni, nj))
         blocks_ao.append((I.ravel(), J.ravel()))
     mixed = {"value": n_int / dt_mixed, "unit": "3c integrals/s", "ms_per_step": dt_mixed * 1e3,
-             "mixed_over_plain": dt / dt_mixed, "threshold": args.threshold,
+             "mixed_over_plain": dt / dt_mixed, "threshold": args.threshold, "granularity": args.mixed_granularity,
+             "estimate": stats.get("estimate", "schwarz"), "estimate_seconds": stats.get("estimate_seconds"),
              "f32_triplet_fraction": stats["f32_triplets"] / (stats["f32_triplets"] + stats["f64_triplets"]),
              "max_abs_err_vs_plain_fp64": max_abs.item(), "rms_err_vs_plain_fp64": (sum_sq.item() / n_el) ** 0.5,
              "max_abs_value": max_ref.item(), "compared_elements": n_el,
@@ -758,7 +770,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--threshold", type=float, default=1e-6, help="float32-tier bound of the mixed split")
+    ap.add_argument("--threshold", type=float, default=1e-6, help="mixed split threshold: on the values (checked) or on the bound (item, pair)")
     ap.add_argument("--slice-atoms", type=int, default=SLICE_ATOMS,
                     help="auxiliary shells of this many atoms form the headline slice")
     ap.add_argument("--headline-only", action="store_true", help="only the headline (profiling runs)")
@@ -767,9 +779,18 @@ def main():
     ap.add_argument("--mixed-full", action="store_true",
                     help="run only the full 200-atom comparison (all 3.9 TB streamed by both paths)")
     ap.add_argument("--mixed-aux-order", choices=("qp", "natural"), default="natural")
+    ap.add_argument("--mixed-granularity", choices=("checked", "item", "pair"), default="item",
+                    help="headline mixed split: whole items ('item', default), the float64 prefix / float32 "
+                         "suffix of an item's primitive pairs ('pair', Schwarz estimate only), or 'checked' "
+                         "(bf16 matrix-core float32 pass first, float64 for the items it flags)")
+    ap.add_argument("--mixed-estimate", choices=("envelope", "schwarz"), default="envelope",
+                    help="triplet estimate of the split: the a-priori term scale with distance decay "
+                         "('envelope', default) or the reference's Schwarz-type bound ub_pp Q_P ('schwarz')")
     ap.add_argument("--tune", metavar="JSON",
                     help="autotune the chunk-plan variants (vector and matrix-core) on the headline slice first "
                          "(or, with --mixed-full, on the full workload) and save the table to JSON")
+    ap.add_argument("--tune-f32-only", action="store_true", help="with --tune: the float32 tier's variants only")
+    ap.add_argument("--tuning", metavar="JSON", help="apply this variant table on top of the shipped one")
     ap.add_argument("--serving", action="store_true", help="run only the serving workload")
     ap.add_argument("--batched-only", action="store_true", help="run only the batched workload")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched workload")
@@ -814,6 +835,8 @@ def main():
 
     torch.cuda.set_device(0)
     variants = apply_tuning().get("variants", {})
+    if args.tuning:
+        variants.update(apply_tuning(args.tuning).get("variants", {}))
     if args.copy_probe:
         copy_probe()
     if args.serving:

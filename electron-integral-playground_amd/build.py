@@ -29,7 +29,7 @@ FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wno-unused-va
 
 def _sources():
     srcs = sorted(GEN.glob("*.hip")) + [GEN / "registry.cpp", CSRC / "generic_kernels.hip", CSRC / "library.hip",
-                                        CSRC / "pair_screening.cpp"]
+                                        CSRC / "mixed_estimate.hip", CSRC / "pair_screening.cpp"]
     # largest first, so the long compiles start early
     return sorted(srcs, key=lambda p: -p.stat().st_size)
 

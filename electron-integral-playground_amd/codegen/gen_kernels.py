@@ -360,6 +360,8 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64", stage=None):
     out.append("    const int lb_ = eip::xcd_remap(blockIdx.x, gridDim.x);")
     out.append("    const int item = a.item_begin + lb_;")
     out.append("    if (item >= a.item_end) return;")
+    if not f32:
+        out.append("    if (a.flags_in && !eip::item_flagged(a.flags_in + (size_t) item * a.flag_stride, a.flag_stride)) return;")
     out.append("    const eip::cptr<J3cData> jt = kJobs ? j3c_item_job(a, item) : (eip::cptr<J3cData>) nullptr;")
     out.append("#define JD(f) (kJobs ? (decltype(a.d.f)) jt->f : a.d.f)")
     out.append("    const int sp = eip::ro(a.items)[2 * item];")
@@ -972,6 +974,7 @@ def emit_j3c_mfma(la, lb, lc, m0, m1):
     out.append("{")
     out.append("    const int item = a.item_begin + eip::xcd_remap(blockIdx.x, gridDim.x);")
     out.append("    if (item >= a.item_end) return;")
+    out.append("    if (a.flags_in && !eip::item_flagged(a.flags_in + (size_t) item * a.flag_stride, a.flag_stride)) return;")
     out.append("    const eip::cptr<J3cData> jt = kJobs ? j3c_item_job(a, item) : (eip::cptr<J3cData>) nullptr;")
     out.append("#define JD(f) (kJobs ? (decltype(a.d.f)) jt->f : a.d.f)")
     out.append("    const int wave = (int) threadIdx.x >> 6, lane = (int) threadIdx.x & 63;")
@@ -1073,6 +1076,289 @@ def emit_j3c_mfma(la, lb, lc, m0, m1):
             out.append("        const long long row = (a.layout == 0 ? eip::packed_row(I + mi, J + mj) : base + ij) * ld + col;")
             for m in ms:
                 out.append(f"        if (a.accumulate) o[row + {m}] += acc_{m}_{b}[{r_}]; else o[row + {m}] = acc_{m}_{b}[{r_}];")
+            out.append("      }")
+            out.append("    }")
+    out.append("#undef JD")
+    out.append("}")
+    out.append(f'extern "C" __global__ void __launch_bounds__({64 * MFMA_WAVES}) {name}(const J3cArgs a) {{ {name}_body<false>(a); }}')
+    out.append(f'extern "C" __global__ void __launch_bounds__({64 * MFMA_WAVES}) {name}_jobs(const J3cArgs a) {{ {name}_body<true>(a); }}')
+    return name, "\n".join(out)
+
+
+# ---------------------------------------------------------------------------------------------
+# Float32-tier three-centre kernel on the bf16 matrix cores (v_mfma_f32_16x16x32_bf16)
+# ---------------------------------------------------------------------------------------------
+#
+# The same work decomposition as the float64 matrix-core kernel (four primitive-pair slots x 16
+# auxiliary shells per wave, four waves per item), with K = 32 = 4 pair slots x 8 consecutive Hermite
+# indices per instruction: lane l holds A[row = l % 16][k = 8 (l / 16) + v] and B[k][aux = l % 16] for
+# v < 8, and accumulator register r holds row 4 (l / 16) + r (checked on the device by a probe).  Every
+# operand is split into BF16_PIECES bfloat16 pieces x = x_0 + x_1 + ... (x_0 = bf16(x), x_1 =
+# bf16(x - x_0), ...); the products of pieces i, j with i + j < BF16_PIECES are accumulated in float32.
+# With three pieces each operand is represented to 2^-24 of its magnitude and the dropped products are
+# of that order, so a product carries float32-level relative error; the Boys function, the Hermite
+# integrals and the auxiliary transform are evaluated in float32 as in the vector float32 kernels.
+
+BF16_PIECES = 3
+# steps (Hermite group, row block) the pair-matrix operand loads run ahead of their use
+BF16_E_AHEAD = 2
+
+
+def bf16_groups(la, lb):
+    """Groups of 8 pair Hermite indices (the K slices of one lane)."""
+    return -(-len(mfma_hermite(la, lb)) // 8)
+
+
+def bf16_dense_width(la, lb):
+    """bfloat16 values per primitive pair of the split dense layout [piece][group][block][row][8]."""
+    return BF16_PIECES * bf16_groups(la, lb) * mfma_blocks(la, lb) * 16 * 8
+
+
+def bf16_chunks(la, lb, lc, budget):
+    """Split the auxiliary components over bf16 matrix-core kernels so that each chunk's live vector
+    registers -- float Hermite integrals, the split auxiliary operands (4 registers per piece and
+    component) and the accumulators (4 per component and row block) -- stay within budget."""
+    nk = 2 * lc + 1
+    nb = mfma_blocks(la, lb)
+
+    def regs(m0, m1):
+        n = m1 - m0
+        return _needed_R(la, lb, lc, m0, m1) + 4 * BF16_PIECES * n + 4 * n * nb + 4 * BF16_PIECES + 48
+
+    for n in range(1, nk + 1):
+        per = -(-nk // n)
+        plan = [(m, min(nk, m + per)) for m in range(0, nk, per)]
+        if all(regs(m0, m1) <= budget for m0, m1 in plan):
+            return plan
+    return [(m, m + 1) for m in range(nk)]
+
+
+def emit_pair_bf16(la, lb):
+    """pairB_<la><lb>: packed float64 pair-matrix rows -> the split bfloat16 dense layout (one thread per
+    primitive pair; 16-byte stores of 8 Hermite indices of one row)."""
+    name = f"pairB_{la}{lb}"
+    ts = mfma_hermite(la, lb)
+    tix = {t: i for i, t in enumerate(ts)}
+    nq, nb = bf16_groups(la, lb), mfma_blocks(la, lb)
+    layout = pair_layout(la, lb, True)
+    stride = pair_stride(la, lb, True)
+    where = {(tix[t], row): k for k, (t, row) in enumerate(layout)}
+    W = bf16_dense_width(la, lb)
+    L = [f"// split bfloat16 dense pair matrix of class ({la},{lb}): {BF16_PIECES} pieces x {nq} groups x {nb} blocks"]
+    L.append(f'extern "C" __global__ void __launch_bounds__(256) {name}(const double* __restrict__ E, '
+             f'eip::bf8* __restrict__ D, int n_pp)')
+    L.append("{")
+    L.append("    const int pp = blockIdx.x * blockDim.x + threadIdx.x;")
+    L.append("    if (pp >= n_pp) return;")
+    L.append(f"    const double* __restrict__ e = E + (size_t) pp * {stride};")
+    L.append(f"    eip::bf8* __restrict__ d = D + (size_t) pp * {W // 8};")
+    for q in range(nq):
+        for b in range(nb):
+            for row in range(16):
+                vals = []
+                for v in range(8):
+                    key = (8 * q + v, 16 * b + row)
+                    vals.append(f"e[{where[key]}]" if key in where else "0.0")
+                L.append("    { const double x[8] = {" + ", ".join(vals) + "};")
+                L.append(f"      eip::bf8 h[{BF16_PIECES}];")
+                L.append(f"      eip::split_bf16<{BF16_PIECES}>(x, h);")
+                for pc in range(BF16_PIECES):
+                    off = ((pc * nq + q) * nb + b) * 16 + row
+                    L.append(f"      d[{off}] = h[{pc}];")
+                L.append("    }")
+    L.append("}")
+    return name, "\n".join(L)
+
+
+def emit_j3c_bf16(la, lb, lc, m0, m1):
+    """Float32-tier three-centre kernel on the bf16 matrix cores for auxiliary components [m0, m1) of
+    class (la lb | lc); reads the split dense pair matrix (J3cData.E32 points to it) and the vector
+    kernels' items, and stores (or adds) its float32 sums into the float64 output."""
+    spherical = True
+    name = f"j3cb_{la}{lb}{lc}_{m0}_{m1}"
+    L = la + lb + lc
+    nij = n_out(la, spherical) * n_out(lb, spherical)
+    nb = mfma_blocks(la, lb)
+    ts = mfma_hermite(la, lb)
+    nq = bf16_groups(la, lb)
+    WB = bf16_dense_width(la, lb) // 8           # bf8 vectors per primitive pair
+    aux_rows = aux_transform(lc, spherical)
+    ms = list(range(m0, m1))
+    needed = set()
+    for t in ts:
+        for m in ms:
+            for s in aux_rows[m][1]:
+                needed.add((t[0] + s[0], t[1] + s[1], t[2] + s[2]))
+    P = BF16_PIECES
+    prods = [(i, j) for i in range(P) for j in range(P) if i + j < P]
+    out = [f"// ({la}{lb}|{lc}) aux components [{m0},{m1}) float32 tier on bf16 matrix cores ({P} pieces): L = {L}"]
+    out.append("template <bool kJobs>")
+    out.append(f"__device__ __forceinline__ void {name}_body(const J3cArgs a)")
+    out.append("{")
+    out.append("    const int item = a.item_begin + eip::xcd_remap(blockIdx.x, gridDim.x);")
+    out.append("    if (item >= a.item_end) return;")
+    out.append("    const eip::cptr<J3cData> jt = kJobs ? j3c_item_job(a, item) : (eip::cptr<J3cData>) nullptr;")
+    out.append("#define JD(f) (kJobs ? (decltype(a.d.f)) jt->f : a.d.f)")
+    out.append("    const int wave = (int) threadIdx.x >> 6, lane = (int) threadIdx.x & 63;")
+    out.append("    const int kslot = lane >> 4, jcol = lane & 15;")
+    out.append("    const int sp = eip::ro(a.items)[2 * item];")
+    out.append("    const int ks_first = eip::ro(a.items)[2 * item + 1] + 16 * wave;")
+    out.append("    const int ks_last = a.items_ks_end ? eip::ro(a.items_ks_end)[item] : (int) JD(n_ks);")
+    out.append("    int* const flag = a.flags_out ? a.flags_out + (size_t) item * a.flag_stride + a.flag_slot + wave : nullptr;")
+    out.append("    if (ks_first >= ks_last) {                       // no auxiliary shell for this wave")
+    out.append("        if (flag && lane == 0) *flag = 0;")
+    out.append("        return;")
+    out.append("    }")
+    out.append("    const int ks_raw = ks_first + jcol;")
+    out.append("    const bool active = ks_raw < ks_last;")
+    out.append("    const int ks = active ? ks_raw : ks_last - 1;")
+    out.append("    const int ksh = JD(ks_perm) ? eip::ro(JD(ks_perm))[ks] : ks;")
+    out.append("    const int kp0 = eip::ro(JD(ks_prim_off))[ksh], kp1 = eip::ro(JD(ks_prim_off))[ksh + 1];")
+    out.append("    const int pp0 = a.items_pp ? eip::ro(a.items_pp)[2 * item] : eip::ro(JD(sp_pp_off))[sp];")
+    out.append("    const int pp1 = a.items_pp ? eip::ro(a.items_pp)[2 * item + 1] : eip::ro(JD(sp_pp_off))[sp + 1];")
+    out.append("    const eip::bf8* __restrict__ EB = (const eip::bf8*) JD(E32);")
+    out.append("    const double* __restrict__ PP = JD(pp_P);")
+    for m in ms:
+        for b in range(nb):
+            out.append(f"    eip::v4f acc_{m}_{b} = {{0.0f, 0.0f, 0.0f, 0.0f}};")
+    # checked mixed precision: sum_k |E_k| A_{|t_k| + lc} per output row and auxiliary shell (leading
+    # pieces), times the largest sum of |transform coefficients| of the chunk's components: the scale of
+    # the float32 rounding of every term the row's sums contain
+    for b in range(nb):
+        out.append(f"    eip::v4f mag_{b} = {{0.0f, 0.0f, 0.0f, 0.0f}};")
+    out.append("    const int nkp_wave = eip::wave_max(kp1 - kp0);")
+    out.append("    for (int t_kp = 0; t_kp < nkp_wave; ++t_kp) {")
+    out.append("      const bool kp_on = kp0 + t_kp < kp1;")
+    out.append("      const int kp = kp_on ? kp0 + t_kp : kp1 - 1;")
+    out.append("      const double Cx = JD(ka_C)[4 * kp + 0], Cy = JD(ka_C)[4 * kp + 1], Cz = JD(ka_C)[4 * kp + 2];")
+    out.append("      const double q = JD(ka_C)[4 * kp + 3];")
+    out.append("      const double ck = kp_on ? JD(ka_coef)[kp] : 0.0;")
+    out.extend(_aux_prefactor_lines(lc))
+    out.append("      double Pn0 = 0, Pn1 = 0, Pn2 = 0, Pn3 = 1;")
+    out.append("      if (pp0 < pp1) { const int q0 = min(pp0 + kslot, pp1 - 1);")
+    out.append("        Pn0 = PP[4 * (size_t) q0 + 0]; Pn1 = PP[4 * (size_t) q0 + 1]; Pn2 = PP[4 * (size_t) q0 + 2]; Pn3 = PP[4 * (size_t) q0 + 3]; }")
+    out.append("      for (int g0 = pp0; g0 < pp1; g0 += 4) {")
+    out.append("        const int pq = g0 + kslot;")
+    out.append("        const bool pp_on = pq < pp1;")
+    out.append("        const int pp = pp_on ? pq : pp1 - 1;")
+    out.append("        const double Px = Pn0, Py = Pn1, Pz = Pn2, p = Pn3;")
+    out.append(f"        const eip::bf8* __restrict__ Eb = EB + (size_t) pp * {WB} + jcol;")
+    steps = [(qg, b) for qg in range(nq) for b in range(nb)]
+
+    def load_step(k, indent):
+        qg_, b_ = steps[k]
+        return [f"{indent}const eip::bf8 ea_{k}_{pc} = Eb[{((pc * nq + qg_) * nb + b_) * 16}];" for pc in range(P)]
+
+    # pair-matrix operands of the first steps issued before the Boys function and the Hermite
+    # integrals (their latency hides behind that work); step k + BF16_E_AHEAD is issued with step k
+    for k in range(min(BF16_E_AHEAD, len(steps))):
+        out.extend(load_step(k, "        "))
+    out.append("        { const int qn = min(g0 + 4 + kslot, pp1 - 1);      // the next group's pair (the last reloads one)")
+    out.append("          Pn0 = PP[4 * (size_t) qn + 0]; Pn1 = PP[4 * (size_t) qn + 1]; Pn2 = PP[4 * (size_t) qn + 2]; Pn3 = PP[4 * (size_t) qn + 3]; }")
+    # prologue in float64 (cheap), then float32 for Boys, R and the transform, as the vector float32 kernels
+    out.append("        const double dPQx = Px - Cx, dPQy = Py - Cy, dPQz = Pz - Cz;")
+    out.append("        double zeta, pf;")
+    out.append("        if (a.omega == 0.0) {")
+    out.append("            const double rs = eip::rsq_nr(p + q);")
+    out.append("            zeta = p * q * (rs * rs);")
+    out.append("            pf = ckq * eip::rcp_nr(p) * rs;")
+    out.append("        } else {")
+    out.append("            const double w2 = a.omega * a.omega;")
+    out.append("            zeta = p * q * eip::rcp_nr(q + p + p * q * eip::rcp_nr(w2));")
+    out.append("            pf = ckq * a.omega * eip::rcp_nr(p) * eip::rsq_nr(p * q + (p + q) * w2);")
+    out.append("        }")
+    out.append("        if (!pp_on) pf = 0.0;                        // an empty slot of the group adds nothing")
+    out.append("        const float PQx = (float) dPQx, PQy = (float) dPQy, PQz = (float) dPQz;")
+    out.append("        const float T = (float) (zeta * (dPQx * dPQx + dPQy * dPQy + dPQz * dPQz));")
+    out.append(f"        float F[{L + 1}];")
+    out.append(f"        eip::boys32<{L}>(T, F);")
+    out.append("        const float m2z = (float) (-2.0 * zeta);")
+    out.append("        const float R000_0 = (float) pf * F[0];")
+    for n in range(1, L + 1):
+        out.append(f"        const float pf_{n} = {'(float) pf' if n == 1 else f'pf_{n - 1}'} * m2z;")
+        out.append(f"        const float R000_{n} = pf_{n} * F[{n}];")
+    # Envelope of the Hermite integrals' terms: A_N^(n) = (N-1) A_{N-2}^(n+1) + |PQ| A_{N-1}^(n+1) from
+    # A_0^(n) = |R000^(n)| -- the Hermite recursion with every term taken positive and |PQ| >= |PQx|,
+    # |PQy|, |PQz| -- bounds the magnitude of every term summed into an R_tuv of order t + u + v = N, the
+    # scale its float32 rounding (amplified by the recursion's cancellation) is relative to.
+    out.append(f"        float Aenv[{L + 1}];")
+    out.append("        {")
+    out.append("          const float Rd = sqrtf(PQx * PQx + PQy * PQy + PQz * PQz);")
+    for N in range(L + 1):
+        for n in range(L - N + 1):
+            if N == 0:
+                ex = f"fabsf(R000_{n})"
+            elif N == 1:
+                ex = f"Rd * ae_0_{n + 1}"
+            else:
+                ex = f"{float(N - 1)}f * ae_{N - 2}_{n + 1} + Rd * ae_{N - 1}_{n + 1}"
+            out.append(f"          const float ae_{N}_{n} = {ex};")
+        out.append(f"          Aenv[{N}] = ae_{N}_0;")
+    out.append("        }")
+    _emit_R(out, L, needed, "float")
+    for qg in range(nq):
+        out.append(f"        // Hermite indices {8 * qg} .. {min(len(ts), 8 * qg + 8) - 1}")
+        for m in ms:
+            r, terms = aux_rows[m]
+            vals = []
+            for v in range(8):
+                ti = 8 * qg + v
+                if ti >= len(ts):
+                    vals.append("0.0f")
+                    continue
+                t = ts[ti]
+                pieces = []
+                for s_, (qf, j_) in terms.items():
+                    assert j_ == 0
+                    coef = SqrtRational(r, qf).to_float()
+                    rn = _r_name(0, (t[0] + s_[0], t[1] + s_[1], t[2] + s_[2]))
+                    pieces.append(rn if coef == 1.0 else f"{fmt(coef)}f * {rn}")
+                vals.append("(" + " + ".join(pieces) + ")")
+            out.append(f"        eip::bf8 Vb_{qg}_{m}[{P}];")
+            out.append(f"        {{ const float x[8] = {{{', '.join(vals)}}};")
+            out.append(f"          eip::split_bf16<{P}>(x, Vb_{qg}_{m}); }}")
+        env = ", ".join(f"Aenv[{sum(ts[8 * qg + v]) + lc}]" if 8 * qg + v < len(ts) else "0.0f" for v in range(8))
+        out.append(f"        eip::bf8 venv_{qg}[1];")
+        out.append(f"        {{ const float x[8] = {{{env}}}; eip::split_bf16<1>(x, venv_{qg}); }}")
+        for b in range(nb):
+            k = steps.index((qg, b))
+            if k + BF16_E_AHEAD < len(steps):
+                out.extend(load_step(k + BF16_E_AHEAD, "        "))
+            # products in piece order, components innermost: consecutive instructions never share an
+            # accumulator
+            for i_, j_ in prods:
+                for m in ms:
+                    out.append(f"        acc_{m}_{b} = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ea_{k}_{i_}, Vb_{qg}_{m}[{j_}], acc_{m}_{b}, 0, 0, 0);")
+            out.append(f"        mag_{b} = __builtin_amdgcn_mfma_f32_16x16x32_bf16(eip::bf16_abs(ea_{k}_0), venv_{qg}[0], mag_{b}, 0, 0, 0);")
+    out.append("      }")
+    out.append("    }")
+    cmax = max(sum(abs(SqrtRational(aux_rows[m][0], qf).to_float()) for qf, _ in aux_rows[m][1].values()) for m in ms)
+    out.append("    if (flag) {                                      // this wave's largest term scale against the threshold")
+    out.append(f"        const float cmax = {fmt(cmax)}f;")
+    out.append("        float amax = 0.0f;")
+    out.append("        if (active) {")
+    for b in range(nb):
+        for r_ in range(4):
+            out.append(f"          if ({16 * b + r_} + 4 * kslot < {nij}) amax = fmaxf(amax, mag_{b}[{r_}]);")
+    out.append("        }")
+    out.append("        amax = eip::wave_max_f(cmax * amax);")
+    out.append("        if (lane == 0) *flag = !(amax < (float) a.flag_threshold);   // NaN flags too")
+    out.append("    }")
+    out.append("    if (!active) return;")
+    out.append("    const int col = eip::ro(JD(ks_col))[ksh];")
+    out.append("    const int I = eip::ro(JD(sp_i))[sp], J = eip::ro(JD(sp_j))[sp];")
+    out.append("    __attribute__((address_space(1))) double* __restrict__ o = (__attribute__((address_space(1))) double*) JD(out) + (JD(sp_out_base) ? eip::ro(JD(sp_out_base))[sp] : 0LL);")
+    out.append("    const long long ld = JD(sp_ld) ? (long long) eip::ro(JD(sp_ld))[sp] : JD(ld);")
+    out.append(f"    const long long base = (long long) sp * {nij};")
+    nj = n_out(lb, spherical)
+    for b in range(nb):
+        for r_ in range(4):
+            out.append(f"    {{ const int ij = {16 * b + r_} + 4 * kslot;")
+            out.append(f"      const int mi = ij / {nj}, mj = ij - mi * {nj};")
+            out.append(f"      if (ij < {nij} && (a.layout != 0 || I != J || mi >= mj)) {{")
+            out.append("        const long long row = (a.layout == 0 ? eip::packed_row(I + mi, J + mj) : base + ij) * ld + col;")
+            for m in ms:
+                out.append(f"        if (a.accumulate) o[row + {m}] += (double) acc_{m}_{b}[{r_}]; else o[row + {m}] = (double) acc_{m}_{b}[{r_}];")
             out.append("      }")
             out.append("    }")
     out.append("#undef JD")

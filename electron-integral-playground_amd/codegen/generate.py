@@ -12,8 +12,9 @@ import os
 import sys
 
 from . import boys_table
-from .gen_kernels import (emit_j2c, emit_j3c, emit_j3c_mfma, emit_nuc, emit_pair_dense, emit_pairE, m_chunks,
-                          mfma_chunks, mfma_dense_width, pair_layout)
+from .gen_kernels import (bf16_chunks, bf16_dense_width, emit_j2c, emit_j3c, emit_j3c_bf16, emit_j3c_mfma, emit_nuc,
+                          emit_pair_bf16, emit_pair_dense, emit_pairE, m_chunks, mfma_chunks, mfma_dense_width,
+                          pair_layout)
 
 SPEC_MAX_L = 3
 SPEC_MAX_AUX_L = 4
@@ -26,6 +27,10 @@ VARIANT_BUDGETS = (200, 240, 160, 120, 320, 280)
 # rows: budgets of live Hermite integrals per chunk (the accumulators sit in the matrix-core registers).
 MFMA_PAIR_CLASSES = ((1, 1), (1, 2), (2, 2), (1, 3), (2, 3), (3, 3))
 MFMA_R_BUDGETS = (84, 112, 160, 220)
+# Float32-tier variants on the bf16 matrix cores (split operands, appended after the float64 matrix-core
+# ones; selectable for the float32 tier only): budgets of live vector registers per chunk.
+BF16_PAIR_CLASSES = ((1, 1), (0, 2), (1, 2), (0, 3), (2, 2), (1, 3), (2, 3), (3, 3))
+BF16_BUDGETS = (256, 384, 512)
 
 HEADER = """// Generated by codegen/generate.py -- do not edit.
 #include "../md_device.hpp"
@@ -107,13 +112,31 @@ def generate(out_dir, shrink=None):
     disp.append("    }\n    return -1;\n}\n")
     reg.append("int eip_pair_dense_width(int la, int lb);")
 
+    # ---- split bfloat16 pair matrices for the float32-tier matrix-core kernels ----
+    src = [HEADER]
+    cases = []
+    for la, lb in BF16_PAIR_CLASSES:
+        name, code = emit_pair_bf16(la, lb)
+        src.append(code)
+        cases.append(f"        case {la * 10 + lb}: hipLaunchKernelGGL({name}, dim3((n_pp + 255) / 256), dim3(256), 0, s, E, (eip::bf8*) D, n_pp); return 0;")
+    src.append("int eip_launch_pairB(int la, int lb, const double* E, void* D, int n_pp, hipStream_t s)\n{\n"
+               "    switch (la * 10 + lb) {\n" + "\n".join(cases) + "\n    }\n    return -1;\n}\n")
+    _write(os.path.join(out_dir, "pairB.hip"), "\n".join(src))
+    reg.append("int eip_launch_pairB(int la, int lb, const double* E, void* D, int n_pp, hipStream_t s);")
+    disp.append("int eip_pair_bf16_width(int la, int lb)\n{\n    switch (la * 10 + lb) {")
+    for la, lb in BF16_PAIR_CLASSES:
+        disp.append(f"        case {la * 10 + lb}: return {bf16_dense_width(la, lb)};")
+    disp.append("    }\n    return -1;\n}\n")
+    reg.append("int eip_pair_bf16_width(int la, int lb);")
+
     # ---- three-centre ----
     # Each class gets up to len(VARIANT_BUDGETS) chunk plans (register budgets); variant 0 is the
     # default, the others are candidates for the autotuner.  Identical chunks are emitted once.
-    plan_rows = []          # (la, lb, lc, variant, chunk, m0, m1, matrix cores)
+    plan_rows = []          # (la, lb, lc, variant, chunk, m0, m1, kind: 0 vector, 1 f64 matrix cores, 2 bf16 matrix cores)
     for la, lb in pair_classes:
         plans = {}
         mplans = {}
+        bplans = {}
         for lc in range(SPEC_MAX_AUX_L + 1):
             cls_plans = []
             for vi, budget in enumerate(VARIANT_BUDGETS):
@@ -133,6 +156,15 @@ def generate(out_dir, shrink=None):
                 for k, plan in enumerate(mplans[lc]):
                     for ci, (m0, m1) in enumerate(plan):
                         plan_rows.append((la, lb, lc, len(cls_plans) + k, ci, m0, m1, 1))
+            bplans[lc] = []
+            if (la, lb) in BF16_PAIR_CLASSES:
+                for budget in BF16_BUDGETS:
+                    plan = bf16_chunks(la, lb, lc, budget)
+                    if plan not in bplans[lc]:
+                        bplans[lc].append(plan)
+                for k, plan in enumerate(bplans[lc]):
+                    for ci, (m0, m1) in enumerate(plan):
+                        plan_rows.append((la, lb, lc, len(cls_plans) + len(mplans[lc]) + k, ci, m0, m1, 2))
         if (la, lb) in MFMA_PAIR_CLASSES:
             src = [HEADER]
             cases = []
@@ -146,6 +178,20 @@ def generate(out_dir, shrink=None):
             src.append(f"int {fn}(int lc, int m0, int m1, const J3cArgs& a, int nblocks, hipStream_t s)\n{{\n"
                        "    switch (lc * 10000 + m0 * 100 + m1) {\n" + "\n".join(cases) + "\n    }\n    return -1;\n}\n")
             _write(os.path.join(out_dir, f"j3cm_{la}{lb}.hip"), "\n".join(src))
+            reg.append(f"int {fn}(int lc, int m0, int m1, const J3cArgs& a, int nblocks, hipStream_t s);")
+        if (la, lb) in BF16_PAIR_CLASSES:
+            src = [HEADER]
+            cases = []
+            for lc, cls_plans in bplans.items():
+                for m0, m1 in sorted({c for plan in cls_plans for c in plan}):
+                    name, code = emit_j3c_bf16(la, lb, lc, m0, m1)
+                    src.append(code)
+                    cases.append(f"        case {lc * 10000 + m0 * 100 + m1}: if (a.jobs) hipLaunchKernelGGL({name}_jobs, dim3(nblocks), dim3(256), 0, s, a);"
+                                 f" else hipLaunchKernelGGL({name}, dim3(nblocks), dim3(256), 0, s, a); return 0;")
+            fn = f"eip_launch_j3cb_{la}{lb}"
+            src.append(f"int {fn}(int lc, int m0, int m1, const J3cArgs& a, int nblocks, hipStream_t s)\n{{\n"
+                       "    switch (lc * 10000 + m0 * 100 + m1) {\n" + "\n".join(cases) + "\n    }\n    return -1;\n}\n")
+            _write(os.path.join(out_dir, f"j3cb_{la}{lb}.hip"), "\n".join(src))
             reg.append(f"int {fn}(int lc, int m0, int m1, const J3cArgs& a, int nblocks, hipStream_t s);")
         for prec in ("f64", "f32"):
             src = [HEADER]
@@ -184,8 +230,16 @@ def generate(out_dir, shrink=None):
     disp.append("int eip_launch_j3c(int la, int lb, int lc, int variant, int chunk, int f32, const J3cArgs& a, int nblocks, hipStream_t s)\n{")
     disp.append("    int m0, m1;")
     disp.append("    if (eip_j3c_chunk(la, lb, lc, variant, chunk, &m0, &m1) != 0) return -1;")
+    disp.append("    if (eip_j3c_variant_mfma(la, lb, lc, variant) == 2) {")
+    disp.append("        if (!f32) return -1;                   // bf16 matrix-core kernels are the float32 tier only")
+    disp.append("        switch (la * 10 + lb) {")
+    for la, lb in BF16_PAIR_CLASSES:
+        disp.append(f"            case {la * 10 + lb}: return eip_launch_j3cb_{la}{lb}(lc, m0, m1, a, nblocks, s);")
+    disp.append("        }")
+    disp.append("        return -1;")
+    disp.append("    }")
     disp.append("    if (eip_j3c_variant_mfma(la, lb, lc, variant) == 1) {")
-    disp.append("        if (f32) return -1;                    // matrix-core kernels are float64 only")
+    disp.append("        if (f32) return -1;                    // float64 matrix-core kernels")
     disp.append("        switch (la * 10 + lb) {")
     for la, lb in MFMA_PAIR_CLASSES:
         disp.append(f"            case {la * 10 + lb}: return eip_launch_j3cm_{la}{lb}(lc, m0, m1, a, nblocks, s);")

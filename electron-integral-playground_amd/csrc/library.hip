@@ -322,6 +322,28 @@ int g_variant[2][EIP_SPEC_MAX_L + 1][EIP_SPEC_MAX_L + 1][EIP_SPEC_MAX_AUX_L + 1]
 
 inline int current_variant(int la, int lb, int lc, int f32) { return g_variant[f32 ? 1 : 0][la][lb][lc]; }
 
+// Largest grid of one three-centre launch (work items = workgroups of up to 256 threads): longer item
+// lists are launched in slices [item_begin, item_end), keeping every grid far inside the runtime's limit
+// of 2^32 threads per dimension.
+constexpr int kMaxItemsPerLaunch = 1 << 22;
+
+// Every chunk kernel of variant v of a class over items [0, n_items), in slices of kMaxItemsPerLaunch.
+// With flags_out set, chunk c of the float32 tier writes its flags at slot 4 c of each item's row.
+int launch_j3c_items(int la, int lb, int lc, int v, int f32, J3cArgs a, int n_items, hipStream_t s)
+{
+    const int nchunks = eip_j3c_nchunks(la, lb, lc, v);
+    for (int c = 0; c < nchunks; ++c) {
+        a.flag_slot = 4 * c;
+        for (int b0 = 0; b0 < n_items; b0 += kMaxItemsPerLaunch) {
+            const int n = std::min(kMaxItemsPerLaunch, n_items - b0);
+            a.item_begin = b0;
+            a.item_end = b0 + n;
+            if (eip_launch_j3c(la, lb, lc, v, c, f32, a, n, s) != 0) return 3;
+        }
+    }
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -549,10 +571,7 @@ int three_center_impl(CallMem& mem, hipStream_t s, int i_L, int j_L, int k_L, co
             a.d.ld = ld;
             a.layout = 1;
             a.omega = omega;
-            const int v = current_variant(i_L, j_L, k_L, 0);
-            const int nchunks = eip_j3c_nchunks(i_L, j_L, k_L, v);
-            for (int c = 0; c < nchunks; ++c)
-                if (eip_launch_j3c(i_L, j_L, k_L, v, c, 0, a, nb * kblocks, s) != 0) return 3;
+            if (launch_j3c_items(i_L, j_L, k_L, current_variant(i_L, j_L, k_L, 0), 0, a, nb * kblocks, s) != 0) return 3;
             EIP_CHECK(hipGetLastError());
             EIP_CHECK(hipStreamSynchronize(s));        // 'hit' is read by the copy above until here
         } else {
@@ -708,9 +727,35 @@ int eip_three_center_chunk(int i_L, int j_L, int k_L, int variant, int chunk, in
 int eip_set_three_center_variant(int i_L, int j_L, int k_L, int use_f32, int variant)
 {
     if (!spec_3c(i_L, j_L, k_L) || variant < 0 || variant >= eip_j3c_nvariants(i_L, j_L, k_L)) return 1;
-    if (use_f32 && eip_j3c_variant_mfma(i_L, j_L, k_L, variant) == 1) return 1;   // float64-only variant
+    const int kind = eip_j3c_variant_mfma(i_L, j_L, k_L, variant);
+    if ((use_f32 && kind == 1) || (!use_f32 && kind == 2)) return 1;   // variant of the other tier only
     g_variant[use_f32 ? 1 : 0][i_L][j_L][k_L] = variant;
     return 0;
+}
+
+int eip_three_center_f32_uses_bf16(int i_L, int j_L, int k_L)
+{
+    return spec_3c(i_L, j_L, k_L) ? (eip_j3c_variant_mfma(i_L, j_L, k_L, current_variant(i_L, j_L, k_L, 1)) == 2) : 0;
+}
+
+int eip_three_center_flag_stride(int i_L, int j_L, int k_L)
+{
+    if (!spec_3c(i_L, j_L, k_L)) return -1;
+    return 4 * eip_j3c_nchunks(i_L, j_L, k_L, current_variant(i_L, j_L, k_L, 1));
+}
+
+int eip_pair_matrix_bf16_width(int i_L, int j_L)
+{
+    if (!(i_L >= 0 && j_L >= 0 && i_L <= j_L && j_L <= EIP_SPEC_MAX_L)) return -1;
+    return eip_pair_bf16_width(i_L, j_L);
+}
+
+int eip_pair_matrix_bf16(int i_L, int j_L, const double* d_E, int n_pp, void* d_EB, void* stream)
+{
+    if (eip_pair_matrix_bf16_width(i_L, j_L) < 0) return 1;
+    if (n_pp <= 0) return 0;
+    if (eip_launch_pairB(i_L, j_L, d_E, d_EB, n_pp, (hipStream_t) stream) != 0) return 3;
+    return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
 int eip_three_center_variant_mfma(int i_L, int j_L, int k_L, int variant)
@@ -765,9 +810,8 @@ int eip_three_center_items(int i_L, int j_L, int k_L, int use_f32, const int* d_
                   d_ka_coef, n_ks, d_out, ld, nullptr, nullptr};
     a.layout = layout;
     a.omega = omega;
-    const int v = current_variant(i_L, j_L, k_L, use_f32);
-    for (int c = 0; c < eip_j3c_nchunks(i_L, j_L, k_L, v); ++c)
-        if (eip_launch_j3c(i_L, j_L, k_L, v, c, use_f32, a, n_items, (hipStream_t) stream) != 0) return 3;
+    if (launch_j3c_items(i_L, j_L, k_L, current_variant(i_L, j_L, k_L, use_f32), use_f32, a, n_items,
+                         (hipStream_t) stream) != 0) return 3;
     return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
@@ -789,9 +833,8 @@ int eip_three_center_items_batched(int i_L, int j_L, int k_L, int use_f32, const
                   d_ka_coef, n_ks, d_out, 0, d_sp_out_base, d_sp_ld};
     a.layout = 0;
     a.omega = omega;
-    const int v = current_variant(i_L, j_L, k_L, use_f32);
-    for (int c = 0; c < eip_j3c_nchunks(i_L, j_L, k_L, v); ++c)
-        if (eip_launch_j3c(i_L, j_L, k_L, v, c, use_f32, a, n_items, (hipStream_t) stream) != 0) return 3;
+    if (launch_j3c_items(i_L, j_L, k_L, current_variant(i_L, j_L, k_L, use_f32), use_f32, a, n_items,
+                         (hipStream_t) stream) != 0) return 3;
     return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
@@ -813,9 +856,33 @@ int eip_three_center_work(int i_L, int j_L, int k_L, int use_f32, const int* d_i
     a.layout = layout;
     a.accumulate = accumulate;
     a.omega = omega;
-    const int v = current_variant(i_L, j_L, k_L, use_f32);
-    for (int c = 0; c < eip_j3c_nchunks(i_L, j_L, k_L, v); ++c)
-        if (eip_launch_j3c(i_L, j_L, k_L, v, c, use_f32, a, n_items, (hipStream_t) stream) != 0) return 3;
+    if (launch_j3c_items(i_L, j_L, k_L, current_variant(i_L, j_L, k_L, use_f32), use_f32, a, n_items,
+                         (hipStream_t) stream) != 0) return 3;
+    return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int eip_three_center_work_checked(int i_L, int j_L, int k_L, int pass, const int* d_items, const int* d_items_ks_end,
+                                  const int* d_items_job, int n_items, const void* d_jobs, int layout, int* d_flags,
+                                  int flag_stride, double threshold, double omega, void* stream)
+{
+    if (!spec_3c(i_L, j_L, k_L)) return 1;
+    if (n_items <= 0) return 0;
+    if (!d_items || !d_jobs || !d_flags || (layout != 0 && layout != 1) || (pass != 0 && pass != 1)) return 4;
+    if (flag_stride < eip_three_center_flag_stride(i_L, j_L, k_L) || !(threshold > 0.0)) return 4;
+    if (eip_three_center_f32_uses_bf16(i_L, j_L, k_L) != 1) return 4;   // the flags come from bf16 kernels
+    J3cArgs a{};
+    a.items = d_items;
+    a.items_ks_end = d_items_ks_end;
+    a.items_job = d_items_job;
+    a.jobs = static_cast<const J3cData*>(d_jobs);
+    a.layout = layout;
+    a.omega = omega;
+    a.flag_stride = flag_stride;
+    a.flag_threshold = threshold;
+    if (pass == 0) a.flags_out = d_flags;
+    else a.flags_in = d_flags;
+    if (launch_j3c_items(i_L, j_L, k_L, current_variant(i_L, j_L, k_L, pass == 0), pass == 0, a, n_items,
+                         (hipStream_t) stream) != 0) return 3;
     return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 

@@ -60,6 +60,14 @@ struct J3cArgs {
     int layout;               // 0: rows = packed lower triangle (i >= j); 1: rows = sp * nij + ij
     int accumulate;           // 0: out = integral; 1: out += integral
     double omega;
+    // Checked mixed precision (both nullptr otherwise).  A bf16 matrix-core (float32-tier) kernel given
+    // flags_out stores, for its item, flags_out[item * flag_stride + flag_slot + w] = 1 when some output
+    // of wave w has sum_m sum_k |E_k| |V_k,m| >= flag_threshold (the scale of its float32 rounding), else
+    // 0, for w < 4; a float64 kernel given flags_in skips items whose flag_stride flags are all zero.
+    int* flags_out;
+    const int* flags_in;
+    int flag_stride, flag_slot;
+    double flag_threshold;
 };
 
 // Nuclear attraction: lanes over point charges, output [(sp * nij + ij) * ld + charge].

@@ -17,6 +17,10 @@ namespace eip {
 
 // Four doubles: the accumulator tile slice one lane holds for v_mfma_f64_16x16x4f64.
 typedef double v4d __attribute__((ext_vector_type(4)));
+// Four floats: the accumulator slice of v_mfma_f32_16x16x32_bf16; eight bfloat16: one lane's operand.
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u4 __attribute__((ext_vector_type(4)));
 
 constexpr double kTwoPi25 = 34.986836655249725;     // 2 pi^(5/2)
 constexpr double kTwoPi = 6.283185307179586;        // 2 pi
@@ -159,6 +163,53 @@ __device__ __forceinline__ void boys32(const float T, float* __restrict__ F)
     }
 }
 
+// Split of eight floats into P bfloat16 pieces by truncation: piece 0 keeps the top 16 bits of x (sign,
+// exponent, 7 mantissa bits), piece k the top 16 bits of what the earlier pieces leave; every remainder is
+// exact in float32, so three pieces represent any float32 value exactly (each piece takes at least 8 of
+// its 24 significant bits).  h[k] holds piece k of the eight values, element v in lane v of the vector.
+template <int P>
+__device__ __forceinline__ void split_bf16(const float (&x)[8], bf8* __restrict__ h)
+{
+    unsigned int r[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) r[v] = __float_as_uint(x[v]);
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        u4 w;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) w[q] = (r[2 * q] >> 16) | (r[2 * q + 1] & 0xffff0000u);
+        h[k] = __builtin_bit_cast(bf8, w);
+        if (k + 1 < P) {
+#pragma unroll
+            for (int v = 0; v < 8; ++v)
+                r[v] = __float_as_uint(__uint_as_float(r[v]) - __uint_as_float(r[v] & 0xffff0000u));
+        }
+    }
+}
+
+template <int P>
+__device__ __forceinline__ void split_bf16(const double (&x)[8], bf8* __restrict__ h)
+{
+    float f[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) f[v] = (float) x[v];
+    split_bf16<P>(f, h);
+}
+
+// |x| of eight bfloat16 values (sign bits cleared).
+__device__ __forceinline__ bf8 bf16_abs(bf8 x)
+{
+    return __builtin_bit_cast(bf8, __builtin_bit_cast(u4, x) & 0x7fff7fffu);
+}
+
+// Largest value of a non-negative float over the 64 lanes of the wave.
+__device__ __forceinline__ float wave_max_f(float v)
+{
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
 // Largest value of v over the 64 lanes of the wave (butterfly over lane-index bits).
 __device__ __forceinline__ int wave_max(int v)
 {
@@ -188,6 +239,15 @@ template <class T>
 __device__ __forceinline__ cptr<T> ro(const T* p)
 {
     return (cptr<T>) p;
+}
+
+// Whether any of the n flags of an item is set (the float32 pass's verdict for the item's waves and
+// component chunks: some |value| reached the threshold, so the float64 pass recomputes the item).
+__device__ __forceinline__ bool item_flagged(const int* __restrict__ f, int n)
+{
+    int any = 0;
+    for (int k = 0; k < n; ++k) any |= ro(f)[k];
+    return any != 0;
 }
 
 // Row of the i >= j packed lower triangle holding element (i, j) of a symmetric matrix.

@@ -18,6 +18,8 @@ import ctypes
 from dataclasses import dataclass, field
 from typing import Dict, List, Tuple
 
+import time as _time
+
 import numpy as np
 
 from . import library
@@ -174,6 +176,29 @@ def _dense_pair_matrix(lib, la, lb, E, device):
     rc = lib.eip_pair_matrix_dense(la, lb, _ptr(E), int(E.shape[0]), _ptr(D), _stream_handle(None))
     if rc != 0:
         raise RuntimeError(f"eip_pair_matrix_dense({la},{lb}) failed with {rc}")
+    torch.cuda.current_stream(device).synchronize()
+    return D
+
+
+def aux_transform_cmax(lc):
+    """max_m sum_s |c_m(s)| of the spherical auxiliary transform of angular momentum lc (coefficient
+    sqrt(r_m) q_s of output m on Hermite index s; Hobson's theorem: |s| = lc)."""
+    from .codegen.tables import aux_transform
+    return max(sum(abs(float(q)) for q, _ in terms.values()) * float(r) ** 0.5
+               for r, terms in aux_transform(lc, True))
+
+
+def _bf16_pair_matrix(lib, la, lb, E, device):
+    """The split bfloat16 pair matrix of the float32-tier matrix-core kernels from the packed float64 one
+    (eip_pair_matrix_bf16), on the current stream, which is then synchronised."""
+    import torch
+    w = lib.eip_pair_matrix_bf16_width(la, lb)
+    if w < 0:
+        raise ValueError(f"pair class ({la},{lb}) has no bf16 matrix-core kernels")
+    D = torch.empty((E.shape[0], w), dtype=torch.bfloat16, device=device)
+    rc = lib.eip_pair_matrix_bf16(la, lb, _ptr(E), int(E.shape[0]), _ptr(D), _stream_handle(None))
+    if rc != 0:
+        raise RuntimeError(f"eip_pair_matrix_bf16({la},{lb}) failed with {rc}")
     torch.cuda.current_stream(device).synchronize()
     return D
 
@@ -386,11 +411,15 @@ class ThreeCenterPlan:
         self._zero = None
         self._job_tables = {}
         self._dense = {}
-        # dense pair matrices of the classes whose float64 kernels run on the matrix cores, built here
-        # (on the caller's stream) rather than at a first launch
+        self._bf16 = {}
+        self._flags = {}
+        # dense pair matrices of the classes whose float64 kernels run on the matrix cores (and split
+        # bfloat16 ones where the float32 kernels do), built here (on the caller's stream) rather than at
+        # a first launch
         for pi in range(len(self.pairs)):
             for ai in range(len(self.aux)):
                 self.kernel_E(pi, ai)
+                self.kernel_E32(pi, ai)
         torch.cuda.current_stream().synchronize()
 
     def dense_E(self, pi):
@@ -409,6 +438,24 @@ class ThreeCenterPlan:
         if self.lib.eip_three_center_uses_mfma(pc.la, pc.lb, ac.lc) == 1:
             return self.dense_E(pi)
         return pc.dev["E"]
+
+    def bf16_E(self, pi):
+        """Split bfloat16 pair matrix of pair class pi (the float32-tier matrix-core kernels' layout),
+        built on first use and kept."""
+        d = self._bf16.get(pi)
+        if d is None:
+            pc = self.pairs[pi]
+            d = _bf16_pair_matrix(self.lib, pc.la, pc.lb, pc.dev["E"], self.device)
+            self._bf16[pi] = d
+        return d
+
+    def kernel_E32(self, pi, ai):
+        """The pair matrix the current float32 kernels of class (pi, ai) read: the split bfloat16 one for
+        bf16 matrix-core variants, the packed float32 one otherwise."""
+        pc, ac = self.pairs[pi], self.aux[ai]
+        if self.lib.eip_three_center_f32_uses_bf16(pc.la, pc.lb, ac.lc) == 1:
+            return self.bf16_E(pi)
+        return pc.dev["E32"]
 
     def _zero_tables(self):
         """(rows, spans) of the packed rows no kernel writes (shell pairs screened out entirely), which
@@ -495,7 +542,7 @@ class ThreeCenterPlan:
         E = dp["E"] if use_f32 else self.kernel_E(pi, ai)
         rc = self.lib.eip_three_center_items(
             pc.la, pc.lb, ac.lc, int(use_f32), _ptr(it), _vp(0), it.shape[0], _ptr(dp["off"]), _ptr(dp["si"]),
-            _ptr(dp["sj"]), _ptr(dp["P"]), _ptr(E), _ptr(dp["E32"]), _vp(0), _ptr(da["off"]),
+            _ptr(dp["sj"]), _ptr(dp["P"]), _ptr(E), _ptr(self.kernel_E32(pi, ai)), _vp(0), _ptr(da["off"]),
             _ptr(da["col"]), _ptr(da["C"]), _ptr(da["c"]), ac.n_ks, _ptr(out), _c_ll(self.n_aux), 0,
             _c_double(self.omega), _stream_handle(stream))
         if rc != 0:
@@ -572,10 +619,11 @@ class ThreeCenterPlan:
     def job_words(self, pi, ai, out_ptr, ld, perm=None):
         """The 16 words of one eip_j3c_job entry: pair class pi, auxiliary class ai, output."""
         dp, da = self.pairs[pi].dev, self.aux[ai].dev
-        # E: what the float64 kernels of the class read (dense for matrix-core variants); the float32
-        # kernels read E32
+        # E: what the float64 kernels of the class read (dense for matrix-core variants); E32: what the
+        # float32 kernels read (split bfloat16 for bf16 matrix-core variants)
         return [dp["off"].data_ptr(), dp["si"].data_ptr(), dp["sj"].data_ptr(), dp["P"].data_ptr(),
-                self.kernel_E(pi, ai).data_ptr(), dp["E32"].data_ptr(), perm.data_ptr() if perm is not None else 0,
+                self.kernel_E(pi, ai).data_ptr(), self.kernel_E32(pi, ai).data_ptr(),
+                perm.data_ptr() if perm is not None else 0,
                 da["off"].data_ptr(), da["col"].data_ptr(), da["C"].data_ptr(), da["c"].data_ptr(),
                 self.aux[ai].n_ks, int(out_ptr), int(ld), 0, 0]
 
@@ -601,8 +649,101 @@ class ThreeCenterPlan:
         if rc != 0:
             raise RuntimeError(f"eip_three_center_work({pc.la},{pc.lb},{ac.lc}, f32={use_f32}) failed with {rc}")
 
+    def flag_buffer(self, n, stream=None):
+        """Item flags of checked mixed precision: at least n int32 per stream, grown on demand and kept
+        (a launch that reads them may still be in flight when the next one is queued)."""
+        key = _stream_handle(stream).value
+        f = self._flags.get(key)
+        if f is None or f.numel() < n:
+            f = self.torch.empty(max(n, 1 << 20), dtype=self.torch.int32, device=self.device)
+            self._flags[key] = f
+        return f
+
+    def launch_checked(self, pi, ai, items, jobs, layout, stream=None):
+        """Checked mixed precision over ``items``: the float32 kernels store every item and flag the items
+        where some |value| reached the threshold; the float64 kernels then recompute exactly those."""
+        if items is None or items.shape[0] == 0:
+            return
+        pc, ac = self.pairs[pi], self.aux[ai]
+        if self.lib.eip_three_center_f32_uses_bf16(pc.la, pc.lb, ac.lc) != 1:
+            # the flags come from the bf16 matrix-core kernels: a class whose float32 tier runs on the
+            # vector kernels is evaluated in float64
+            self.launch_work(pi, ai, items, jobs, layout=layout, stream=stream)
+            return
+        stride = self.lib.eip_three_center_flag_stride(pc.la, pc.lb, ac.lc)
+        n = int(items.shape[0])
+        flags = self.flag_buffer(n * stride, stream)
+        for pass_ in (0, 1):
+            rc = self.lib.eip_three_center_work_checked(
+                pc.la, pc.lb, ac.lc, pass_, _ptr(items), _vp(0), _vp(0), n, _ptr(jobs), int(layout), _ptr(flags),
+                int(stride), _c_double(self.mixed_stats["threshold"]), _c_double(self.omega), _stream_handle(stream))
+            if rc != 0:
+                raise RuntimeError(f"eip_three_center_work_checked({pc.la},{pc.lb},{ac.lc}, pass {pass_}) failed with {rc}")
+        tally = self.__dict__.get("checked_tally")
+        if tally is not None:
+            # diagnostics (off unless a dict is installed): triplets of all and of the flagged items
+            torch = self.torch
+            f = flags[:n * stride].view(n, stride).ne(0).any(1)
+            n_pp_sp = self.__dict__.setdefault("_n_pp_sp", {})
+            if pi not in n_pp_sp:
+                n_pp_sp[pi] = torch.as_tensor(np.diff(np.asarray(pc.sp_pp_off, dtype=np.int64)), device=self.device)
+            it = items.long()
+            w = n_pp_sp[pi][it[:, 0]] * (ac.n_ks - it[:, 1]).clamp(max=64)
+            key = (pc.la, pc.lb, ac.lc)
+            t = tally.setdefault(key, [torch.zeros((), dtype=torch.int64, device=self.device),
+                                       torch.zeros((), dtype=torch.int64, device=self.device)])
+            t[0] += w.sum()
+            t[1] += (w * f).sum()
+
+    def checked_fraction(self, pi, ai, items, stream=None):
+        """Fraction of ``items`` the last checked launch of class (pi, ai) on ``stream`` flagged for
+        float64 (reads the flag buffer; synchronises)."""
+        pc, ac = self.pairs[pi], self.aux[ai]
+        stride = self.lib.eip_three_center_flag_stride(pc.la, pc.lb, ac.lc)
+        n = int(items.shape[0])
+        f = self.flag_buffer(n * stride, stream)[:n * stride].view(n, stride)
+        return float((f != 0).any(1).float().mean().item()) if n else 0.0
+
     # -- mixed precision ---------------------------------------------------------------------------
-    def setup_mixed(self, pair_list, q_aux, threshold=1e-6, granularity="item", aux_order="natural"):
+    def pair_esum(self, pi):
+        """[n_pp][la + lb + 1] float32 of pair class pi: Esum[pp][n] = max_ij sum_{|t| = n} |E_pp[ij][t]|,
+        the pair matrix's row sums per Hermite level (from the float64 matrix; built on first use)."""
+        cache = self.__dict__.setdefault("_esum", {})
+        if pi in cache:
+            return cache[pi]
+        from .codegen.gen_kernels import pair_layout
+        torch = self.torch
+        pc = self.pairs[pi]
+        lay = pair_layout(pc.la, pc.lb, True)
+        nlev, nij = pc.la + pc.lb + 1, (2 * pc.la + 1) * (2 * pc.lb + 1)
+        group = torch.as_tensor([row * nlev + sum(t) for t, row in lay], dtype=torch.int64, device=self.device)
+        E = pc.dev["E"]
+        out = torch.empty((pc.n_pp, nlev), dtype=torch.float32, device=self.device)
+        step = 1 << 16
+        for c0 in range(0, pc.n_pp, step):
+            e = E[c0:c0 + step, :len(lay)].abs()
+            S = torch.zeros((e.shape[0], nij * nlev), dtype=torch.float64, device=self.device).index_add_(1, group, e)
+            out[c0:c0 + step] = S.view(-1, nij, nlev).amax(1).float()
+        cache[pi] = out
+        return out
+
+    def item_scales(self, pi, ai, items=None, stream=None):
+        """A-priori term scale (eip_three_center_scale) of the work items of class (pi, ai) -- by default
+        every item, in the order of ``self.items`` -- as a float32 tensor."""
+        torch = self.torch
+        pc, ac = self.pairs[pi], self.aux[ai]
+        items = self.items[(pi, ai)] if items is None else items
+        out = torch.empty(items.shape[0], dtype=torch.float32, device=self.device)
+        rc = self.lib.eip_three_center_scale(
+            pc.la, pc.lb, ac.lc, _ptr(items), int(items.shape[0]), _ptr(pc.dev["off"]), _ptr(pc.dev["P"]),
+            _ptr(self.pair_esum(pi)), _ptr(ac.dev["off"]), _ptr(ac.dev["C"]), _ptr(ac.dev["c"]), ac.n_ks,
+            _c_double(aux_transform_cmax(ac.lc)), _c_double(self.omega), _ptr(out), _stream_handle(stream))
+        if rc != 0:
+            raise RuntimeError(f"eip_three_center_scale({pc.la},{pc.lb},{ac.lc}) failed with {rc}")
+        return out
+
+    def setup_mixed(self, pair_list, q_aux, threshold=1e-6, granularity="item", aux_order="natural",
+                    estimate="schwarz"):
         """Split every primitive triplet between the float64 and float32 kernels.
 
         Bound of a triplet (primitive pair pp, auxiliary shell P): ub_pp Q_P, with ub_pp the reference's
@@ -626,8 +767,16 @@ class ThreeCenterPlan:
         longest-first; "qp" groups them by descending Q_P (more float32 items, scattered columns).
         Measured on the 200-atom workload: natural 1.06x the float64 stream, qp 0.88x.
         """
-        if granularity not in ("item", "pair"):
-            raise ValueError("granularity is 'item' or 'pair'")
+        if granularity not in ("item", "pair", "checked"):
+            raise ValueError("granularity is 'item', 'pair' or 'checked'")
+        if granularity == "checked":
+            return self.setup_checked(threshold)
+        if estimate not in ("schwarz", "envelope"):
+            raise ValueError("estimate is 'schwarz' or 'envelope'")
+        if estimate == "envelope":
+            if granularity != "item" or aux_order != "natural":
+                raise ValueError("the envelope estimate splits whole items in the natural auxiliary order")
+            return self.setup_envelope(threshold)
         from .pair_utility import _as_arrays
         torch = self.torch
         T = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=self.device)
@@ -684,10 +833,92 @@ class ThreeCenterPlan:
         self.mixed_stats["f32_triplets"] = all_trip - f64_trip
         return self.mixed_stats
 
+    def setup_envelope(self, threshold=1e-6):
+        """Mixed precision split on the a-priori term scale of every work item (``item_scales``): an
+        item runs in float32 when its scale -- a bound of the magnitude of every term summed into its
+        outputs, distance decay and the cancellation between a contracted shell pair's primitive pairs
+        included -- is below ``threshold`` and none of its shell pair's primitive pairs is diffuse
+        (p < DIFFUSE_PAIR_EXPONENT); otherwise in float64.  The float32 error of an item is then a few
+        float32 roundings of the threshold.  Items keep the plain path's order (natural auxiliary order),
+        so the split plugs into the item-granularity launches."""
+        torch = self.torch
+        dev = self.device
+        self.mixed = {}
+        self.mixed_aux_order = "natural"
+        self._mixed_generation = self.__dict__.get("_mixed_generation", 0) + 1
+        f64_total = torch.zeros((), dtype=torch.int64, device=dev)
+        all_total = torch.zeros((), dtype=torch.int64, device=dev)
+        t0 = _time.perf_counter()
+        for pi, pc in enumerate(self.pairs):
+            off_np = np.asarray(pc.sp_pp_off, dtype=np.int64)
+            diffuse_sp = np.zeros(pc.n_sp, dtype=bool)
+            dpp = np.flatnonzero(self._pair_p[pi] < DIFFUSE_PAIR_EXPONENT)
+            if dpp.size:
+                diffuse_sp[np.searchsorted(off_np, dpp, side="right") - 1] = True
+            diffuse_t = torch.as_tensor(diffuse_sp, device=dev)
+            n_pp_sp = torch.as_tensor(np.diff(off_np), device=dev)
+            for ai, ac in enumerate(self.aux):
+                items = self.items[(pi, ai)]
+                G = (ac.n_ks + 63) // 64
+                scale = self.item_scales(pi, ai, items)
+                sp = items[:, 0].long()
+                f32 = (scale < threshold) & ~diffuse_t[sp]
+                npp = n_pp_sp[sp]
+                kprim = torch.as_tensor(np.diff(ac.ks_prim_off.astype(np.int64)), device=dev)
+                gprim = torch.zeros(G, dtype=torch.int64, device=dev).index_add_(
+                    0, torch.arange(ac.n_ks, device=dev) // 64, kprim)
+                trip = npp * gprim[items[:, 1].long() // 64]
+                all_total += trip.sum()
+                f64_total += trip[~f32].sum()
+
+                def offsets(mask):
+                    per_sp = mask.reshape(pc.n_sp, G).sum(1)
+                    return np.concatenate([[0], np.cumsum(per_sp.cpu().numpy())]).astype(np.int64)
+
+                self.mixed[(pi, ai)] = dict(perm=None, G=G, it64=items[~f32], pp64=None, off64=offsets(~f32),
+                                            it32=items[f32], pp32=None, off32=offsets(f32), acc32=0,
+                                            cost64=npp[~f32], cost32=npp[f32])
+        f64_trip, all_trip = int(f64_total.item()), int(all_total.item())
+        self.mixed_stats = {"f64_triplets": f64_trip, "f32_triplets": all_trip - f64_trip, "threshold": threshold,
+                            "aux_order": "natural", "granularity": "item", "estimate": "envelope",
+                            "estimate_seconds": _time.perf_counter() - t0}
+        return self.mixed_stats
+
+    def setup_checked(self, threshold=1e-6):
+        """Checked mixed precision: every work item of a class whose float32 tier runs on the bf16
+        matrix cores is first evaluated there; the kernels also accumulate, per output, the term scale
+        sum_m sum_k |E_k| |V_k,m| (pair-matrix entries times the auxiliary side's Hermite sums, over every
+        primitive pair of the item) and flag each item where some output's scale reaches ``threshold``;
+        the float64 kernels then recompute exactly the flagged items (and every item of the other
+        classes).  The split follows the computed terms, with their distance decay, instead of a
+        distance-free bound, and a float32 item's error stays a few float32 roundings of the threshold
+        even where the primitive pairs of a contracted shell pair cancel."""
+        self.mixed = {(pi, ai): {"checked": True, "perm": None} for pi in range(len(self.pairs))
+                      for ai in range(len(self.aux))}
+        self.mixed_aux_order = "natural"
+        self._mixed_generation = self.__dict__.get("_mixed_generation", 0) + 1
+        trip = sum(self.primitive_triplets().values())
+        self.mixed_stats = {"f64_triplets": None, "f32_triplets": trip, "threshold": threshold,
+                            "aux_order": "natural", "granularity": "checked"}
+        return self.mixed_stats
+
+    @property
+    def checked(self):
+        return getattr(self, "mixed_stats", {}).get("granularity") == "checked"
+
     def run_mixed(self, out=None, stream=None, zero=True):
         """Mixed-precision (ij|P) into the packed float64 output (requires ``setup_mixed``)."""
         if out is None:
             out = self.new_output()
+        if self.checked:
+            keys = list(self.mixed)
+            jobs = self.job_table([self.job_words(pi, ai, out.data_ptr(), self.n_aux) for pi, ai in keys])
+            with _fork(stream, int(zero and self.zero_spans.shape[0] > 0)) as sides:
+                if sides:
+                    self.zero_unwritten(out, sides[0], ZERO_BLOCKS_BESIDE_COMPUTE)
+                for j, (pi, ai) in enumerate(keys):
+                    self.launch_checked(pi, ai, self.class_items(pi, ai), jobs[j], layout=0, stream=stream)
+            return out
         keys = list(self.mixed)
         jobs = self.job_table([self.job_words(pi, ai, out.data_ptr(), self.n_aux, self.mixed[(pi, ai)]["perm"])
                                for pi, ai in keys])
@@ -742,6 +973,9 @@ class ThreeCenterPlan:
             if not mixed:
                 self.launch_work(pi, ai, self._block_items(pi, ai, sp0, sp1)[0], jobs[ai], layout=1, stream=stream)
                 continue
+            if self.checked:
+                self.launch_checked(pi, ai, self._block_items(pi, ai, sp0, sp1)[0], jobs[ai], layout=1, stream=stream)
+                continue
             if self.mixed_aux_order == "natural":
                 it64, pp64 = self._block_items(pi, ai, sp0, sp1, "f64")
                 it32, pp32 = self._block_items(pi, ai, sp0, sp1, "f32")
@@ -790,7 +1024,8 @@ class ThreeCenterPlan:
     def prepare_blocks(self, blocks, mixed=False):
         """Build the dispatch tables ``run_block`` uses for every (pi, sp0, sp1) in ``blocks`` ahead of a
         timed or captured sequence (a first use builds them on the host and synchronises)."""
-        kinds = ("f64", "f32") if mixed and getattr(self, "mixed_aux_order", None) == "natural" else ()
+        kinds = (("f64", "f32") if mixed and getattr(self, "mixed_aux_order", None) == "natural" and not self.checked
+                 else ())
         for pi, sp0, sp1 in blocks:
             for ai in range(len(self.aux)):
                 for kind in ("plain",) + kinds:
@@ -1031,21 +1266,23 @@ RESOURCES_FILE = _pathlib.Path(__file__).resolve().parent / "lib" / "kernel_reso
 def _variant_kernels(lib, la, lb, lc, v, f32):
     m0, m1 = ctypes.c_int(), ctypes.c_int()
     names, c = [], 0
-    prefix = "j3cm" if lib.eip_three_center_variant_mfma(la, lb, lc, v) == 1 else "j3c"
+    kind = lib.eip_three_center_variant_mfma(la, lb, lc, v)
+    prefix = {1: "j3cm", 2: "j3cb"}.get(kind, "j3c")
+    suffix = "_f32" if f32 and kind == 0 else ""
     while lib.eip_three_center_chunk(la, lb, lc, v, c, ctypes.byref(m0), ctypes.byref(m1)) == 0:
-        names.append(f"{prefix}_{la}{lb}{lc}_{m0.value}_{m1.value}" + ("_f32" if f32 else ""))
+        names.append(f"{prefix}_{la}{lb}{lc}_{m0.value}_{m1.value}{suffix}")
         c += 1
     return names
 
 
 def usable_variants(lib, la, lb, lc, f32=False):
-    """Variants of a class none of whose kernels spill to scratch memory (variant 0 is always kept);
-    matrix-core variants are float64 only."""
+    """Variants of a class none of whose kernels spill to scratch memory (variant 0 is always kept) of
+    one tier: float64 matrix-core variants are float64 only, bf16 matrix-core variants float32 only."""
     from .build import spills
     res = _json.loads(RESOURCES_FILE.read_text()) if RESOURCES_FILE.exists() else {}
     out = []
     for v in range(max(0, lib.eip_three_center_nvariants(la, lb, lc))):
-        if f32 and lib.eip_three_center_variant_mfma(la, lb, lc, v) == 1:
+        if lib.eip_three_center_variant_mfma(la, lb, lc, v) == (1 if f32 else 2):
             continue
         names = _variant_kernels(lib, la, lb, lc, v, f32)
         if v == 0 or all(n in res and not spills(res[n]) for n in names):

@@ -83,19 +83,33 @@ int eip_set_three_center_variant(int i_L, int j_L, int k_L, int use_f32, int var
 
 /* Variants after the vector ones run the float64 contraction on the matrix cores
  * (v_mfma_f64_16x16x4f64: four primitive pairs x 16 auxiliary shells per instruction, one workgroup of
- * four waves per work item) and read the dense pair matrix below instead of the packed one.
- * eip_three_center_variant_mfma: 1 for such a variant, 0 for a vector variant, -1 for no such class;
- * eip_three_center_uses_mfma: 1 when the class's current float64 variant is one (then every E pointer
- * given for that class -- argument or eip_j3c_job -- must be the dense matrix).  Float32 variants never
- * are: setting one for use_f32 = 1 returns 1. */
+ * four waves per work item) and read the dense pair matrix below instead of the packed one.  After those
+ * come float32-tier variants on the bf16 matrix cores (v_mfma_f32_16x16x32_bf16 over operands split into
+ * three bfloat16 pieces, six products per element: float32-level accuracy; four primitive pairs x 8
+ * Hermite indices x 16 auxiliary shells per instruction), which read the split bfloat16 pair matrix below
+ * through the E32 pointer.
+ * eip_three_center_variant_mfma: 1 for a float64 matrix-core variant, 2 for a bf16 one, 0 for a vector
+ * variant, -1 for no such class; setting a variant of the other tier (1 with use_f32 = 1, 2 with
+ * use_f32 = 0) returns 1.  eip_three_center_uses_mfma: 1 when the class's current float64 variant is a
+ * matrix-core one (then every E pointer given for that class -- argument or eip_j3c_job -- must be the
+ * dense matrix); eip_three_center_f32_uses_bf16: 1 when its current float32 variant is a bf16 one (then
+ * every E32 pointer must be the split bfloat16 matrix). */
 int eip_three_center_variant_mfma(int i_L, int j_L, int k_L, int variant);
 int eip_three_center_uses_mfma(int i_L, int j_L, int k_L);
+int eip_three_center_f32_uses_bf16(int i_L, int j_L, int k_L);
 
 /* Doubles per primitive pair of the dense pair matrix [n_pp][n_t][16 * ceil(nij / 16)] of class
  * (i_L, j_L) (structural zeros stored, -1 when the class has no matrix-core kernels), and its
  * construction from the packed float64 matrix d_E. */
 int eip_pair_matrix_dense_width(int i_L, int j_L);
 int eip_pair_matrix_dense(int i_L, int j_L, const double* d_E, int n_pp, double* d_ED, void* stream);
+
+/* bfloat16 values per primitive pair of the split pair matrix
+ * [n_pp][3 pieces][ceil(n_t / 8)][ceil(nij / 16)][16 rows][8 Hermite indices] of class (i_L, j_L) (-1 when
+ * the class has no bf16 kernels), and its construction from the packed float64 matrix d_E: each entry,
+ * rounded to float32, is the exact sum of its three truncated bfloat16 pieces. */
+int eip_pair_matrix_bf16_width(int i_L, int j_L);
+int eip_pair_matrix_bf16(int i_L, int j_L, const double* d_E, int n_pp, void* d_EB, void* stream);
 
 /* Pair matrix (Hermite expansion contracted into real solid harmonics, pair coefficient folded in)
  * for n_pp primitive pairs: d_E[n_pp][nnz] float64 and, when d_E32 is not null, a float32 copy. */
@@ -161,6 +175,38 @@ int eip_three_center_work(int i_L, int j_L, int k_L, int use_f32, const int* d_i
  * 8 n bytes written, with one 8-byte load and store per lane.  No counterpart in the reference: it
  * calibrates the profiler's HBM byte counters (FETCH_SIZE, WRITE_SIZE) for the traffic figures. */
 int eip_copy_probe(const double* d_src, double* d_dst, long long n, void* stream);
+
+/* A-priori term scale of work items (d_items[w] = (shell pair, first auxiliary shell), 64 auxiliary shells
+ * each) of class (i_L j_L | k_L), for the mixed-precision split: d_scale[w] = cmax * max over the item's
+ * auxiliary shells P of sum over primitive pairs pp and primitives of P of
+ * sum_n d_pp_esum[pp][n] A_{n + k_L}, where d_pp_esum[pp][n] = max_ij sum_{|t| = n} |E_pp[ij][t]| (n <= i_L
+ * + j_L; from the float64 pair matrix), cmax = max_m sum_s |c_m(s)| of the auxiliary transform, and A_N the
+ * Hermite recursion with every term positive (A_0^(n) = |prefactor (2 zeta)^n F_n(T)|, A_N^(n) = (N - 1)
+ * A_{N-2}^(n+1) + |PQ| A_{N-1}^(n+1)): a bound of the magnitude of every term summed into the item's
+ * outputs, with their distance decay -- the scale of their float32 rounding.  Pair data as in J3cData
+ * (sp_pp_off, pp_P), auxiliary data as ks_prim_off / ka_C / ka_coef.  No counterpart in the reference (its
+ * split is the distance-free bound of pair_utility.py:95-114).  Returns 1 when i_L + j_L + k_L > 10, 4 on
+ * bad arguments. */
+int eip_three_center_scale(int i_L, int j_L, int k_L, const int* d_items, int n_items, const int* d_sp_pp_off,
+                           const double* d_pp_P, const float* d_pp_esum, const int* d_ks_prim_off,
+                           const double* d_ka_C, const double* d_ka_coef, int n_ks, double cmax, double omega,
+                           float* d_scale, void* stream);
+
+/* Checked mixed precision over the items of eip_three_center_work (all primitive pairs of each item).
+ * pass 0 runs the class's float32 kernels -- its current float32 variant must be a bf16 matrix-core one
+ * -- which store every item's results and, in d_flags[item * flag_stride ..], whether some output's term
+ * scale sum_m sum_k |E_k| |V_k,m| (the magnitude its float32 rounding is relative to, so cancellation
+ * between primitive pairs of a contracted shell pair counts) reached threshold (one flag per wave and
+ * component chunk); pass 1 runs the float64 kernels over the same items, and each item whose flags are
+ * all clear is skipped, so the float64 results replace exactly the flagged items.  Run pass 0 then pass 1
+ * on one stream with the same arguments.  flag_stride >= eip_three_center_flag_stride (4 x the chunk
+ * count of the current float32 variant; the flags need no clearing).  No counterpart in the reference
+ * (its three_center.cpp computes every triplet in float64).  Returns 4 on bad arguments or when the
+ * class's float32 variant is not a bf16 one. */
+int eip_three_center_flag_stride(int i_L, int j_L, int k_L);
+int eip_three_center_work_checked(int i_L, int j_L, int k_L, int pass, const int* d_items, const int* d_items_ks_end,
+                                  const int* d_items_job, int n_items, const void* d_jobs, int layout, int* d_flags,
+                                  int flag_stride, double threshold, double omega, void* stream);
 
 /* Zero rows d_rows[0..n_rows) (each ld doubles long) of d_out: the packed rows of screened-out shell
  * pairs, which no three-centre kernel writes. */
