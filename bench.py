@@ -194,6 +194,8 @@ COPY_PROBE_RUNS = 8
 # three-centre kernels of a plain FP64 pass: vector (j3c_) or matrix-core (j3cm_) chunk kernels, direct or
 # job-table forms; the float32 kernels carry an _f32 suffix and do not match
 J3C_F64_KERNEL = re.compile(r"^j3cm?_(\d)(\d)(\d)_\d+_\d+(_jobs)?$")
+# float32-tier kernels of a mixed pass: vector (_f32 suffix) or bf16 matrix-core (j3cb_)
+J3C_F32_KERNEL = re.compile(r"^(?:j3c_(\d)(\d)(\d)_\d+_\d+_f32|j3cb_(\d)(\d)(\d)_\d+_\d+)(_jobs)?$")
 
 
 def copy_probe():
@@ -224,7 +226,11 @@ def profile_plain_leg(args):
     mol, mol_s, arrays, pairs, aux, _ = slice_workload(args.slice_atoms)
     plan = ThreeCenterPlan(mol_s, pairs, aux)
     blocks = plan.row_blocks(BLOCK_BYTES)
-    plan.prepare_blocks(blocks)
+    mixed = bool(args.profile_mixed)
+    if mixed:
+        stats = plan.setup_mixed(arrays, None, threshold=args.threshold, estimate="envelope")
+        log(f"profile-mixed: split {stats}")
+    plan.prepare_blocks(blocks, mixed=mixed)
     rows = max(plan.block_rows(*b) for b in blocks)
     A = torch.empty((rows, mol_s.n_aux), dtype=torch.float64, device="cuda")
     stream = torch.cuda.current_stream()
@@ -232,17 +238,17 @@ def profile_plain_leg(args):
         "1 event pass")
     for _ in range(max(1, args.warmup)):
         for b in blocks:
-            plan.run_block(*b, A)
+            plan.run_block(*b, A, mixed=mixed)
     torch.cuda.synchronize()
     log("profile-plain: warmup done")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         for b in blocks:
-            plan.run_block(*b, A)
+            plan.run_block(*b, A, mixed=mixed)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
     log(f"profile-plain: {dt * 1e3:.1f} ms/step")
-    times = class_times(plan, blocks, A, False, stream)
+    times = class_times(plan, blocks, A, mixed, stream)
     trip = plan.primitive_triplets()
     out_bytes, in_bytes = {}, {}
     for pi, sp0, sp1 in blocks:
@@ -258,7 +264,8 @@ def profile_plain_leg(args):
     del A, plan
     torch.cuda.empty_cache()
     return {"profile_plain": {"passes": max(1, args.warmup) + args.steps + 1, "ms_per_step": dt * 1e3,
-                              "blocks_per_step": len(blocks), "classes": classes}}
+                              "blocks_per_step": len(blocks), "mode": "mixed" if mixed else "plain",
+                              "threshold": args.threshold if mixed else None, "classes": classes}}
 
 
 def _csv_rows(path):
@@ -266,9 +273,15 @@ def _csv_rows(path):
         return list(csv.DictReader(f))
 
 
-def _class_of(kernel):
-    m = J3C_F64_KERNEL.match(kernel.strip())
-    return None if m is None else "%s%s%s" % m.groups()[:3]
+def _class_of(kernel, tier="f64"):
+    if tier == "f64":
+        m = J3C_F64_KERNEL.match(kernel.strip())
+        return None if m is None else "%s%s%s" % m.groups()[:3]
+    m = J3C_F32_KERNEL.match(kernel.strip())
+    if m is None:
+        return None
+    g = m.groups()
+    return "%s%s%s" % (g[:3] if g[0] is not None else g[3:6])
 
 
 def trace_summary(stats_csv, run_json):
@@ -278,17 +291,35 @@ def trace_summary(stats_csv, run_json):
     passes = run["passes"]
     agg = {}
     for r in _csv_rows(stats_csv):
-        cls = _class_of(r.get("Name") or r.get("Kernel_Name") or "")
-        if cls is None:
-            continue
-        a = agg.setdefault(cls, [0, 0.0])
-        a[0] += int(r["Calls"])
-        a[1] += float(r["TotalDurationNs"]) * 1e-9
+        name = r.get("Name") or r.get("Kernel_Name") or ""
+        for tier in ("f64", "f32"):
+            cls = _class_of(name, tier)
+            if cls is None:
+                continue
+            a = agg.setdefault(cls, {"f64": [0, 0.0], "f32": [0, 0.0]})[tier]
+            a[0] += int(r["Calls"])
+            a[1] += float(r["TotalDurationNs"]) * 1e-9
+    if run.get("mode") == "mixed":
+        lines = [f"mixed passes per class: {passes} (threshold {run['threshold']:g}); bench ms/step "
+                 f"{run['ms_per_step']:.1f}", "",
+                 "| class | float64 dispatches | float64 s per pass | float32 dispatches | float32 s per pass | "
+                 "s per pass (trace) | s per pass (HIP events) | trace / events |",
+                 "|---|---|---|---|---|---|---|---|"]
+        for cls, t in sorted(agg.items(), key=lambda kv: -(kv[1]["f64"][1] + kv[1]["f32"][1])):
+            c = run["classes"].get(cls)
+            ev = c["event_s_per_pass"] if c else float("nan")
+            per = (t["f64"][1] + t["f32"][1]) / passes
+            lines.append(f"| j3c_{cls} | {t['f64'][0]} | {t['f64'][1] / passes:.5f} | {t['f32'][0]} | "
+                         f"{t['f32'][1] / passes:.5f} | {per:.5f} | {ev:.5f} | {per / ev:.3f} |")
+        return "\n".join(lines)
     lines = [f"plain passes per class: {passes}; bench ms/step {run['ms_per_step']:.1f}", "",
              "| class | dispatches | kernel s (trace) | s per pass (trace) | s per pass (HIP events) | "
              "trace / events | model TFLOP/s (trace) | frac of 78.6 |",
              "|---|---|---|---|---|---|---|---|"]
-    for cls, (calls, tot) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+    for cls, t in sorted(agg.items(), key=lambda kv: -kv[1]["f64"][1]):
+        calls, tot = t["f64"]
+        if not calls:
+            continue
         c = run["classes"].get(cls)
         ev = c["event_s_per_pass"] if c else float("nan")
         per = tot / passes
@@ -811,6 +842,8 @@ def main():
                     help="headline (or, with --plain80-only, the 80-atom block): only the plain FP64 passes and "
                          "one per-class event pass, printing the per-class figures the --trace-summary and "
                          "--pmc-summary tables use (the command profiled for them)")
+    ap.add_argument("--profile-mixed", action="store_true",
+                    help="with --profile-plain: the mixed-precision step (envelope split at --threshold) instead")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "RUN_JSON"),
                     help="write the per-class HBM traffic table from two rocprofv3 --pmc counter_collection.csv "
                          "files of a --profile-plain run whose printed line is in RUN_JSON, then exit")

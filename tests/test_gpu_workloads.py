@@ -297,14 +297,27 @@ def test_mixed_200_atoms_float64_and_mixed_vs_oracle():
         got64.append(A[:n].index_select(1, cols))
         got32.append(B[:n].index_select(1, cols))
     torch.cuda.synchronize()
-    _say(f"  streamed {len(blocks)} blocks (float64 and mixed)")
+    _say(f"  streamed {len(blocks)} blocks (float64 and mixed, Schwarz-type split)")
+    # the default split: the a-priori term-scale (envelope) estimate
+    stats_e = plan.setup_mixed(arrays, q, threshold=1e-6, estimate="envelope")
+    _say(f"  envelope split {stats_e}")
+    assert stats_e["f32_triplets"] > 10 * stats["f32_triplets"]
+    plan.prepare_blocks(blocks, mixed=True)
+    gotE = []
+    for bi, b in enumerate(blocks):
+        n = plan.block_rows(*b)
+        plan.run_block(*b, B, mixed=True)
+        gotE.append(B[:n].index_select(1, cols))
+    torch.cuda.synchronize()
+    _say(f"  streamed {len(blocks)} blocks (mixed, envelope split)")
     del A, B
     _free()
     ao = [_block_ao(plan.pairs[pi], sp0, sp1) for pi, sp0, sp1 in blocks]
     lib = oracle_lib()
     c0 = 0
-    worst64 = worst32 = 0.0
-    sq32, n_el = 0.0, 0
+    worst64 = worst32 = worstE = 0.0
+    sq32 = sqE = 0.0
+    n_el = 0
     n_arb = 0
     for lc, starts in sample.items():
         nk = 2 * lc + 1
@@ -320,17 +333,25 @@ def test_mixed_200_atoms_float64_and_mixed_vs_oracle():
                 g32 = got32[bi][:, c0:c0 + nk].cpu().numpy()
                 e64, n64 = _compare(g64, r, f"mixed workload float64 l={lc} shell {start} block {bi}", ATOL, RTOL, arb)
                 e32, n32 = _compare(g32, r, f"mixed workload mixed l={lc} shell {start} block {bi}", MIXED_BOUND, 0.0, arb)
-                worst64, worst32, n_arb = max(worst64, e64), max(worst32, e32), n_arb + n64 + n32
+                gE = gotE[bi][:, c0:c0 + nk].cpu().numpy()
+                eE, nE = _compare(gE, r, f"mixed workload envelope split l={lc} shell {start} block {bi}", MIXED_BOUND,
+                                  0.0, arb)
+                worst64, worst32, worstE = max(worst64, e64), max(worst32, e32), max(worstE, eE)
+                n_arb += n64 + n32 + nE
                 d = np.abs(g32 - r)
                 sq32 += float((d * d).sum())
+                d = np.abs(gE - r)
+                sqE += float((d * d).sum())
                 n_el += d.size
             _say(f"  l={lc} shell {start} (aux class {ai}): worst vs float64 oracle: float64 path {worst64:.3e}, "
-                 f"mixed {worst32:.3e}; rows decided by the long-double oracle so far {n_arb}")
+                 f"mixed {worst32:.3e}, envelope split {worstE:.3e}; rows decided by the long-double oracle so far "
+                 f"{n_arb}")
             c0 += nk
-    _say(f"  mixed vs oracle: max abs {worst32:.3e}, RMS {(sq32 / n_el) ** 0.5:.3e} over {n_el} sampled elements, "
-         f"{n_arb} rows decided by the long-double oracle")
+    _say(f"  mixed vs oracle: Schwarz split max abs {worst32:.3e}, RMS {(sq32 / n_el) ** 0.5:.3e}; envelope split "
+         f"max abs {worstE:.3e}, RMS {(sqE / n_el) ** 0.5:.3e}; over {n_el} sampled elements, {n_arb} rows decided "
+         "by the long-double oracle")
     assert n_arb <= 1e-4 * n_el
-    del got64, got32, plan
+    del got64, got32, gotE, plan
     _free()
 
 
