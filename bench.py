@@ -514,15 +514,55 @@ def headline_leg(args):
     return res
 
 
-def plain80_leg(args):
-    """The 80-atom workload: every (ij|P) into the packed device output plus (P|Q), per step."""
+def dropin_leg(repeats=3):
+    """The reference-style drop-in loop -- molecular_integral.three_center / two_center over every class
+    through the C ABI, host arrays in and the dense reference layout out -- on the 12-atom workload,
+    timed end to end (best of ``repeats`` after one warm-up) beside the compiled reference (oracle/_ref,
+    OpenMP) called the same way, and their largest difference."""
+    from electron_integral_playground_amd import generators, pair_utility
+    from electron_integral_playground_amd.molecular_integral import three_center, two_center
+    from tests.helpers import REF_SO, ORACLE_SO, run_three_center_oracle, run_two_center_oracle
+    mol = generators.make_molecule("small")
+    pairs = pair_utility.form_pair_data(mol, 1e-14)
+    aux = pair_utility.form_auxiliary_pair_data(mol)
+    kind, path = ("reference", REF_SO) if REF_SO.exists() else ("port", ORACLE_SO)
+    lib = ctypes.CDLL(str(path))
+    f3 = "three_center" if kind == "reference" else "oracle_three_center"
+    f2 = "two_center" if kind == "reference" else "oracle_two_center"
+
+    def best(fn):
+        fn()
+        ts = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            out = fn()
+            ts.append(time.perf_counter() - t0)
+        return min(ts), out
+
+    t3, J3 = best(lambda: three_center(pairs, aux, mol))
+    r3, R3 = best(lambda: run_three_center_oracle(lib, pairs, aux, mol, fn=f3))
+    t2, J2 = best(lambda: two_center(aux, mol))
+    r2, R2 = best(lambda: run_two_center_oracle(lib, aux, mol, fn=f2))
+    n3 = mol.n_aux * mol.n_ao * (mol.n_ao + 1) // 2
+    res = {"workload": "12 atoms (small), n_ao %d, n_aux %d; dense (n_aux, n_ao, n_ao) host output" % (mol.n_ao, mol.n_aux),
+           "three_center_s": t3, "three_center_reference_s": r3, "speedup_three_center": r3 / t3,
+           "three_center_integrals_per_s": n3 / t3, "two_center_s": t2, "two_center_reference_s": r2,
+           "max_abs_diff_three_center": float(np.abs(J3 - R3).max()), "max_abs_diff_two_center": float(np.abs(J2 - R2).max()),
+           "reference_kind": kind, "reference_threads": int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)}
+    log(f"drop-in loop: {res}")
+    return res
+
+
+def plain80_leg(args, omega=0.0):
+    """The 80-atom workload: every (ij|P) into the packed device output plus (P|Q), per step (range-
+    separated erf(omega r)/r operator when omega > 0)."""
     import torch
     from electron_integral_playground_amd import generators, pair_utility
     from electron_integral_playground_amd.engine import ThreeCenterPlan, three_center_flops_per_triplet
     mol = generators.make_molecule("plain")
     pairs = pair_utility.form_pair_data(mol, 1e-14)
     aux = pair_utility.form_auxiliary_pair_data(mol)
-    plan = ThreeCenterPlan(mol, pairs, aux)
+    plan = ThreeCenterPlan(mol, pairs, aux, omega=omega)
     out = plan.new_output()
     J2c = torch.zeros((mol.n_aux, mol.n_aux), dtype=torch.float64, device="cuda")
     stream = torch.cuda.current_stream()
@@ -566,7 +606,8 @@ def plain80_leg(args):
     key, t, fl = per_class[0]
     log(f"plain80 step {dt * 1e3:.2f} ms -> {n_int / dt:.4e} integrals/s; dominant class {key} {t * 1e3:.3f} ms "
         f"{fl / t / 1e12:.2f} TFLOP/s (model)")
-    res = {"workload": "80 atoms, 14 A box, 3s2p1d (1-6 prim), aux s-f 8 exp/l ratio 2.2",
+    res = {"workload": "80 atoms, 14 A box, 3s2p1d (1-6 prim), aux s-f 8 exp/l ratio 2.2" +
+                       (f"; erf(omega r)/r, omega {omega}" if omega else ""),
            "n_ao": mol.n_ao, "n_aux": mol.n_aux, "value": n_int / dt, "unit": "3c integrals/s",
            "ms_per_step": dt * 1e3,
            "output": "packed (n_ao(n_ao+1)/2, n_aux) float64 (ij|P) + dense (n_aux, n_aux) (P|Q) on the device, every step",
@@ -890,10 +931,12 @@ def main():
         print(json.dumps({"plain80": plain80_leg(args)}), flush=True)
         return
     head = headline_leg(args)
-    plain80 = batched = serving = None
+    plain80 = batched = serving = omega80 = dropin = None
     if not args.headline_only:
         if not args.no_plain80:
             plain80 = plain80_leg(args)
+            omega80 = plain80_leg(argparse.Namespace(**{**vars(args), "steps": 2}), omega=0.2)
+        dropin = dropin_leg()
         if not args.no_batched:
             batched = batched_leg()
         if not args.no_serving:
@@ -915,7 +958,7 @@ def main():
             "mixed_precision": head["mixed"],
             "max_abs_err_vs_cpu_reference_sample": head["max_abs_err_vs_cpu_reference_sample"],
             "class_seconds_per_step": head["class_seconds_per_step"],
-            "plain80": plain80, "batched": batched, "serving": serving,
+            "plain80": plain80, "plain80_omega": omega80, "dropin_loop": dropin, "batched": batched, "serving": serving,
             "three_center_variants": variants}
     print(json.dumps(line), flush=True)
 
