@@ -195,10 +195,13 @@ class IntegralServer:
 
     def __init__(self, budget_bytes=200e9, mixed=True, threshold=1e-6, keep_outputs=False, prep_workers=12,
                  max_steps_in_flight=2, unit_cap_bytes=None, granularity="item", device="cuda", log=None,
-                 on_unit=None):
+                 on_unit=None, estimate="envelope"):
         """``on_unit(job, unit)``, if given, consumes every finished unit: it is called when the unit's
         step has retired, with the compute stream current and ``unit.out`` still valid (rows of
-        ``unit.blocks`` from ``unit.row_off``), before the unit's output memory is reused."""
+        ``unit.blocks`` from ``unit.row_off``), before the unit's output memory is reused.
+        ``estimate``: the mixed split's triplet estimate -- "envelope" (a-priori term scale of every work
+        item, distance-aware; whole items) or "schwarz" (the reference's bound ub_pp Q_P, item or pair
+        granularity)."""
         import torch
         self.torch = torch
         self.budget = float(budget_bytes)
@@ -206,6 +209,11 @@ class IntegralServer:
         self.mixed = mixed
         self.threshold = threshold
         self.granularity = granularity
+        if estimate not in ("envelope", "schwarz"):
+            raise ValueError("estimate is 'envelope' or 'schwarz'")
+        if estimate == "envelope" and granularity != "item":
+            raise ValueError("the envelope estimate splits whole items (granularity 'item')")
+        self.estimate = estimate
         self.keep_outputs = keep_outputs
         self.max_steps_in_flight = max_steps_in_flight
         self.device = torch.device(device)
@@ -291,7 +299,8 @@ class IntegralServer:
                 job.J2c_checksum = torch.stack([job.J2c.sum(), torch.linalg.vector_norm(job.J2c)])
                 job.J2c = None
             if self.mixed:
-                plan.setup_mixed(arrays, q, threshold=self.threshold, granularity=self.granularity)
+                plan.setup_mixed(arrays, q, threshold=self.threshold, granularity=self.granularity,
+                                 estimate=self.estimate)
             stream.synchronize()
         job.plan = plan
         job.input_bytes = plan.device_bytes() + (job.J2c.numel() * 8 if job.J2c is not None else 0)
