@@ -28,8 +28,10 @@ FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wno-unused-va
 
 
 def _sources():
-    srcs = sorted(GEN.glob("*.hip")) + [GEN / "registry.cpp", CSRC / "generic_kernels.hip", CSRC / "library.hip",
-                                        CSRC / "mixed_estimate.hip", CSRC / "pair_screening.cpp"]
+    # the persistent queue kernels (j3cq_*) are generated and compiled in a second pass
+    srcs = sorted(p for p in GEN.glob("*.hip") if not p.name.startswith("j3cq_")) + [
+        GEN / "registry.cpp", CSRC / "generic_kernels.hip", CSRC / "library.hip", CSRC / "mixed_estimate.hip",
+        CSRC / "pair_screening.cpp"]
     # largest first, so the long compiles start early
     return sorted(srcs, key=lambda p: -p.stat().st_size)
 
@@ -40,6 +42,11 @@ def _compile(src):
     cmd = [HIPCC] + FLAGS + ["-Rpass-analysis=kernel-resource-usage", "-c", str(src), "-o", str(obj)]
     if src.suffix == ".cpp":
         cmd = [HIPCC, "-x", "hip"] + FLAGS + ["-c", str(src), "-o", str(obj)]
+    elif src.name.startswith("j3cq_"):
+        # Persistent queue kernels: machine LICM would hoist every body's constants (float64 literals in
+        # scalar registers) out of the persistent loop, where all of them stay live at once (hundreds of
+        # scalar-register spills); without it each body keeps the register footprint of its own kernel.
+        cmd = cmd[:1] + ["-mllvm", "-disable-machine-licm"] + cmd[1:]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src.name}:\n{r.stderr[-4000:]}")
@@ -116,6 +123,21 @@ def build(jobs=None, verbose=False):
         if verbose:
             print("regenerating with smaller chunks:", sorted(spilling))
     shrink_file.write_text(json.dumps({json.dumps(list(k)): v for k, v in shrink.items()}))
+    # Second pass: the persistent queue kernels, one per register tier (tiers from the occupancy the class
+    # kernels just compiled to).
+    sys.path.insert(0, str(PKG))
+    try:
+        from codegen.generate import generate_queue
+    finally:
+        sys.path.pop(0)
+    qsrcs = [pathlib.Path(p) for p in generate_queue(str(GEN), {k: v for _, sc in results.values() for k, v in sc.items()},
+                                                     shrink)]
+    qsrcs.sort(key=lambda p: -p.stat().st_size)
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        for src, res in zip(qsrcs, ex.map(_compile, qsrcs)):
+            results[src] = res
+    objs = [o for o, _ in results.values()]
+    resources = {k: v for _, sc in results.values() for k, v in sc.items()}
     # per-kernel registers / scratch / occupancy, read by the autotuner (variants that spill are skipped)
     LIBDIR.mkdir(parents=True, exist_ok=True)
     (LIBDIR / "kernel_resources.json").write_text(json.dumps(resources, indent=0, sort_keys=True))

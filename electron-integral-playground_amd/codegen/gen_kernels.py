@@ -352,12 +352,18 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64", stage=None):
             for s in aux_rows[m][1]:
                 needed.add((t[0] + s[0], t[1] + s[1], t[2] + s[2]))
     out = [f"// ({la}{lb}|{lc}) aux components [{m0},{m1}) {precision}: L = {L}, {nnz} pair entries"]
-    # One body, two kernels: '{name}' reads the molecule's data from its kernel arguments (rematerialised
-    # scalar loads from the kernarg segment), '{name}_jobs' from the per-item job table.
-    out.append("template <bool kJobs>")
-    out.append(f"__device__ __forceinline__ void {name}_body(const J3cArgs a)")
+    # One body, three callers: '{name}' reads the molecule's data from its kernel arguments (rematerialised
+    # scalar loads from the kernarg segment), '{name}_jobs' from the per-item job table, and the
+    # persistent queue kernels (one per register tier) call it once per work item they pull.  lb_ is the
+    # item's index in the launch (items a.item_begin + lb_), smem the caller's LDS staging buffer
+    # (j3c_smem_bytes(...) bytes, 16-byte aligned; unused by unstaged and DPP-staged bodies).
+    # ArgsT: J3cArgs (the kernels' own by-value argument) or the constant-address-space J3cArgs of a queue
+    # slot; either way the fields are invariant loads the compiler can re-issue where they are used.
+    out.append("template <bool kJobs, class ArgsT>")
+    # tid: the lane (threadIdx.x), passed in so a persistent caller can hand over a per-item copy (values
+    # derived from it then stay inside the item instead of being hoisted out of the caller's loop).
+    out.append(f"__device__ __forceinline__ void {name}_body(const ArgsT& a, const int lb_, char* __restrict__ smem, const int tid)")
     out.append("{")
-    out.append("    const int lb_ = eip::xcd_remap(blockIdx.x, gridDim.x);")
     out.append("    const int item = a.item_begin + lb_;")
     out.append("    if (item >= a.item_end) return;")
     if not f32:
@@ -366,7 +372,7 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64", stage=None):
     out.append("#define JD(f) (kJobs ? (decltype(a.d.f)) jt->f : a.d.f)")
     out.append("    const int sp = eip::ro(a.items)[2 * item];")
     out.append("    const int ks_first = eip::ro(a.items)[2 * item + 1];")
-    out.append("    const int ks_raw = ks_first + (int) threadIdx.x;")
+    out.append("    const int ks_raw = ks_first + tid;")
     out.append("    const int ks_last = a.items_ks_end ? eip::ro(a.items_ks_end)[item] : (int) JD(n_ks);")
     out.append("    const bool active = ks_raw < ks_last;")
     out.append("    const int ks = active ? ks_raw : ks_last - 1;")
@@ -387,9 +393,10 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64", stage=None):
     K = max(1, min(STAGE_MAX_ROWS, STAGE_BATCH_BYTES // row_bytes))
     n_pieces = (K * row_bytes + 1023) // 1024              # 64 lanes x 16 bytes per global->LDS load
     if use_lds:
-        out.append(f"    __shared__ __attribute__((aligned(16))) {ty} sE[2][{n_pieces * 1024 // esize}];")
-        out.append(f"    __shared__ __attribute__((aligned(16))) double sP[2][{4 * K}];")
-        out.append("    const int lane = (int) threadIdx.x;")
+        ne = n_pieces * 1024 // esize
+        out.append(f"    {ty} (* __restrict__ sE)[{ne}] = reinterpret_cast<{ty} (*)[{ne}]>(smem);")
+        out.append(f"    double (* __restrict__ sP)[{4 * K}] = reinterpret_cast<double (*)[{4 * K}]>(smem + {2 * ne * esize});")
+        out.append("    const int lane = tid;")
         out.append(f"    const char* __restrict__ Ebase = (const char*) JD({'E32' if f32 else 'E'});")
         out.append("    const char* __restrict__ Pbase = (const char*) JD(pp_P);")
         # stage(pp, nrows, buf): issue the asynchronous loads of rows [pp, pp + nrows) into buffer buf
@@ -405,7 +412,7 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64", stage=None):
         out.append("    } while (0)")
     nv = (nnz + 15) // 16
     if use_dpp:
-        out.append("    const int l16 = (int) threadIdx.x & 15;")
+        out.append("    const int l16 = tid & 15;")
         out.append(f"    const auto Erow = eip::ro(JD({'E32' if f32 else 'E'}));")
         out.append("    const auto Prow = eip::ro(JD(pp_P));")
         for v in range(nv):
@@ -654,9 +661,33 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64", stage=None):
     if use_lds:
         out.append("#undef EIP_STAGE")
     out.append("}")
-    out.append(f'extern "C" __global__ void __launch_bounds__(64) {name}(const J3cArgs a) {{ {name}_body<false>(a); }}')
-    out.append(f'extern "C" __global__ void __launch_bounds__(64) {name}_jobs(const J3cArgs a) {{ {name}_body<true>(a); }}')
+    smem_bytes = j3c_smem_bytes(la, lb, lc, m0, m1, precision, stage)
+    # the persistent queue kernels include this source for its bodies only
+    out.append("#ifndef EIP_BODIES_ONLY")
+    for kname, jobs in ((name, "false"), (name + "_jobs", "true")):
+        if smem_bytes:
+            out.append(f'extern "C" __global__ void __launch_bounds__(64) {kname}(const J3cArgs a) {{ '
+                       f'__shared__ __attribute__((aligned(16))) char smem[{smem_bytes}]; '
+                       f'{name}_body<{jobs}>(a, eip::xcd_remap(blockIdx.x, gridDim.x), smem, (int) threadIdx.x); }}')
+        else:
+            out.append(f'extern "C" __global__ void __launch_bounds__(64) {kname}(const J3cArgs a) {{ '
+                       f'{name}_body<{jobs}>(a, eip::xcd_remap(blockIdx.x, gridDim.x), nullptr, (int) threadIdx.x); }}')
+    out.append("#endif")
     return name, "\n".join(out)
+
+
+def j3c_smem_bytes(la, lb, lc, m0, m1, precision="f64", stage=None):
+    """LDS bytes the vector three-centre body of this class chunk stages its pair-matrix rows in (two
+    buffers of K rows and their centres; 0 for DPP-staged bodies)."""
+    stage = stage or STAGE
+    use_dpp = stage == "dpp" and pair_nnz(la, lb, True) <= DPP_MAX_ENTRIES
+    if use_dpp or not STAGE_LDS:
+        return 0
+    esize = 4 if precision == "f32" else 8
+    row_bytes = pair_stride(la, lb, True) * esize
+    K = max(1, min(STAGE_MAX_ROWS, STAGE_BATCH_BYTES // row_bytes))
+    n_pieces = (K * row_bytes + 1023) // 1024
+    return 2 * n_pieces * 1024 + 2 * 4 * K * 8
 
 
 # ---------------------------------------------------------------------------------------------

@@ -59,6 +59,16 @@ def stage_table(path=STAGE_CHOICE):
         return json.load(f)
 
 
+def vector_plans(la, lb, lc, shrink):
+    """The distinct chunk plans of the vector three-centre kernels of a class, in variant order."""
+    cls_plans = []
+    for vi, budget in enumerate(VARIANT_BUDGETS):
+        plan = m_chunks(la, lb, lc, shrink.get((la, lb, lc), 0) if vi == 0 else 0, budget)
+        if plan not in cls_plans:
+            cls_plans.append(plan)
+    return cls_plans
+
+
 def generate(out_dir, shrink=None):
     """shrink: {(la, lb, lc): k} lowers the register budget of a class by 2^k (see m_chunks)."""
     shrink = shrink or {}
@@ -138,11 +148,7 @@ def generate(out_dir, shrink=None):
         mplans = {}
         bplans = {}
         for lc in range(SPEC_MAX_AUX_L + 1):
-            cls_plans = []
-            for vi, budget in enumerate(VARIANT_BUDGETS):
-                plan = m_chunks(la, lb, lc, shrink.get((la, lb, lc), 0) if vi == 0 else 0, budget)
-                if plan not in cls_plans:
-                    cls_plans.append(plan)
+            cls_plans = vector_plans(la, lb, lc, shrink)
             plans[lc] = cls_plans
             for v, plan in enumerate(cls_plans):
                 for ci, (m0, m1) in enumerate(plan):
@@ -203,8 +209,10 @@ def generate(out_dir, shrink=None):
                     cases.append(f"        case {lc * 10000 + m0 * 100 + m1}: if (a.jobs) hipLaunchKernelGGL({name}_jobs, dim3(nblocks), dim3(64), 0, s, a);"
                                  f" else hipLaunchKernelGGL({name}, dim3(nblocks), dim3(64), 0, s, a); return 0;")
             fn = f"eip_launch_j3c_{la}{lb}_{prec}"
+            src.append("#ifndef EIP_BODIES_ONLY")
             src.append(f"int {fn}(int lc, int m0, int m1, const J3cArgs& a, int nblocks, hipStream_t s)\n{{\n"
                        "    switch (lc * 10000 + m0 * 100 + m1) {\n" + "\n".join(cases) + "\n    }\n    return -1;\n}\n")
+            src.append("#endif")
             _write(os.path.join(out_dir, f"j3c_{la}{lb}_{prec}.hip"), "\n".join(src))
             reg.append(f"int {fn}(int lc, int m0, int m1, const J3cArgs& a, int nblocks, hipStream_t s);")
     reg.append("int eip_launch_j3c(int la, int lb, int lc, int variant, int chunk, int f32, const J3cArgs& a, int nblocks, hipStream_t s);")
@@ -227,6 +235,11 @@ def generate(out_dir, shrink=None):
                 "    for (const auto& r : kChunks)\n"
                 "        if (r[0] == la && r[1] == lb && r[2] == lc && r[3] == variant) return r[7];\n    return -1;\n}\n")
     reg.append("int eip_j3c_variant_mfma(int la, int lb, int lc, int variant);")
+    # persistent queue kernels (second generation pass, generate_queue: needs the compiled registers)
+    reg.append("int eip_j3cq_ntiers(int f32);")
+    reg.append("int eip_j3cq_chunk(int la, int lb, int lc, int f32, int chunk, int* m0, int* m1, int* tier, int* kind);")
+    reg.append("int eip_launch_j3cq(int f32, int tier, const J3cQueueArgs& q, int nblocks, hipStream_t s);")
+    reg.append("const void* eip_j3cq_kernel(int f32, int tier);")
     disp.append("int eip_launch_j3c(int la, int lb, int lc, int variant, int chunk, int f32, const J3cArgs& a, int nblocks, hipStream_t s)\n{")
     disp.append("    int m0, m1;")
     disp.append("    if (eip_j3c_chunk(la, lb, lc, variant, chunk, &m0, &m1) != 0) return -1;")
@@ -341,3 +354,143 @@ def emit_generic_tables(out_dir):
 
 if __name__ == "__main__":
     print(generate(sys.argv[1]))
+
+
+# ---------------------------------------------------------------------------------------------
+# Persistent queue kernels (second pass: tiers come from the compiled kernels' register use)
+# ---------------------------------------------------------------------------------------------
+
+TUNING_FILE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tuning",
+                           "three_center_variants.json")
+# Register tiers of the persistent kernels, by the occupancy (waves per SIMD) of the class chunk's own
+# kernel: a tier kernel runs at the occupancy of its largest body, so bodies are grouped with bodies of
+# the same occupancy.
+QUEUE_TIERS = ((3, 99), (2, 2), (1, 1))
+
+
+# Largest register footprint (VGPRs + AGPRs per lane, of the chunk's own kernel) of a body in a queue
+# kernel: the persistent loop around the bodies needs a few registers of its own, so a body at the
+# 512-register limit would spill there.
+QUEUE_MAX_REGS = 448
+
+
+def _regs(resources, la, lb, lc, m0, m1, prec):
+    r = resources.get(f"j3c_{la}{lb}{lc}_{m0}_{m1}" + ("_f32" if prec == "f32" else "") + "_jobs", {})
+    return r.get("vgpr", 512) + r.get("agpr", 0)
+
+
+def queue_plan(la, lb, lc, prec, shrink, variants, resources=None):
+    """Chunk plan of a class in the queue kernels: the autotuned vector variant (tuning table), or the
+    default plan where the table picks a matrix-core variant (those run four waves per workgroup); when a
+    chunk of that plan needs more than QUEUE_MAX_REGS registers, the vector plan with the fewest chunks
+    among those that stay within it."""
+    plans = vector_plans(la, lb, lc, shrink)
+    v = variants.get(f"{la}{lb}{lc}_{prec}", 0)
+    plan = plans[v] if v < len(plans) else plans[0]
+    if resources is None:
+        return plan
+    fits = lambda p: all(_regs(resources, la, lb, lc, m0, m1, prec) <= QUEUE_MAX_REGS for m0, m1 in p)
+    if fits(plan):
+        return plan
+    ok = [p for p in plans if fits(p)]
+    # None: no plan fits -- the class is left to its own kernels
+    return min(ok, key=len) if ok else None
+
+
+def generate_queue(out_dir, resources, shrink=None):
+    """Write j3cq_<prec>_t<tier>.hip (one persistent kernel each) and j3cq_registry.cpp; returns the
+    source paths.  resources: {kernel name: {"occupancy": ...}} of the compiled class kernels."""
+    from .gen_kernels import j3c_smem_bytes
+    shrink = shrink or {}
+    stages = stage_table()
+    variants = {}
+    if os.path.exists(TUNING_FILE):
+        with open(TUNING_FILE) as f:
+            variants = json.load(f).get("variants", {})
+    pair_classes = [(la, lb) for lb in range(SPEC_MAX_L + 1) for la in range(lb + 1)]
+    rows = []
+    bodies = {(f32, t): [] for f32 in (0, 1) for t in range(len(QUEUE_TIERS))}
+    for f32, prec in ((0, "f64"), (1, "f32")):
+        for la, lb in pair_classes:
+            for lc in range(SPEC_MAX_AUX_L + 1):
+                for ci, (m0, m1) in enumerate(queue_plan(la, lb, lc, prec, shrink, variants, resources) or []):
+                    name = f"j3c_{la}{lb}{lc}_{m0}_{m1}" + ("_f32" if f32 else "")
+                    occ = resources.get(name + "_jobs", {}).get("occupancy", 1)
+                    tier = next((t for t, (lo, hi) in enumerate(QUEUE_TIERS) if lo <= occ <= hi), 0)
+                    smem = j3c_smem_bytes(la, lb, lc, m0, m1, prec, stages.get(f"{la}{lb}{lc}_{prec}"))
+                    kind = len(bodies[(f32, tier)])
+                    bodies[(f32, tier)].append((name, smem, (la, lb)))
+                    rows.append((la, lb, lc, f32, ci, m0, m1, tier, kind))
+    paths = []
+    reg = ['// Generated by codegen/generate.py -- dispatch to the persistent queue kernels.',
+           '#include "registry.hpp"', '']
+    for (f32, tier), bl in bodies.items():
+        prec = "f32" if f32 else "f64"
+        kname = f"j3cq_{prec}_t{tier}"
+        smem = max([16] + [b[1] for b in bl])
+        src = [HEADER, '#include "../queue_device.hpp"', "#define EIP_BODIES_ONLY"]
+        for la, lb in sorted({b[2] for b in bl}):
+            src.append(f'#include "j3c_{la}{lb}_{prec}.hip"')
+        src.append("#undef EIP_BODIES_ONLY")
+        src.append(f"// {len(bl)} class-chunk bodies of occupancy {QUEUE_TIERS[tier][0]}-{QUEUE_TIERS[tier][1]}")
+        # the tier's occupancy as a register target (the bodies fit it alone; the loop around them adds a few)
+        src.append(f'extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu({QUEUE_TIERS[tier][0]})))'
+                   f' {kname}(const J3cQueueArgs q)')
+        src.append("{")
+        src.append(f"    __shared__ __attribute__((aligned(16))) char smem[{smem}];")
+        src.append("    int part = (int) (blockIdx.x % kQueueParts);")
+        src.append("    for (;;) {")
+        src.append("        const int e = eip::queue_next(q.ctl, part);")
+        src.append("        if (e < 0) break;")
+        src.append("        const eip::cptr<J3cQueueEntry> en = eip::ro(q.entries) + e;")
+        src.append("        const int slot = __builtin_amdgcn_readfirstlane(en->slot);")
+        src.append("        const int item0 = __builtin_amdgcn_readfirstlane(en->item);")
+        src.append("        const int count = __builtin_amdgcn_readfirstlane(en->count);")
+        src.append("        const eip::cptr<J3cQueueSlot> sl = eip::ro(q.slots) + slot;")
+        # a fresh, opaque copy of the lane index and the LDS buffer per item: the bodies' lane- and
+        # buffer-derived values are then computed inside the item, not hoisted out of this loop (where
+        # every body's values would stay live at once)
+        src.append("        int tid = (int) threadIdx.x, smo = 0;")
+        src.append('        asm volatile("" : "+v"(tid), "+s"(smo));')
+        src.append("        char* const sm = smem + smo;")
+        src.append("        switch (sl->kind) {")
+        for kind, (name, _, _) in enumerate(bl):
+            src.append(f"            case {kind}: for (int k = 0; k < count; ++k) {name}_body<true>(sl->a, item0 + k, sm, tid); break;")
+        src.append("            default: break;")
+        src.append("        }")
+        src.append("    }")
+        src.append("    eip::queue_finish(q.ctl);")
+        src.append("}")
+        src.append(f"int eip_launch_{kname}(const J3cQueueArgs& q, int nblocks, hipStream_t s)\n{{\n"
+                   f"    hipLaunchKernelGGL({kname}, dim3(nblocks), dim3(64), 0, s, q);\n    return 0;\n}}")
+        src.append(f"const void* eip_{kname}_kernel() {{ return (const void*) {kname}; }}")
+        path = os.path.join(out_dir, kname + ".hip")
+        _write(path, "\n".join(src) + "\n")
+        paths.append(path)
+        reg.append(f"int eip_launch_{kname}(const J3cQueueArgs& q, int nblocks, hipStream_t s);")
+        reg.append(f"const void* eip_{kname}_kernel();")
+    reg.append("static const short kQueueChunks[][9] = {")
+    for row in rows:
+        reg.append("    {%d, %d, %d, %d, %d, %d, %d, %d, %d}," % row)
+    reg.append("};")
+    reg.append(f"int eip_j3cq_ntiers(int f32) {{ (void) f32; return {len(QUEUE_TIERS)}; }}")
+    reg.append("int eip_j3cq_chunk(int la, int lb, int lc, int f32, int chunk, int* m0, int* m1, int* tier, int* kind)\n{\n"
+               "    for (const auto& r : kQueueChunks)\n"
+               "        if (r[0] == la && r[1] == lb && r[2] == lc && r[3] == f32 && r[4] == chunk) {\n"
+               "            *m0 = r[5]; *m1 = r[6]; *tier = r[7]; *kind = r[8]; return 0;\n        }\n"
+               "    return -1;\n}")
+    reg.append("int eip_launch_j3cq(int f32, int tier, const J3cQueueArgs& q, int nblocks, hipStream_t s)\n{\n"
+               "    switch (f32 * 10 + tier) {")
+    for f32, tier in bodies:
+        prec = "f32" if f32 else "f64"
+        reg.append(f"        case {f32 * 10 + tier}: return eip_launch_j3cq_{prec}_t{tier}(q, nblocks, s);")
+    reg.append("    }\n    return -1;\n}")
+    reg.append("const void* eip_j3cq_kernel(int f32, int tier)\n{\n    switch (f32 * 10 + tier) {")
+    for f32, tier in bodies:
+        prec = "f32" if f32 else "f64"
+        reg.append(f"        case {f32 * 10 + tier}: return eip_j3cq_{prec}_t{tier}_kernel();")
+    reg.append("    }\n    return nullptr;\n}")
+    path = os.path.join(out_dir, "j3cq_registry.cpp")
+    _write(path, "\n".join(reg) + "\n")
+    paths.append(path)
+    return paths

@@ -886,6 +886,86 @@ int eip_three_center_work_checked(int i_L, int j_L, int k_L, int pass, const int
     return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
+// ---- persistent queue kernels: one kernel per register tier and precision over a device work queue ----
+
+int eip_three_center_queue_tiers(int use_f32)
+{
+    return eip_j3cq_ntiers(use_f32 ? 1 : 0);
+}
+
+int eip_three_center_queue_chunk(int i_L, int j_L, int k_L, int use_f32, int chunk, int* m0, int* m1, int* tier)
+{
+    if (!spec_3c(i_L, j_L, k_L) || !m0 || !m1 || !tier) return -1;
+    int kind = 0;
+    return eip_j3cq_chunk(i_L, j_L, k_L, use_f32 ? 1 : 0, chunk, m0, m1, tier, &kind);
+}
+
+long long eip_three_center_queue_slot_bytes(void) { return (long long) sizeof(J3cQueueSlot); }
+
+int eip_three_center_queue_ctl_words(void) { return kQueueCtlWords; }
+
+int eip_three_center_queue_slot(int i_L, int j_L, int k_L, int use_f32, int chunk, const int* d_items,
+                                const int* d_items_ks_end, const int* d_items_pp, const int* d_items_job,
+                                int n_items, const void* d_jobs, int layout, int accumulate, double omega,
+                                void* h_slot)
+{
+    if (!spec_3c(i_L, j_L, k_L)) return -1;
+    if (!d_items || !d_jobs || !h_slot || n_items < 0 || (layout != 0 && layout != 1)
+        || (accumulate != 0 && accumulate != 1)) return -4;
+    int m0 = 0, m1 = 0, tier = 0, kind = 0;
+    if (eip_j3cq_chunk(i_L, j_L, k_L, use_f32 ? 1 : 0, chunk, &m0, &m1, &tier, &kind) != 0) return -1;
+    J3cQueueSlot sl{};
+    sl.a.items = d_items;
+    sl.a.items_ks_end = d_items_ks_end;
+    sl.a.items_pp = d_items_pp;
+    sl.a.items_job = d_items_job;
+    sl.a.item_begin = 0;
+    sl.a.item_end = n_items;
+    sl.a.jobs = static_cast<const J3cData*>(d_jobs);
+    sl.a.layout = layout;
+    sl.a.accumulate = accumulate;
+    sl.a.omega = omega;
+    sl.kind = kind;
+    sl.tier = tier;
+    std::memcpy(h_slot, &sl, sizeof sl);
+    return tier;
+}
+
+int eip_three_center_queue_grid(int use_f32, int tier, int waves_per_simd)
+{
+    const void* k = eip_j3cq_kernel(use_f32 ? 1 : 0, tier);
+    if (!k) return -1;
+    int dev = 0, n_cu = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess
+        || hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess
+        || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 64, 0) != hipSuccess) return -2;
+    if (waves_per_simd > 0) per_cu = std::min(per_cu, 4 * waves_per_simd);   // 4 SIMDs per CU
+    return std::max(1, per_cu) * n_cu;
+}
+
+// Whether p is an address of device memory (or of host memory the device can read) known to the runtime:
+// the persistent kernels dereference the slot, entry and control pointers unchecked, so a pointer that
+// is not one (a truncated or host address) is rejected here instead of faulting the device.
+static bool device_accessible(const void* p)
+{
+    hipPointerAttribute_t attr{};
+    if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+        (void) hipGetLastError();
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged || attr.devicePointer != nullptr;
+}
+
+int eip_three_center_queue_run(int use_f32, int tier, const void* d_slots, const int* d_entries, int* d_ctl,
+                               int grid, void* stream)
+{
+    if (!d_slots || !d_entries || !d_ctl || grid <= 0) return 4;
+    if (!device_accessible(d_slots) || !device_accessible(d_entries) || !device_accessible(d_ctl)) return 4;
+    J3cQueueArgs q{static_cast<const J3cQueueSlot*>(d_slots), reinterpret_cast<const J3cQueueEntry*>(d_entries), d_ctl};
+    if (eip_launch_j3cq(use_f32 ? 1 : 0, tier, q, grid, (hipStream_t) stream) != 0) return 3;
+    return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
 int eip_copy_probe(const double* d_src, double* d_dst, long long n, void* stream)
 {
     if (n <= 0) return 0;

@@ -70,6 +70,42 @@ struct J3cArgs {
     double flag_threshold;
 };
 
+// Persistent (queue) three-centre kernels: one kernel per register tier and precision runs the bodies of
+// every class chunk of that tier; each wave pulls work items (slot, item) from a device queue until it
+// is empty.  A slot is the argument block of one class chunk over one item list (built on the host by
+// eip_three_center_queue_slot); 'kind' selects the body inside the tier kernel.
+struct J3cQueueSlot {
+    J3cArgs a;
+    int kind;                 // body index in the tier kernel's switch
+    int tier;
+};
+
+// The queue is split into kQueueParts parts (one per XCD: the wave of workgroup b starts on part b % 8,
+// whose entries the host dealt so that neighbours in the longest-first order share that XCD's L2) and
+// moves on to the other parts when its own is empty.  An entry is a run of consecutive items of one slot
+// (the host groups cheap items, so that a wave takes one entry per few microseconds of work at least).
+// Control block, int32 words; each part's counter has a 128-byte cache line of its own (atomics on one
+// line serialise in the L2):
+//   [32 p] next entry of part p (0 before a launch; the last wave to finish resets them),
+//   [kQueueFirst + p] first entry of part p, [kQueueCount + p] entries of part p, [kQueueDone] waves done.
+constexpr int kQueueParts = 8;
+constexpr int kQueueLine = 32;                         // int32 words per 128-byte line
+constexpr int kQueueFirst = kQueueParts * kQueueLine;  // 256
+constexpr int kQueueCount = kQueueFirst + kQueueParts; // 264
+constexpr int kQueueDone = kQueueFirst + kQueueLine;   // 288 (a line of its own)
+constexpr int kQueueCtlWords = kQueueDone + kQueueLine;
+struct J3cQueueEntry {
+    int slot;                 // argument block (class chunk, item list)
+    int item;                 // first item of the run
+    int count;                // items in the run
+    int pad;
+};
+struct J3cQueueArgs {
+    const J3cQueueSlot* slots;
+    const J3cQueueEntry* entries;
+    int* ctl;                 // control block (kQueueCtlWords int32 words)
+};
+
 // Nuclear attraction: lanes over point charges, output [(sp * nij + ij) * ld + charge].
 struct NucArgs {
     const int* sp_pp_off;
@@ -102,7 +138,8 @@ struct J2cArgs {
 
 // Job-table entry of work item 'item' (a.jobs, indexed by a.items_job or 0), viewed through the constant
 // address space: its fields are wave-uniform invariant loads, re-read where they are used.
-__device__ __forceinline__ const __attribute__((address_space(4))) J3cData* j3c_item_job(const J3cArgs a, int item)
+template <class ArgsT>
+__device__ __forceinline__ const __attribute__((address_space(4))) J3cData* j3c_item_job(const ArgsT& a, int item)
 {
     typedef const __attribute__((address_space(4))) J3cData* job_ptr;
     typedef const __attribute__((address_space(4))) int* int_ptr;

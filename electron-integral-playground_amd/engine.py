@@ -1043,6 +1043,117 @@ def metric_cholesky(J2c):
     return L
 
 
+class CoulombFit:
+    """Density-fitted Coulomb matrix from (ij|P) consumed on the device as it is produced:
+
+        gamma_P = sum_ij (P|ij) D_ij,   (P|Q) d = gamma,   J_ij = sum_P (ij|P) d_P
+
+    (the standard RI-J contraction; the reference computes only the integrals).  The metric (P|Q) is
+    factorised once in float64 (metric_cholesky).  ``from_packed`` consumes a packed (ij|P) output (two
+    matrix-vector products over it); ``streamed`` consumes compact blocks of shell pairs streamed through
+    one device buffer, evaluating the integrals twice (gamma in the first pass, J in the second) so that
+    nothing larger than one block is ever materialised -- outputs of terabytes need no more than the
+    block.  All products are float64 library matrix-vector products on the block in HBM."""
+
+    def __init__(self, plan, J2c=None, L=None):
+        self.plan = plan
+        torch = plan.torch
+        if L is None:
+            if J2c is None:
+                J2c = plan.two_center()
+            L = metric_cholesky(J2c)
+        self.L = L
+        n = plan.n_ao
+        i, j = np.tril_indices(n)
+        self.tri_i = torch.as_tensor(i, device=plan.device)
+        self.tri_j = torch.as_tensor(j, device=plan.device)
+
+    def solve(self, gamma):
+        """d = (P|Q)^-1 gamma through the float64 Cholesky factor."""
+        return self.plan.torch.cholesky_solve(gamma.view(-1, 1), self.L).view(-1)
+
+    def _weights(self, D):
+        torch = self.plan.torch
+        return D[self.tri_i, self.tri_j] * torch.where(self.tri_i == self.tri_j, 1.0, 2.0).to(D.dtype)
+
+    def gamma_packed(self, packed, D):
+        """gamma_P = sum_ij (P|ij) D_ij from the packed output J3c[tri(i, j), P] (i >= j)."""
+        return self.plan.torch.mv(packed.t(), self._weights(D))
+
+    def contract_packed(self, packed, d):
+        """J_ij = sum_P (ij|P) d_P from the packed output (symmetric (n_ao, n_ao))."""
+        torch = self.plan.torch
+        jp = torch.mv(packed, d)
+        n = self.plan.n_ao
+        J = torch.zeros((n, n), dtype=jp.dtype, device=jp.device)
+        J[self.tri_i, self.tri_j] = jp
+        J[self.tri_j, self.tri_i] = jp
+        return J
+
+    def from_packed(self, packed, D):
+        """J (n_ao, n_ao) from the packed output and a symmetric density D."""
+        return self.contract_packed(packed, self.solve(self.gamma_packed(packed, D)))
+
+    def _block_index(self, pi, sp0, sp1):
+        """AO indices (I, J) of the compact rows of shell pairs [sp0, sp1) and each row's weight in gamma
+        (1 inside a diagonal shell pair, whose block holds both (i, j) and (j, i); 2 otherwise)."""
+        cache = self.__dict__.setdefault("_idx", {})
+        key = (pi, sp0, sp1)
+        if key not in cache:
+            torch = self.plan.torch
+            pc = self.plan.pairs[pi]
+            ni, nj = 2 * pc.la + 1, 2 * pc.lb + 1
+            sp = np.arange(sp0, sp1)
+            si, sj = pc.sp_i[sp].astype(np.int64), pc.sp_j[sp].astype(np.int64)
+            I = np.repeat(si[:, None] + np.arange(ni)[None, :], nj, axis=1).ravel()
+            J = np.tile(sj[:, None] + np.arange(nj)[None, :], (1, ni)).ravel()
+            w = np.repeat(np.where(si == sj, 1.0, 2.0), ni * nj)
+            cache[key] = tuple(torch.as_tensor(a, device=self.plan.device) for a in (I, J, w))
+        return cache[key]
+
+    def _blocks(self, block_bytes):
+        plan = self.plan
+        torch = plan.torch
+        blocks = plan.row_blocks(block_bytes)
+        rows = max(plan.block_rows(*b) for b in blocks)
+        buf = self.__dict__.get("_buf")
+        if buf is None or buf.shape[0] < rows:
+            buf = self._buf = torch.empty((rows, plan.n_aux), dtype=torch.float64, device=plan.device)
+        return blocks, buf
+
+    def gamma_streamed(self, D, block_bytes, mixed=False, stream=None):
+        """gamma from one streamed pass of compact blocks (one reused buffer of at most block_bytes)."""
+        plan = self.plan
+        blocks, buf = self._blocks(block_bytes)
+        gamma = plan.torch.zeros(plan.n_aux, dtype=plan.torch.float64, device=plan.device)
+        for pi, sp0, sp1 in blocks:
+            n = plan.block_rows(pi, sp0, sp1)
+            plan.run_block(pi, sp0, sp1, buf, mixed=mixed, stream=stream)
+            I, J, w = self._block_index(pi, sp0, sp1)
+            gamma.addmv_(buf[:n].t(), D[I, J] * w)
+        return gamma
+
+    def contract_streamed(self, d, block_bytes, mixed=False, stream=None):
+        """J from one streamed pass of compact blocks."""
+        plan = self.plan
+        torch = plan.torch
+        blocks, buf = self._blocks(block_bytes)
+        out = torch.zeros((plan.n_ao, plan.n_ao), dtype=torch.float64, device=plan.device)
+        for pi, sp0, sp1 in blocks:
+            n = plan.block_rows(pi, sp0, sp1)
+            plan.run_block(pi, sp0, sp1, buf, mixed=mixed, stream=stream)
+            I, J, _ = self._block_index(pi, sp0, sp1)
+            jr = torch.mv(buf[:n], d)
+            out[I, J] = jr
+            out[J, I] = jr
+        return out
+
+    def streamed(self, D, block_bytes, mixed=False, stream=None):
+        """J from two streamed passes (gamma, then J): nothing larger than one block is materialised."""
+        d = self.solve(self.gamma_streamed(D, block_bytes, mixed, stream))
+        return self.contract_streamed(d, block_bytes, mixed, stream)
+
+
 def packed_to_dense(packed, n_ao):
     """(n_ao(n_ao+1)/2, n_aux) packed -> (n_aux, n_ao, n_ao) dense (reference layout), on the device."""
     import torch
@@ -1052,6 +1163,257 @@ def packed_to_dense(packed, n_ao):
     dense[:, idx_i, idx_j] = packed.t()
     dense[:, idx_j, idx_i] = packed.t()
     return dense
+
+
+# ------------------------------------------------------------------------------------------------
+# Persistent queue kernels: work items of many classes (and molecules or jobs) in one device queue per
+# register tier, run by one persistent kernel per tier.
+# ------------------------------------------------------------------------------------------------
+
+QUEUE_PARTS = 8          # one queue part per XCD
+QUEUE_RUN = 16           # consecutive entries dealt to one part (items of one shell pair share its L2)
+# Cheap items are grouped into one queue entry until the entry holds about this much work (per-lane
+# flops: primitive pairs x flops per triplet of the chunk, ~25 us of one wave's time) but at most
+# QUEUE_MAX_GROUP items: one atomic fetch per entry, not per item.
+QUEUE_ENTRY_COST = 3.0e4
+QUEUE_MAX_GROUP = 64
+# Control block words (one 128-byte line per part counter; see eip_three_center_queue_run)
+_CTL_FIRST = QUEUE_PARTS * 32
+_CTL_COUNT = _CTL_FIRST + QUEUE_PARTS
+
+
+class WorkQueue:
+    """Three-centre work items of many classes for the persistent tier kernels.
+
+    ``add`` registers one class's item list (through a job table, so items of several molecules or jobs
+    can share it) as one slot per component chunk of the class's queue plan; ``build`` orders every tier's
+    entries -- classes by decreasing mean item cost, items longest-first within a class ("sorted by class
+    and primitive count") -- and deals them in runs of QUEUE_RUN entries over the 8 XCD parts; ``upload``
+    writes slots, entries and control blocks to the device (into existing buffers when they are given and
+    large enough, so a captured graph can be replayed over new work); ``run`` launches one kernel per tier,
+    the tiers concurrently on side streams."""
+
+    def __init__(self, lib=None, device="cuda", use_f32=False, waves_per_simd=0):
+        import torch
+        self.torch = torch
+        self.lib = lib or library()
+        self.device = torch.device(device)
+        self.f32 = int(bool(use_f32))
+        self.waves_per_simd = int(waves_per_simd)
+        self.slot_bytes = int(self.lib.eip_three_center_queue_slot_bytes())
+        self.ctl_words = int(self.lib.eip_three_center_queue_ctl_words())
+        self.ntiers = int(self.lib.eip_three_center_queue_tiers(self.f32))
+        self._slots = []
+        self._tier = {t: [] for t in range(self.ntiers)}
+        self.n_triplet_items = 0
+        self.host = None
+        self.dev = None
+
+    def supports(self, la, lb, lc):
+        """Whether the class has bodies in the tier kernels (classes whose bodies need more registers than
+        a persistent loop leaves -- the highest (df|g) and (ff|x) chunks -- keep their own kernels)."""
+        m0, m1, tier = _c_int(), _c_int(), _c_int()
+        return self.lib.eip_three_center_queue_chunk(la, lb, lc, self.f32, 0, ctypes.byref(m0), ctypes.byref(m1),
+                                                     ctypes.byref(tier)) == 0
+
+    def add(self, la, lb, lc, items, n_items, jobs_ptr, costs, ks_end=None, pp=None, job=None, layout=0,
+            accumulate=0, omega=0.0):
+        """items / ks_end / pp / job: device int32 tensors of eip_three_center_work's item list (kept alive by
+        the caller); jobs_ptr: device address of the job table; costs: host array of each item's relative
+        cost (primitive pairs x auxiliary primitives)."""
+        if n_items <= 0:
+            return
+        m0, m1, tier = _c_int(), _c_int(), _c_int()
+        per_comp = three_center_flops_per_triplet(la, lb, lc) / (2 * lc + 1)
+        costs = np.asarray(costs, dtype=np.float64)
+        c = 0
+        while self.lib.eip_three_center_queue_chunk(la, lb, lc, self.f32, c, ctypes.byref(m0), ctypes.byref(m1),
+                                                    ctypes.byref(tier)) == 0:
+            buf = ctypes.create_string_buffer(self.slot_bytes)
+            t = self.lib.eip_three_center_queue_slot(la, lb, lc, self.f32, c, _ptr(items), _ptr(ks_end), _ptr(pp),
+                                                     _ptr(job), int(n_items), _vp(int(jobs_ptr)), int(layout),
+                                                     int(accumulate), _c_double(omega), buf)
+            if t < 0:
+                raise RuntimeError(f"eip_three_center_queue_slot({la},{lb},{lc}, chunk {c}) failed with {t}")
+            self._tier[t].append((len(self._slots), costs * per_comp * (m1.value - m0.value)))
+            self._slots.append(buf.raw)
+            c += 1
+        if c == 0:
+            raise RuntimeError(f"no queue kernels for class ({la}{lb}|{lc})")
+        self.n_triplet_items += int(n_items)
+
+    def build(self):
+        """Host arrays: slots (bytes), entries [n][4] int32 (slot, first item, item count, 0) and the
+        control block of every tier.  A slot's items are grouped in their stored order into runs of about
+        QUEUE_ENTRY_COST work; runs are ordered longest-first within a slot, slots by decreasing mean item
+        cost, and the ordered runs are dealt QUEUE_RUN at a time over the 8 parts."""
+        entries, ctls, tot_cost = [], [], []
+        for t in range(self.ntiers):
+            lst = sorted(self._tier[t], key=lambda e: -float(e[1].mean()))
+            rows = []
+            for s, cst in lst:
+                n = len(cst)
+                g = int(np.clip(np.ceil(QUEUE_ENTRY_COST / max(float(cst.mean()), 1e-30)), 1, QUEUE_MAX_GROUP))
+                first = np.arange(0, n, g, dtype=np.int64)
+                count = np.minimum(g, n - first)
+                run_cost = np.add.reduceat(cst, first)
+                o = np.argsort(-run_cost, kind="stable")
+                rows.append(np.stack([np.full(len(first), s), first[o], count[o], np.zeros(len(first), np.int64)], 1))
+            ent = np.concatenate(rows).astype(np.int32) if rows else np.zeros((0, 4), np.int32)
+            n = len(ent)
+            part = (np.arange(n) // QUEUE_RUN) % QUEUE_PARTS
+            order = np.argsort(part, kind="stable")
+            counts = np.bincount(part, minlength=QUEUE_PARTS).astype(np.int32)
+            ctl = np.zeros(self.ctl_words, np.int32)
+            ctl[_CTL_FIRST:_CTL_FIRST + QUEUE_PARTS] = np.concatenate([[0], np.cumsum(counts)[:-1]])
+            ctl[_CTL_COUNT:_CTL_COUNT + QUEUE_PARTS] = counts
+            entries.append(np.ascontiguousarray(ent[order]))
+            ctls.append(ctl)
+            tot_cost.append(sum(float(c.sum()) for _, c in lst))
+        # heaviest tier first: launched first, it takes the device while the lighter ones fill its tail
+        self.tier_order = [int(t) for t in np.argsort(-np.array(tot_cost), kind="stable")]
+        self.host = {"slots": b"".join(self._slots), "entries": entries, "ctl": ctls}
+        return self
+
+    def upload(self, into=None, stream=None):
+        """Copy the built queue to the device (into ``into``, a previous ``self.dev``-like dict of device
+        buffers, when every buffer there is large enough); returns the buffers."""
+        torch = self.torch
+        if self.host is None:
+            self.build()
+        h = self.host
+        slots = torch.frombuffer(bytearray(h["slots"]) or bytearray(8), dtype=torch.uint8)
+        ent = [torch.from_numpy(e.reshape(-1)) for e in h["entries"]]
+        ctl = torch.from_numpy(np.concatenate(h["ctl"]))
+        fits = (into is not None and into["slots"].numel() >= slots.numel()
+                and all(d.numel() >= max(1, e.numel()) for d, e in zip(into["entries"], ent)))
+        if not fits:
+            into = {"slots": torch.empty(slots.numel(), dtype=torch.uint8, device=self.device),
+                    "entries": [torch.empty(max(1, e.numel()), dtype=torch.int32, device=self.device) for e in ent],
+                    "ctl": torch.empty(ctl.numel(), dtype=torch.int32, device=self.device)}
+        s = stream if stream is not None else torch.cuda.current_stream()
+        with torch.cuda.stream(s):
+            into["slots"][:slots.numel()].copy_(slots.to(self.device, non_blocking=False))
+            for d, e in zip(into["entries"], ent):
+                if e.numel():
+                    d[:e.numel()].copy_(e.to(self.device))
+            into["ctl"].copy_(ctl.to(self.device))
+        self.dev = into
+        return into
+
+    @staticmethod
+    def buffers_for(queues):
+        """Device buffers large enough for every one of the (built) queues: one captured launch sequence
+        can then be replayed over each of them in turn after ``upload(into=...)``."""
+        q0 = queues[0]
+        torch = q0.torch
+        nslot = max(len(q.host["slots"]) for q in queues)
+        nent = [max(max(1, q.host["entries"][t].size) for q in queues) for t in range(q0.ntiers)]
+        return {"slots": torch.empty(max(8, nslot), dtype=torch.uint8, device=q0.device),
+                "entries": [torch.empty(n, dtype=torch.int32, device=q0.device) for n in nent],
+                "ctl": torch.empty(q0.ctl_words * q0.ntiers, dtype=torch.int32, device=q0.device)}
+
+    def grid(self, t):
+        key = (self.f32, t, self.waves_per_simd)
+        g = _QUEUE_GRIDS.get(key)
+        if g is None:
+            g = int(self.lib.eip_three_center_queue_grid(self.f32, t, self.waves_per_simd))
+            if g <= 0:
+                raise RuntimeError(f"eip_three_center_queue_grid({self.f32}, {t}) failed with {g}")
+            _QUEUE_GRIDS[key] = g
+        return g
+
+    def fits(self, bufs):
+        h = self.host
+        return (bufs is not None and bufs["slots"].numel() >= len(h["slots"])
+                and all(d.numel() >= e.size for d, e in zip(bufs["entries"], h["entries"])))
+
+    def run(self, stream=None, concurrent=True):
+        """One launch per tier (all tiers, empty ones exit at once: the launch sequence does not depend on
+        the work, so it can be captured once and replayed after new uploads)."""
+        if self.dev is None:
+            self.upload(stream=stream)
+        d = self.dev
+        order = getattr(self, "tier_order", list(range(self.ntiers)))
+        with _fork(stream, len(order) - 1 if concurrent else 0) as sides:
+            streams = [stream] + list(sides)
+            for k, t in enumerate(order):
+                st = streams[k] if concurrent else stream
+                rc = self.lib.eip_three_center_queue_run(
+                    self.f32, t, _ptr(d["slots"]), _ptr(d["entries"][t]),
+                    _vp(d["ctl"].data_ptr() + 4 * self.ctl_words * t), self.grid(t), _stream_handle(st))
+                if rc != 0:
+                    raise RuntimeError(f"eip_three_center_queue_run(tier {t}) failed with {rc}")
+
+
+_QUEUE_GRIDS = {}
+
+
+class QueueGraph:
+    """One captured HIP graph of the persistent tier kernels, replayed over successive work queues: each
+    queue is uploaded into the same device buffers (slots, entries, control blocks), which the launches
+    read, so the graph is captured once and replayed for every batch.  The buffers grow with headroom --
+    and the graph is captured again -- only when a queue does not fit (``captures`` counts this)."""
+
+    def __init__(self, headroom=1.5):
+        self.headroom = headroom
+        self.bufs = None
+        self.graph = None
+        self.order = None
+        self.captures = 0
+
+    def _grow(self, q):
+        torch = q.torch
+        h = q.host
+        hr = self.headroom
+        old = self.bufs
+        self.bufs = {"slots": torch.empty(int(max(8, len(h["slots"])) * hr) + 8, dtype=torch.uint8, device=q.device),
+                     "entries": [torch.empty(int(max(1, e.size) * hr) + 2, dtype=torch.int32, device=q.device)
+                                 for e in h["entries"]],
+                     "ctl": torch.zeros(q.ctl_words * q.ntiers, dtype=torch.int32, device=q.device)}
+        if old is not None:
+            for k in ("slots", "entries"):
+                v = self.bufs[k]
+                if isinstance(v, list):
+                    for i, (a, b) in enumerate(zip(v, old[k])):
+                        if a.numel() < b.numel():
+                            v[i] = torch.empty(b.numel(), dtype=b.dtype, device=b.device)
+                elif v.numel() < old[k].numel():
+                    self.bufs[k] = torch.empty(old[k].numel(), dtype=old[k].dtype, device=old[k].device)
+        self.graph = None
+
+    def run(self, q, stream=None):
+        """Upload queue q into the graph's buffers and replay the graph on ``stream`` (captured first if
+        needed)."""
+        torch = q.torch
+        s = stream if stream is not None else torch.cuda.current_stream()
+        self.ensure(q, s)
+        q.tier_order = self.order
+        q.upload(into=self.bufs, stream=s)
+        with torch.cuda.stream(s):
+            self.graph.replay()
+
+    def ensure(self, q, stream=None):
+        """Buffers that hold q and a captured graph over them (the capture is preceded by one launch over
+        an empty queue, which does no work).  Call it before forking other streams from ``stream``."""
+        torch = q.torch
+        s = stream if stream is not None else torch.cuda.current_stream()
+        if not q.fits(self.bufs):
+            self._grow(q)
+        if self.graph is None:
+            for t in range(q.ntiers):
+                q.grid(t)
+            self.order = list(getattr(q, "tier_order", range(q.ntiers)))
+            empty = WorkQueue(q.lib, q.device, use_f32=bool(q.f32), waves_per_simd=q.waves_per_simd).build()
+            empty.tier_order = self.order
+            empty.upload(into=self.bufs, stream=s)
+            empty.run(s)                                   # warm-up outside capture: no work
+            s.synchronize()
+            self.graph = torch.cuda.CUDAGraph()
+            with torch.cuda.stream(s):
+                with torch.cuda.graph(self.graph, stream=s):
+                    empty.run(s)
+            self.captures += 1
 
 
 class BatchPlan:
@@ -1130,6 +1492,7 @@ class BatchPlan:
             self.aux.append((lc, n_ks, d))
         # items per (pair class, aux class), molecule by molecule
         self.items = {}
+        self.item_costs = {}
         self.n_triplet_items = 0
         for pi, (key, _, _, _) in enumerate(self.pairs):
             for ai, (lc, _, _) in enumerate(self.aux):
@@ -1149,6 +1512,7 @@ class BatchPlan:
                     # longest-first dispatch (cost: primitive pairs of the item's shell pair)
                     order = longest_first(pair_n_pp[pi][sp])
                     self.items[(pi, ai)] = (T(items[order], torch.int32), T(end[order], torch.int32))
+                    self.item_costs[(pi, ai)] = pair_n_pp[pi][sp][order].astype(np.float64)
                     self.n_triplet_items += len(sp)
         self.mol_ks = mol_ks
         # per molecule: packed rows that no kernel writes (shell pairs screened out entirely)
@@ -1227,9 +1591,70 @@ class BatchPlan:
                         raise RuntimeError(f"eip_two_center_device({la},{lb}) failed with {rc}")
         return out2c
 
-    def run(self, out, stream=None, zero=True):
+    def job_table(self, out):
+        """Device table (int64 [n_classes][16], eip_j3c_job) of every (pair class, auxiliary class) of the
+        batch writing into ``out``, and the row of each class; the vector pair matrices (the queue kernels'
+        bodies are the vector kernels)."""
+        keys = sorted(self.items)
+        rows = []
+        for pi, ai in keys:
+            dp, da = self.pairs[pi][3], self.aux[ai][2]
+            rows.append([dp["off"].data_ptr(), dp["si"].data_ptr(), dp["sj"].data_ptr(), dp["P"].data_ptr(),
+                         dp["E"].data_ptr(), dp["E32"].data_ptr(), 0, da["off"].data_ptr(), da["col"].data_ptr(),
+                         da["C"].data_ptr(), da["c"].data_ptr(), self.aux[ai][1], out.data_ptr(), 0,
+                         dp["base"].data_ptr(), dp["ld"].data_ptr()])
+        table = self.torch.tensor(np.array(rows, dtype=np.int64).reshape(-1, 16), dtype=self.torch.int64,
+                                  device=self.device)
+        return table, {k: i for i, k in enumerate(keys)}
+
+    def work_queue(self, out, waves_per_simd=0):
+        """The batch's (ij|P) as one WorkQueue (persistent tier kernels) writing into ``out``."""
+        table, row = self.job_table(out)
+        q = WorkQueue(self.lib, self.device, waves_per_simd=waves_per_simd)
+        q.job_table = table                      # kept alive with the queue
+        q.fallback = []                          # classes without queue bodies: their own kernels
+        for (pi, ai), (it, end) in self.items.items():
+            (la, lb), _, _, _ = self.pairs[pi]
+            lc = self.aux[ai][0]
+            if not q.supports(la, lb, lc):
+                q.fallback.append((pi, ai))
+                continue
+            q.add(la, lb, lc, it, int(it.shape[0]), table.data_ptr() + 128 * row[(pi, ai)], self.item_costs[(pi, ai)],
+                  ks_end=end, layout=0, omega=self.omega)
+        return q.build()
+
+    def zero_unwritten(self, out, stream=None):
+        """Zero the packed rows of every molecule's screened-out shell pairs (no kernel writes them)."""
+        if self.zero_spans.shape[0] > 0:
+            rc = self.lib.eip_zero_spans(_ptr(self.zero_spans), int(self.zero_spans.shape[0]), _ptr(out), 0,
+                                         _stream_handle(stream))
+            if rc != 0:
+                raise RuntimeError(f"eip_zero_spans failed with {rc}")
+
+    def run_persistent(self, out, queue, graph, out2c=None, stream=None, zero=True):
+        """The batch through the persistent tier kernels: ``queue`` (this batch's WorkQueue writing into
+        out) replayed from ``graph`` (a QueueGraph shared by every batch), the zeroing of screened rows and
+        -- when out2c is given -- every molecule's (P|Q) beside it on side streams."""
+        torch = self.torch
+        graph.ensure(queue, stream)
+        with _fork(stream, 2) as sides:
+            if zero and self.zero_spans.shape[0] > 0:
+                rc = self.lib.eip_zero_spans(_ptr(self.zero_spans), int(self.zero_spans.shape[0]), _ptr(out),
+                                             ZERO_BLOCKS_BESIDE_COMPUTE, _stream_handle(sides[0]))
+                if rc != 0:
+                    raise RuntimeError(f"eip_zero_spans failed with {rc}")
+            if out2c is not None:
+                self.two_center(out2c, stream=sides[1])
+            for pi, ai in getattr(queue, "fallback", []):
+                self._launch_class(pi, ai, out, sides[0])
+            graph.run(queue, stream if stream is not None else torch.cuda.current_stream())
+        return out
+
+    def run(self, out, stream=None, zero=True, queue=None):
         """(ij|P) of every molecule into its packed block of out; rows of screened-out shell pairs are
-        zeroed by a kernel (no memset: the launch sequence can be captured in a graph)."""
+        zeroed by a kernel (no memset: the launch sequence can be captured in a graph).  With ``queue`` (a
+        WorkQueue of this batch writing into out, uploaded), the class launches are replaced by the
+        persistent tier kernels over that queue."""
         with _fork(stream, int(zero and self.zero_spans.shape[0] > 0)) as sides:
             if sides:
                 # concurrent with the class kernels: the zeroed rows are disjoint from the written ones
@@ -1237,18 +1662,25 @@ class BatchPlan:
                                              ZERO_BLOCKS_BESIDE_COMPUTE, _stream_handle(sides[0]))
                 if rc != 0:
                     raise RuntimeError(f"eip_zero_spans failed with {rc}")
-            for (pi, ai), (it, end) in self.items.items():
-                (la, lb), _, _, dp = self.pairs[pi]
-                lc, n_ks, da = self.aux[ai]
-                E = self.kernel_E(pi, ai)
-                rc = self.lib.eip_three_center_items_batched(
-                    la, lb, lc, 0, _ptr(it), _ptr(end), it.shape[0], _ptr(dp["off"]), _ptr(dp["si"]), _ptr(dp["sj"]),
-                    _ptr(dp["base"]), _ptr(dp["ld"]), _ptr(dp["P"]), _ptr(E), _ptr(dp["E32"]), _vp(0),
-                    _ptr(da["off"]), _ptr(da["col"]), _ptr(da["C"]), _ptr(da["c"]), n_ks, _ptr(out),
-                    _c_double(self.omega), _stream_handle(stream))
-                if rc != 0:
-                    raise RuntimeError(f"eip_three_center_items_batched({la},{lb},{lc}) failed with {rc}")
+            for pi, ai in (self.items if queue is None else queue.fallback):
+                self._launch_class(pi, ai, out, stream)
+            if queue is not None:
+                queue.run(stream)
         return out
+
+    def _launch_class(self, pi, ai, out, stream):
+        """One class of the batch through its own kernels (current variant)."""
+        it, end = self.items[(pi, ai)]
+        (la, lb), _, _, dp = self.pairs[pi]
+        lc, n_ks, da = self.aux[ai]
+        E = self.kernel_E(pi, ai)
+        rc = self.lib.eip_three_center_items_batched(
+            la, lb, lc, 0, _ptr(it), _ptr(end), it.shape[0], _ptr(dp["off"]), _ptr(dp["si"]), _ptr(dp["sj"]),
+            _ptr(dp["base"]), _ptr(dp["ld"]), _ptr(dp["P"]), _ptr(E), _ptr(dp["E32"]), _vp(0),
+            _ptr(da["off"]), _ptr(da["col"]), _ptr(da["C"]), _ptr(da["c"]), n_ks, _ptr(out),
+            _c_double(self.omega), _stream_handle(stream))
+        if rc != 0:
+            raise RuntimeError(f"eip_three_center_items_batched({la},{lb},{lc}) failed with {rc}")
 
 
 # ------------------------------------------------------------------------------------------------

@@ -171,6 +171,41 @@ int eip_three_center_work(int i_L, int j_L, int k_L, int use_f32, const int* d_i
                           const int* d_items_pp, const int* d_items_job, int n_items, const void* d_jobs, int layout,
                           int accumulate, double omega, void* stream);
 
+/* Persistent (queue) three-centre kernels.  The work items of many classes -- of one molecule, a batch of
+ * molecules or several serving jobs -- run in one persistent kernel per register tier and precision: each
+ * wave pulls (slot, item) entries from a device queue until it is empty, so a whole step is a handful of
+ * launches (one graph node each) whatever the number of classes and molecules.  Replaces, for these
+ * callers, the per-class launches of the reference's dispatcher (CL/MI/three_center.cpp:319-631) and its
+ * OpenMP loop over (pair, auxiliary primitive) (three_center.cpp:291-317).
+ *
+ * eip_three_center_queue_tiers: number of register tiers (kernels) per precision.
+ * eip_three_center_queue_chunk: the auxiliary-component chunk [m0, m1) and tier of chunk 'chunk' of a class
+ *   in the queue kernels; -1 past the last chunk.
+ * eip_three_center_queue_slot: writes into host memory h_slot (eip_three_center_queue_slot_bytes() bytes) the
+ *   slot of one class chunk over the item list of eip_three_center_work (same arguments); returns its tier,
+ *   or a negative value (-1 unsupported class or chunk, -4 bad arguments).
+ * eip_three_center_queue_grid: workgroups that fill the device at the tier kernel's occupancy (capped at
+ *   waves_per_simd waves per SIMD when > 0).
+ * eip_three_center_queue_run: one launch of a tier kernel over the queue d_entries ([n][4] int32: slot index
+ *   into d_slots, first item within the slot's items, number of consecutive items, 0) described by the
+ *   control block d_ctl (eip_three_center_queue_ctl_words() int32 words, 128-byte aligned: word 32 p the
+ *   counter of queue part p = 0..7, zero; words 256 + p the first entry of part p; 264 + p the entries of
+ *   part p; 288 zero).  The wave of workgroup b starts on part b % 8, the part its XCD serves, and
+ *   continues with the others when it is empty.  The kernel leaves the counters zero, so the same control
+ *   block can be launched again (or replayed in a graph) without clearing.  Returns 4 on bad arguments
+ *   (including slot, entry or control pointers the runtime does not know as device-accessible memory). */
+int eip_three_center_queue_tiers(int use_f32);
+int eip_three_center_queue_chunk(int i_L, int j_L, int k_L, int use_f32, int chunk, int* m0, int* m1, int* tier);
+long long eip_three_center_queue_slot_bytes(void);
+int eip_three_center_queue_ctl_words(void);
+int eip_three_center_queue_slot(int i_L, int j_L, int k_L, int use_f32, int chunk, const int* d_items,
+                                const int* d_items_ks_end, const int* d_items_pp, const int* d_items_job,
+                                int n_items, const void* d_jobs, int layout, int accumulate, double omega,
+                                void* h_slot);
+int eip_three_center_queue_grid(int use_f32, int tier, int waves_per_simd);
+int eip_three_center_queue_run(int use_f32, int tier, const void* d_slots, const int* d_entries, int* d_ctl,
+                               int grid, void* stream);
+
 /* d_dst[i] = d_src[i] for i < n (device pointers): a plain copy kernel of known size, 8 n bytes read and
  * 8 n bytes written, with one 8-byte load and store per lane.  No counterpart in the reference: it
  * calibrates the profiler's HBM byte counters (FETCH_SIZE, WRITE_SIZE) for the traffic figures. */

@@ -14,7 +14,7 @@ HEADER = pathlib.Path(__file__).resolve().parents[1] / "include" / "molecular_in
 
 def declared_functions():
     text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
-    return re.findall(r"^\s*(?:int|double|void)\s+(\w+)\s*\(", text, flags=re.M)
+    return re.findall(r"^\s*(?:int|double|void|long long)\s+(\w+)\s*\(", text, flags=re.M)
 
 
 def test_exports_every_declared_symbol():
