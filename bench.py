@@ -725,9 +725,11 @@ def mixed_leg(threshold=1e-6, block_bytes=40e9, n_atom=None, tune=None, aux_orde
     return res
 
 
-def serving_leg(n_jobs, rate, budget=200e9, mixed=True, part=0, latency_file=None):
+def serving_leg(n_jobs, rate, budget=200e9, mixed=True, part=0, latency_file=None, persistent=True):
     """Poisson stream of integral jobs (atom counts lognormal, median 40, clipped 10-250; 4s3p2d1f
-    orbital and s-g auxiliary generators) through the continuous-batching server."""
+    orbital and s-g auxiliary generators) through the continuous-batching server (each step one work
+    queue per precision through the persistent tier kernels, or -- persistent=False -- one launch per
+    class and precision)."""
     import torch
     from electron_integral_playground_amd import generators
     from electron_integral_playground_amd.serving import IntegralServer, run_poisson
@@ -735,13 +737,13 @@ def serving_leg(n_jobs, rate, budget=200e9, mixed=True, part=0, latency_file=Non
     jobs = generators.serving_jobs(n_jobs=n_jobs, rate=rate, seed=generators.SEED_BASE + 5 + 1000 * part)
     jobs = [(t, n, seed + 100000 * part) for t, n, seed in jobs]
     # warm-up: one small job through a throw-away server (kernel loading, allocator)
-    warm = IntegralServer(budget_bytes=budget, mixed=mixed)
+    warm = IntegralServer(budget_bytes=budget, mixed=mixed, persistent=persistent)
     warm.submit(0, generators.serving_molecule(10, 1))
     warm.drain(timeout=300)
     warm.close()
     del warm
     torch.cuda.empty_cache()
-    server = IntegralServer(budget_bytes=budget, mixed=mixed)
+    server = IntegralServer(budget_bytes=budget, mixed=mixed, persistent=persistent)
     span = run_poisson(server, jobs, generators.serving_molecule, log=log)
     server.close()
     done = [j for j in server.completed if j.error is None]
@@ -754,6 +756,8 @@ def serving_leg(n_jobs, rate, budget=200e9, mixed=True, part=0, latency_file=Non
     res = {"jobs": n_jobs, "completed": len(done), "errors": len(server.completed) - len(done),
            "latency_phases_s": phases,
            "rate_jobs_per_s": rate, "budget_GB": budget / 1e9, "mixed_precision": mixed,
+           "step_kernels": "persistent tier kernels (one work queue per precision)" if persistent
+                           else "one launch per class and precision",
            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
            "mean_latency_s": float(lat.mean()), "max_latency_s": float(lat.max()),
            "span_s": span, "jobs_per_s": len(done) / span, "integrals_per_s": n_int / span,
@@ -925,6 +929,8 @@ def main():
                     help="jobs of the short serving run included in the default line")
     ap.add_argument("--serving-rate", type=float, default=1.8, help="Poisson arrival rate, jobs/s")
     ap.add_argument("--serving-plain", action="store_true", help="serve in plain float64 instead of mixed")
+    ap.add_argument("--serving-class-launches", action="store_true",
+                    help="serving steps as one launch per class and precision instead of the persistent queue")
     ap.add_argument("--serving-part", type=int, default=0, help="part index of a stream run in several calls")
     ap.add_argument("--serving-latencies", metavar="JSON", help="write every job's latency to this file")
     ap.add_argument("--single-stream", action="store_true",
@@ -968,7 +974,8 @@ def main():
         copy_probe()
     if args.serving:
         print(json.dumps({"serving": serving_leg(args.serving_jobs, args.serving_rate, mixed=not args.serving_plain,
-                                                 part=args.serving_part, latency_file=args.serving_latencies)}),
+                                                 part=args.serving_part, latency_file=args.serving_latencies,
+                                                 persistent=not args.serving_class_launches)}),
               flush=True)
         return
     if args.batched_only:

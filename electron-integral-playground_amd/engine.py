@@ -616,13 +616,21 @@ class ThreeCenterPlan:
         return out
 
     # -- generic work launches (job table) ---------------------------------------------------------
-    def job_words(self, pi, ai, out_ptr, ld, perm=None):
-        """The 16 words of one eip_j3c_job entry: pair class pi, auxiliary class ai, output."""
+    def mean_pairs_per_shell_pair(self, pi):
+        """Mean primitive pairs per shell pair of pair class pi (a work item's relative cost)."""
+        pc = self.pairs[pi]
+        return float(pc.n_pp) / max(1, pc.n_sp)
+
+    def job_words(self, pi, ai, out_ptr, ld, perm=None, vector=False):
+        """The 16 words of one eip_j3c_job entry: pair class pi, auxiliary class ai, output.  vector: the
+        packed pair matrices the vector kernels (and the persistent queue kernels) read."""
         dp, da = self.pairs[pi].dev, self.aux[ai].dev
         # E: what the float64 kernels of the class read (dense for matrix-core variants); E32: what the
         # float32 kernels read (split bfloat16 for bf16 matrix-core variants)
+        E = dp["E"] if vector else self.kernel_E(pi, ai)
+        E32 = dp["E32"] if vector else self.kernel_E32(pi, ai)
         return [dp["off"].data_ptr(), dp["si"].data_ptr(), dp["sj"].data_ptr(), dp["P"].data_ptr(),
-                self.kernel_E(pi, ai).data_ptr(), self.kernel_E32(pi, ai).data_ptr(),
+                E.data_ptr(), E32.data_ptr(),
                 perm.data_ptr() if perm is not None else 0,
                 da["off"].data_ptr(), da["col"].data_ptr(), da["C"].data_ptr(), da["c"].data_ptr(),
                 self.aux[ai].n_ks, int(out_ptr), int(ld), 0, 0]
@@ -1220,12 +1228,13 @@ class WorkQueue:
             accumulate=0, omega=0.0):
         """items / ks_end / pp / job: device int32 tensors of eip_three_center_work's item list (kept alive by
         the caller); jobs_ptr: device address of the job table; costs: host array of each item's relative
-        cost (primitive pairs x auxiliary primitives)."""
+        cost (primitive pairs x auxiliary primitives), or one number for every item (the items then run in
+        their stored order, and the queue is built on the device: ``build_device``)."""
         if n_items <= 0:
             return
         m0, m1, tier = _c_int(), _c_int(), _c_int()
         per_comp = three_center_flops_per_triplet(la, lb, lc) / (2 * lc + 1)
-        costs = np.asarray(costs, dtype=np.float64)
+        costs = float(costs) if np.isscalar(costs) else np.asarray(costs, dtype=np.float64)
         c = 0
         while self.lib.eip_three_center_queue_chunk(la, lb, lc, self.f32, c, ctypes.byref(m0), ctypes.byref(m1),
                                                     ctypes.byref(tier)) == 0:
@@ -1235,7 +1244,7 @@ class WorkQueue:
                                                      int(accumulate), _c_double(omega), buf)
             if t < 0:
                 raise RuntimeError(f"eip_three_center_queue_slot({la},{lb},{lc}, chunk {c}) failed with {t}")
-            self._tier[t].append((len(self._slots), costs * per_comp * (m1.value - m0.value)))
+            self._tier[t].append((len(self._slots), costs * per_comp * (m1.value - m0.value), int(n_items)))
             self._slots.append(buf.raw)
             c += 1
         if c == 0:
@@ -1249,9 +1258,9 @@ class WorkQueue:
         cost, and the ordered runs are dealt QUEUE_RUN at a time over the 8 parts."""
         entries, ctls, tot_cost = [], [], []
         for t in range(self.ntiers):
-            lst = sorted(self._tier[t], key=lambda e: -float(e[1].mean()))
+            lst = sorted(self._tier[t], key=lambda e: -float(np.mean(e[1])))
             rows = []
-            for s, cst in lst:
+            for s, cst, _ in lst:
                 n = len(cst)
                 g = int(np.clip(np.ceil(QUEUE_ENTRY_COST / max(float(cst.mean()), 1e-30)), 1, QUEUE_MAX_GROUP))
                 first = np.arange(0, n, g, dtype=np.int64)
@@ -1269,11 +1278,71 @@ class WorkQueue:
             ctl[_CTL_COUNT:_CTL_COUNT + QUEUE_PARTS] = counts
             entries.append(np.ascontiguousarray(ent[order]))
             ctls.append(ctl)
-            tot_cost.append(sum(float(c.sum()) for _, c in lst))
+            tot_cost.append(sum(float(np.sum(c)) for _, c, _ in lst))
         # heaviest tier first: launched first, it takes the device while the lighter ones fill its tail
         self.tier_order = [int(t) for t in np.argsort(-np.array(tot_cost), kind="stable")]
         self.host = {"slots": b"".join(self._slots), "entries": entries, "ctl": ctls}
         return self
+
+    def build_device(self, stream=None):
+        """Build and upload the queue with the entries formed on the device (for item lists that live on
+        the device and may hold 10^8 items: only per-slot numbers are handled on the host).  Items run in
+        their stored order, grouped into runs by the slot's mean cost; slots by decreasing mean cost; runs
+        dealt QUEUE_RUN at a time over the 8 parts.  Returns the device buffers."""
+        torch = self.torch
+        dev = self.device
+        s = stream if stream is not None else torch.cuda.current_stream()
+        ents, ctls, tot_cost = [], [], []
+        self._staging = []
+
+        def h2d(a, dtype=None):
+            # pinned staging and an asynchronous copy on s: the host does not wait for work already
+            # queued on s (a serving step in flight); the staging buffers live as long as the queue
+            h = torch.from_numpy(np.ascontiguousarray(a)).pin_memory()
+            self._staging.append(h)
+            return h.to(dev, non_blocking=True)
+
+        with torch.cuda.stream(s):
+            for t in range(self.ntiers):
+                lst = sorted(self._tier[t], key=lambda e: -float(np.mean(e[1])))
+                slot_ids, n_it, grp, n_run = [], [], [], []
+                for sl, cst, n in lst:
+                    g = int(np.clip(np.ceil(QUEUE_ENTRY_COST / max(float(np.mean(cst)), 1e-30)), 1, QUEUE_MAX_GROUP))
+                    slot_ids.append(sl); n_it.append(n); grp.append(g); n_run.append((n + g - 1) // g)
+                N = int(sum(n_run))
+                if N:
+                    per_slot = h2d(np.array([n_run, slot_ids, grp, n_it], dtype=np.int64))
+                    nr = per_slot[0]
+                    sid = torch.repeat_interleave(per_slot[1], nr, output_size=N)
+                    start = torch.repeat_interleave(torch.cumsum(nr, 0) - nr, nr, output_size=N)
+                    g = torch.repeat_interleave(per_slot[2], nr, output_size=N)
+                    n = torch.repeat_interleave(per_slot[3], nr, output_size=N)
+                    j = torch.arange(N, dtype=torch.int64, device=dev)
+                    first = (j - start) * g
+                    count = torch.minimum(g, n - first)
+                    ent = torch.stack([sid, first, count, torch.zeros_like(first)], 1).to(torch.int32)
+                    # deal: ordered entry j goes to part p = (j // RUN) % 8, after the runs dealt to p before
+                    p = (j // QUEUE_RUN) % QUEUE_PARTS
+                    counts = np.bincount((np.arange(N) // QUEUE_RUN) % QUEUE_PARTS, minlength=QUEUE_PARTS)
+                    offs = np.concatenate([[0], np.cumsum(counts)[:-1]])
+                    dest = (h2d(offs.astype(np.int64))[p] + (j // (QUEUE_RUN * QUEUE_PARTS)) * QUEUE_RUN
+                            + j % QUEUE_RUN)
+                    out = torch.empty_like(ent)
+                    out[dest] = ent
+                else:
+                    counts = np.zeros(QUEUE_PARTS, np.int64)
+                    offs = counts
+                    out = torch.zeros((1, 4), dtype=torch.int32, device=dev)
+                ctl = np.zeros(self.ctl_words, np.int32)
+                ctl[_CTL_FIRST:_CTL_FIRST + QUEUE_PARTS] = offs
+                ctl[_CTL_COUNT:_CTL_COUNT + QUEUE_PARTS] = counts
+                ents.append(out.reshape(-1))
+                ctls.append(ctl)
+                tot_cost.append(sum(float(np.mean(c)) * nn for _, c, nn in lst))
+            self.tier_order = [int(t) for t in np.argsort(-np.array(tot_cost), kind="stable")]
+            slots = np.frombuffer(b"".join(self._slots) or bytes(8), dtype=np.uint8)
+            self.dev = {"slots": h2d(slots), "entries": ents, "ctl": h2d(np.concatenate(ctls))}
+        return self.dev
 
     def upload(self, into=None, stream=None):
         """Copy the built queue to the device (into ``into``, a previous ``self.dev``-like dict of device

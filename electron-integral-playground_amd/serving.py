@@ -27,7 +27,7 @@ from typing import Dict, List, Optional
 import numpy as np
 
 from . import pair_utility
-from .engine import ThreeCenterPlan
+from .engine import ThreeCenterPlan, WorkQueue
 
 UNIT_CAP_FRACTION = 0.25      # a unit's output may use at most this share of the budget
 INPUT_FRACTION = 0.25         # admitted jobs' resident inputs (pair tables, metric) use at most this share
@@ -195,15 +195,17 @@ class IntegralServer:
 
     def __init__(self, budget_bytes=200e9, mixed=True, threshold=1e-6, keep_outputs=False, prep_workers=12,
                  max_steps_in_flight=2, unit_cap_bytes=None, granularity="item", device="cuda", log=None,
-                 on_unit=None, estimate="envelope"):
+                 on_unit=None, estimate="envelope", persistent=True):
         """``on_unit(job, unit)``, if given, consumes every finished unit: it is called when the unit's
         step has retired, with the compute stream current and ``unit.out`` still valid (rows of
         ``unit.blocks`` from ``unit.row_off``), before the unit's output memory is reused.
         ``estimate``: the mixed split's triplet estimate -- "envelope" (a-priori term scale of every work
         item, distance-aware; whole items) or "schwarz" (the reference's bound ub_pp Q_P, item or pair
-        granularity)."""
+        granularity).  ``persistent``: each step runs the classes the persistent tier kernels hold as one
+        work queue per precision (one launch per register tier); the others with their own kernels."""
         import torch
         self.torch = torch
+        self.persistent = persistent
         self.budget = float(budget_bytes)
         self.unit_cap = min(UNIT_CAP_FRACTION * self.budget, unit_cap_bytes or float("inf"))
         self.mixed = mixed
@@ -396,7 +398,7 @@ class IntegralServer:
             self.outputs_used += step_bytes
             # per angular class: items of every unit, one job-table entry per (unit, pair class, aux class)
             work = {}
-            entries = []
+            entries, entries_vec = [], []
             for u in units:
                 plan = u.job.plan
                 for (pi, sp0, sp1), r0 in zip(u.blocks, u.row_off):
@@ -406,6 +408,7 @@ class IntegralServer:
                     for ai, ac in enumerate(plan.aux):
                         perm = plan.mixed[(pi, ai)]["perm"] if self.mixed else None
                         entries.append(plan.job_words(pi, ai, base, u.job.n_aux, perm))
+                        entries_vec.append(plan.job_words(pi, ai, base, u.job.n_aux, perm, vector=True))
                         j = len(entries) - 1
                         key = (pc.la, pc.lb, ac.lc)
                         w = work.setdefault(key, {"f64": [], "f32": [], "pp64": [], "pp32": [], "j64": [], "j32": [],
@@ -431,6 +434,14 @@ class IntegralServer:
 
             jobs = upload(np.array(entries, dtype=np.int64).reshape(-1, 16))
             keep.append(jobs)
+            queues = {}
+            if self.persistent:
+                # every class the persistent tier kernels hold goes into one queue per precision: the
+                # step is then a handful of launches whatever the number of jobs and classes
+                jobs_vec = upload(np.array(entries_vec, dtype=np.int64).reshape(-1, 16))
+                keep.append(jobs_vec)
+                queues = {prec: WorkQueue(units[0].job.plan.lib, self.device, use_f32=(prec == "f32"))
+                          for prec in ("f64", "f32")}
             for key, w in work.items():
                 plan, pi, ai = w["ref"]
                 for prec in ("f64", "f32"):
@@ -442,10 +453,21 @@ class IntegralServer:
                     jid = upload(np.repeat(np.array([j for j, _ in w["j" + prec[1:]]], dtype=np.int32),
                                            [n for _, n in w["j" + prec[1:]]]))
                     keep += [items, jid] + ([pp] if pp is not None else [])
-                    plan.launch_work(pi, ai, items, jobs, job=jid, pp=pp, layout=1,
-                                     accumulate=w["acc32"] if prec == "f32" else 0, use_f32=(prec == "f32"),
-                                     stream=self.compute_stream)
+                    acc = w["acc32"] if prec == "f32" else 0
+                    q = queues.get(prec)
+                    if q is not None and acc == 0 and q.supports(*key):
+                        q.add(*key, items, int(items.shape[0]), jobs_vec.data_ptr(), plan.mean_pairs_per_shell_pair(pi),
+                              pp=pp, job=jid, layout=1, accumulate=0, omega=plan.omega)
+                        continue
+                    plan.launch_work(pi, ai, items, jobs, job=jid, pp=pp, layout=1, accumulate=acc,
+                                     use_f32=(prec == "f32"), stream=self.compute_stream)
                     self.n_launches += 1
+            for q in queues.values():
+                if q.n_triplet_items:
+                    q.build_device(self.compute_stream)
+                    q.run(self.compute_stream)
+                    keep.append(q)
+                    self.n_launches += q.ntiers
             ev = torch.cuda.Event()
             ev.record(self.compute_stream)
         self.in_flight.append((ev, units, keep))
