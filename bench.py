@@ -606,6 +606,9 @@ def plain80_leg(args, omega=0.0):
     key, t, fl = per_class[0]
     log(f"plain80 step {dt * 1e3:.2f} ms -> {n_int / dt:.4e} integrals/s; dominant class {key} {t * 1e3:.3f} ms "
         f"{fl / t / 1e12:.2f} TFLOP/s (model)")
+    coulomb = None
+    if omega == 0.0:
+        coulomb = coulomb_fit_timing(plan, out, J2c, args.steps)
     res = {"workload": "80 atoms, 14 A box, 3s2p1d (1-6 prim), aux s-f 8 exp/l ratio 2.2" +
                        (f"; erf(omega r)/r, omega {omega}" if omega else ""),
            "n_ao": mol.n_ao, "n_aux": mol.n_aux, "value": n_int / dt, "unit": "3c integrals/s",
@@ -614,9 +617,40 @@ def plain80_leg(args, omega=0.0):
            "roofline": {"bound": "fp64-valu", "kernel": "j3c_%d%d%d" % key, "achieved": fl / t / 1e12,
                         "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": fl / t / PEAK_FP64,
                         "traffic": measured_traffic(PMC_SUMMARY_PLAIN80, key)},
-           "top_classes_ms": {"%d%d%d" % k: t * 1e3 for k, t, _ in per_class[:8]}}
+           "top_classes_ms": {"%d%d%d" % k: t * 1e3 for k, t, _ in per_class[:8]},
+           "coulomb_fit": coulomb}
     del out, plan, J2c
     torch.cuda.empty_cache()
+    return res
+
+
+def coulomb_fit_timing(plan, packed, J2c, steps):
+    """The density-fitted Coulomb matrix consumed from the packed (ij|P) on the device (engine.CoulombFit):
+    gamma = (P|ij) D, (P|Q) d = gamma through the float64 Cholesky factor, J = (ij|P) d, for a random
+    symmetric density; the factorisation is timed once, the contraction per step."""
+    import torch
+    from electron_integral_playground_amd.engine import CoulombFit
+    rng = np.random.default_rng(9)
+    c = rng.standard_normal((plan.n_ao, plan.n_ao // 4))
+    D = torch.as_tensor(c @ c.T / plan.n_ao, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fit = CoulombFit(plan, J2c=J2c)
+    torch.cuda.synchronize()
+    t_chol = time.perf_counter() - t0
+    J = fit.from_packed(packed, D)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        J = fit.from_packed(packed, D)
+    torch.cuda.synchronize()
+    t_fit = (time.perf_counter() - t0) / steps
+    res = {"what": "J = (ij|P) (P|Q)^-1 (P|kl) D_kl from the packed output on the device (two float64 "
+                   "matrix-vector products over it and a Cholesky solve)",
+           "cholesky_s": t_chol, "ms_per_fit": t_fit * 1e3,
+           "packed_GB_read_per_fit": 2 * packed.numel() * 8 / 1e9,
+           "GB_per_s": 2 * packed.numel() * 8 / t_fit / 1e9, "J_symmetric": bool(torch.equal(J, J.T))}
+    log(f"coulomb fit: {res}")
     return res
 
 

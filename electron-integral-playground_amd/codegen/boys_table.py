@@ -23,14 +23,21 @@ BOYS_T_SWITCH = 36.0       # grid covers [0, 36]
 BOYS_NMAX = 32             # columns F_0 .. F_32 (L <= 24 with K = 8)
 BOYS_K64 = 8               # Taylor terms for float64
 BOYS_K32 = 5               # Taylor terms for float32
+# Coarse grid for kernels that stage their slice of the table in LDS: h = 1/2 on [0, 36] (73 rows) and
+# BOYS_KC + 1 = 15 Taylor terms (|d| <= 1/4: truncation below (1/4)^15/15! ~ 7e-22 relative), stored per
+# order L as the columns L .. L + BOYS_KC, column-major ([L][column][row]): a wave's lanes read the same
+# column at scattered rows, which spreads them over the LDS banks.
+BOYS_C_H_INV = 2
+BOYS_KC = 14
+BOYS_LDS_LMAX = 18         # orders L with a staged slice (L + BOYS_KC <= BOYS_NMAX)
 
 
-def grid_values():
-    n_rows = int(BOYS_T_SWITCH * BOYS_H_INV) + 1
+def grid_values(h_inv=BOYS_H_INV):
+    n_rows = int(BOYS_T_SWITCH * h_inv) + 1
     rows = []
     with mpmath.workdps(40):
         for k in range(n_rows):
-            T = mpmath.mpf(k) / BOYS_H_INV
+            T = mpmath.mpf(k) / h_inv
             F = [mpmath.mpf(0)] * (BOYS_NMAX + 1)
             if k == 0:
                 F = [mpmath.mpf(1) / (2 * n + 1) for n in range(BOYS_NMAX + 1)]
@@ -74,6 +81,20 @@ def emit(path):
     lines.append("__device__ const float boys_grid32[BOYS_NROW * BOYS_NCOL] = {")
     for r in rows:
         lines.append("  " + ", ".join(repr(float(f"{v:.9e}")) + "f" for v in r) + ",")
+    lines.append("};")
+    coarse = grid_values(BOYS_C_H_INV)
+    nrc = len(coarse)
+    lines += ["#define BOYS_C_H_INV %d" % BOYS_C_H_INV, "#define BOYS_C_NROW %d" % nrc,
+              "#define BOYS_KC %d" % BOYS_KC, "#define BOYS_LDS_LMAX %d" % BOYS_LDS_LMAX,
+              "// [L][c][k] = F_{L+c}(k/%d), L = 0..%d, c = 0..%d, k = 0..%d" % (BOYS_C_H_INV, BOYS_LDS_LMAX, BOYS_KC, nrc - 1),
+              # slices padded to a multiple of 16 bytes (16-byte global->LDS copies)
+              "#define BOYS_LDS_STRIDE %d" % (((BOYS_KC + 1) * nrc + 1) // 2 * 2),
+              "__device__ __attribute__((aligned(16))) const double boys_lds64[(BOYS_LDS_LMAX + 1) * BOYS_LDS_STRIDE] = {"]
+    for L in range(BOYS_LDS_LMAX + 1):
+        for c in range(BOYS_KC + 1):
+            lines.append("  " + ", ".join(repr(coarse[k][L + c]) for k in range(nrc)) + ",")
+        if ((BOYS_KC + 1) * nrc) % 2:
+            lines.append("  0.0,")
     lines.append("};")
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")

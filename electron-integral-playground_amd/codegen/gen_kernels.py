@@ -318,15 +318,27 @@ def _aux_prefactor_lines(lc, indent="      "):
 
 
 
-def emit_j3c(la, lb, lc, m0, m1, precision="f64", stage=None):
+# Bytes of one order's coarse Boys slice staged in LDS (boys_table.py: 15 columns x 73 rows, padded)
+BOYS_LDS_BYTES = 8768
+BOYS_LDS_LMAX = 18
+
+
+def boys_in_lds(la, lb, lc, precision, boys_lds):
+    return bool(boys_lds) and precision != "f32" and la + lb + lc <= BOYS_LDS_LMAX
+
+
+def emit_j3c(la, lb, lc, m0, m1, precision="f64", stage=None, boys_lds=False):
     """Spherical three-centre kernel for aux components [m0, m1) of class (la lb | lc); ``stage`` is
-    "lds" or "dpp" (default STAGE), the way pair-matrix entries reach the lanes."""
+    "lds", "dpp" or "sgpr" (default STAGE), the way pair-matrix entries reach the lanes; ``boys_lds``:
+    the float64 Boys function reads the class's coarse table slice staged in LDS (boys64_lds) instead of
+    the fine grid in global memory."""
     spherical = True
     stage = stage or STAGE
-    if stage not in ("lds", "dpp"):
+    if stage not in ("lds", "dpp", "sgpr"):
         raise ValueError(f"unknown staging mode {stage!r}")
+    # sgpr: no staging -- the wave-uniform entries are scalar loads used as SGPR operands of the FMAs
     use_dpp = stage == "dpp" and pair_nnz(la, lb, spherical) <= DPP_MAX_ENTRIES
-    use_lds = not use_dpp and STAGE_LDS
+    use_lds = not use_dpp and STAGE_LDS and stage != "sgpr"
     nij_ = n_out(la, spherical) * n_out(lb, spherical)
     prefetch = (use_dpp and precision != "f32" and not LEIBNIZ_AUX
                 and nij_ * (m1 - m0) + _needed_R(la, lb, lc, m0, m1) <= PREFETCH_MAX_DOUBLES)
@@ -411,6 +423,20 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64", stage=None):
                    "(__attribute__((address_space(3))) void*) &sP[BUF][0], 16, 0, 0); \\")
         out.append("    } while (0)")
     nv = (nnz + 15) // 16
+    blds = boys_in_lds(la, lb, lc, precision, boys_lds)
+    if blds:
+        boff = j3c_smem_bytes(la, lb, lc, m0, m1, precision, stage, False)
+        npc = (BOYS_LDS_BYTES + 1023) // 1024
+        out.append(f"    const double* __restrict__ sB = reinterpret_cast<const double*>(smem + {boff});")
+        out.append("    {   // the coarse Boys slice of order L into LDS (64 lanes x 16 bytes per copy), awaited at once")
+        out.append(f"        const char* src_ = (const char*) (boys_lds64 + {L} * BOYS_LDS_STRIDE) + 16 * tid;")
+        for r in range(npc):
+            cond = "" if (r + 1) * 1024 <= BOYS_LDS_BYTES else f"if (16 * tid + {r * 1024} < {BOYS_LDS_BYTES}) "
+            out.append(f"        {cond}__builtin_amdgcn_global_load_lds((const void*) (src_ + {r * 1024}), "
+                       f"(__attribute__((address_space(3))) void*) (smem + {boff + r * 1024}), 16, 0, 0);")
+        out.append("        __builtin_amdgcn_s_waitcnt(0x0F70);")
+        out.append("        __syncthreads();")
+        out.append("    }")
     if use_dpp:
         out.append("    const int l16 = tid & 15;")
         out.append(f"    const auto Erow = eip::ro(JD({'E32' if f32 else 'E'}));")
@@ -520,7 +546,10 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64", stage=None):
         out.append(f"        const {ty} PQx = ({ty}) dPQx, PQy = ({ty}) dPQy, PQz = ({ty}) dPQz;")
         out.append(f"        const {ty} T = ({ty}) (zeta * (dPQx * dPQx + dPQy * dPQy + dPQz * dPQz));")
         out.append(f"        {ty} F[{L + 1}];")
-        out.append(f"        eip::boys{'32' if f32 else '64'}<{L}>(T, F);")
+        if blds:
+            out.append(f"        eip::boys64_lds<{L}>(T, sB, F);")
+        else:
+            out.append(f"        eip::boys{'32' if f32 else '64'}<{L}>(T, F);")
     if prefetch:
         pass                      # centres are loaded two ahead above
     elif use_dpp:
@@ -661,7 +690,7 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64", stage=None):
     if use_lds:
         out.append("#undef EIP_STAGE")
     out.append("}")
-    smem_bytes = j3c_smem_bytes(la, lb, lc, m0, m1, precision, stage)
+    smem_bytes = j3c_smem_bytes(la, lb, lc, m0, m1, precision, stage, boys_lds)
     # the persistent queue kernels include this source for its bodies only
     out.append("#ifndef EIP_BODIES_ONLY")
     for kname, jobs in ((name, "false"), (name + "_jobs", "true")):
@@ -676,18 +705,20 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64", stage=None):
     return name, "\n".join(out)
 
 
-def j3c_smem_bytes(la, lb, lc, m0, m1, precision="f64", stage=None):
+def j3c_smem_bytes(la, lb, lc, m0, m1, precision="f64", stage=None, boys_lds=False):
     """LDS bytes the vector three-centre body of this class chunk stages its pair-matrix rows in (two
-    buffers of K rows and their centres; 0 for DPP-staged bodies)."""
+    buffers of K rows and their centres; 0 for DPP-staged bodies), followed by the coarse Boys slice when
+    boys_lds applies to the class."""
     stage = stage or STAGE
+    extra = BOYS_LDS_BYTES if boys_in_lds(la, lb, lc, precision, boys_lds) else 0
     use_dpp = stage == "dpp" and pair_nnz(la, lb, True) <= DPP_MAX_ENTRIES
-    if use_dpp or not STAGE_LDS:
-        return 0
+    if use_dpp or not STAGE_LDS or stage == "sgpr":
+        return extra
     esize = 4 if precision == "f32" else 8
     row_bytes = pair_stride(la, lb, True) * esize
     K = max(1, min(STAGE_MAX_ROWS, STAGE_BATCH_BYTES // row_bytes))
     n_pieces = (K * row_bytes + 1023) // 1024
-    return 2 * n_pieces * 1024 + 2 * 4 * K * 8
+    return 2 * n_pieces * 1024 + 2 * 4 * K * 8 + extra
 
 
 # ---------------------------------------------------------------------------------------------

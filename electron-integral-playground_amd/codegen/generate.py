@@ -75,7 +75,8 @@ def generate(out_dir, shrink=None):
     stages = stage_table()
     os.makedirs(out_dir, exist_ok=True)
     boys_path = os.path.join(out_dir, "boys_table.inc")
-    if not os.path.exists(boys_path):
+    # (re)written when missing or from an older layout without the coarse LDS slices
+    if not os.path.exists(boys_path) or "boys_lds64" not in open(boys_path).read():
         boys_table.emit(boys_path)
 
     pair_classes = [(la, lb) for lb in range(SPEC_MAX_L + 1) for la in range(lb + 1)]

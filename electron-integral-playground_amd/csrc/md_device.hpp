@@ -91,6 +91,39 @@ __device__ __forceinline__ void boys64(const double T, double* __restrict__ F)
     }
 }
 
+// boys64 from the coarse table slice of order L staged in LDS (tab: BOYS_KC + 1 columns of BOYS_C_NROW
+// rows, column c holding F_{L+c} on the grid T0 = k / BOYS_C_H_INV): Taylor expansion of F_L about the
+// nearest grid point with BOYS_KC + 1 terms (|T - T0| <= 1/4), then the same downward recursion (and the
+// same large-T branch) as boys64.
+template <int L>
+__device__ __forceinline__ void boys64_lds(const double T, const double* __restrict__ tab, double* __restrict__ F)
+{
+    static_assert(L <= BOYS_LDS_LMAX, "order without a staged slice");
+    double eT = 0.0;
+    if constexpr (L > 0) eT = exp(-T);
+    if (T < BOYS_T_SWITCH) {
+        const int k = __double2int_rn(T * BOYS_C_H_INV);
+        const double x = k * (1.0 / BOYS_C_H_INV) - T;   // -(T - T0)
+        double acc = tab[BOYS_KC * BOYS_C_NROW + k];
+#pragma unroll
+        for (int j = BOYS_KC - 1; j >= 0; --j) acc = fma(x * (1.0 / (j + 1)), acc, tab[j * BOYS_C_NROW + k]);
+        F[L] = acc;
+        if constexpr (L > 0) {
+            const double twoT = T + T;
+#pragma unroll
+            for (int m = L - 1; m >= 0; --m) F[m] = fma(twoT, F[m + 1], eT) * (1.0 / (2 * m + 1));
+        }
+    } else {
+        const double rT = rsq_nr(T);
+        F[0] = kHalfSqrtPi * rT;                          // sqrt(pi / T) / 2
+        if constexpr (L > 0) {
+            const double half_invT = 0.5 * (rT * rT);
+#pragma unroll
+            for (int m = 0; m < L; ++m) F[m + 1] = fma((double) (2 * m + 1), F[m], -eT) * half_invT;
+        }
+    }
+}
+
 // boys64 split in two for software pipelining: boys64_fetch loads the grid row a Taylor evaluation of
 // F_L(T) reads (clamped to the last row, so it can be issued before the branch on T is known; a row
 // fetched for T beyond the grid is simply unused), boys64_eval evaluates from that row.

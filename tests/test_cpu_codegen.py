@@ -1,6 +1,7 @@
 """Kernel generator checks that need no GPU: the register layout of pair-matrix rows in the DPP staging
 mode, the two-stage two-centre contraction, and the per-class staging choice."""
 
+import math
 import re
 import sys
 
@@ -129,3 +130,35 @@ def test_pair_hermite_recurrence_is_stable_for_a_tight_second_centre():
     hrr = sum(comb(3, 3 - m) * ab ** (3 - m) * E0[(2 + m, 0)] for m in range(4))
     assert abs(rec - float(exact)) <= 1e-12 * abs(float(exact))
     assert abs(hrr - float(exact)) > 1e3 * abs(rec - float(exact))
+
+
+def test_coarse_boys_slices_reproduce_mpmath():
+    """The coarse grid the LDS-staged Boys function reads (h = 1/2, BOYS_KC + 1 Taylor terms about the
+    nearest grid point, then the downward recursion) reproduces F_m(T) to 1e-15 relative for every order
+    with a staged slice; the slice size the kernels reserve matches the table layout."""
+    import mpmath
+    import numpy as np
+    from electron_integral_playground_amd.codegen import boys_table as bt
+    rows = bt.grid_values(bt.BOYS_C_H_INV)
+    nrow = len(rows)
+    assert nrow == int(bt.BOYS_T_SWITCH * bt.BOYS_C_H_INV) + 1
+    assert gen_kernels.BOYS_LDS_BYTES == 8 * (((bt.BOYS_KC + 1) * nrow + 1) // 2 * 2)
+    assert gen_kernels.BOYS_LDS_LMAX == bt.BOYS_LDS_LMAX and bt.BOYS_LDS_LMAX + bt.BOYS_KC <= bt.BOYS_NMAX
+    rng = np.random.default_rng(4)
+    mpmath.mp.dps = 40
+    for T in list(rng.uniform(0, bt.BOYS_T_SWITCH, 40)) + [0.0, 0.2499, 0.25, 35.9]:
+        k = int(round(T * bt.BOYS_C_H_INV))
+        x = k / bt.BOYS_C_H_INV - T
+        eT = math.exp(-T)
+        for L in (0, 3, 9, bt.BOYS_LDS_LMAX):
+            acc = rows[k][L + bt.BOYS_KC]
+            for j in range(bt.BOYS_KC - 1, -1, -1):
+                acc = x * (1.0 / (j + 1)) * acc + rows[k][L + j]
+            F = [0.0] * (L + 1)
+            F[L] = acc
+            for m in range(L - 1, -1, -1):
+                F[m] = (2 * T * F[m + 1] + eT) / (2 * m + 1)
+            for m in range(L + 1):
+                ref = mpmath.mpf(1) / (2 * m + 1) if T == 0 else \
+                    mpmath.gammainc(m + 0.5, 0, T) / (2 * mpmath.mpf(T) ** (m + 0.5))
+                assert abs(F[m] - float(ref)) <= 1e-15 * float(ref), (T, L, m)
