@@ -32,6 +32,7 @@ import os
 import pathlib
 import re
 import sys
+import threading
 import time
 
 import numpy as np
@@ -808,12 +809,15 @@ def serving_leg(n_jobs, rate, budget=200e9, mixed=True, part=0, latency_file=Non
     return res
 
 
+BATCH_BUILDERS = 4      # host threads building wave plans concurrently
+
+
 def batched_leg(budget=200e9, n_mol=256, repeats=2):
     """Batched workload: 256 molecules, atom counts lognormal (median 20, sigma 0.5, clipped 4-120), plain
     generators (orbital l <= 2, auxiliary l <= 3), box scaled with the atom count.  Molecules are grouped
     in arrival order into waves whose outputs fit the device budget, all written into one shared output
-    buffer.  Host work is pipelined: 8 threads form the pair lists, one thread builds the next wave's
-    BatchPlan and work queue while the device runs the current one.  Each wave's (ij|P) runs in the
+    buffer.  Host work is pipelined: 8 threads form the pair lists, BATCH_BUILDERS threads build the wave
+    plans and work queues ahead while the device runs the current wave.  Each wave's (ij|P) runs in the
     persistent tier kernels over its work queue, from ONE captured HIP graph replayed for every wave (the
     queue is uploaded into the graph's buffers); the zeroing of screened rows and every molecule's (P|Q)
     run beside it on side streams.  Every wave then runs once more as one launch per class (the previous
@@ -843,8 +847,8 @@ def batched_leg(budget=200e9, n_mol=256, repeats=2):
     prep_pool = cf.ThreadPoolExecutor(8)
     pair_futs = [prep_pool.submit(lambda m: (pair_utility.form_pair_data(m, 1e-14),
                                              pair_utility.form_auxiliary_pair_data(m)), mol) for mol in mols]
-    builder = cf.ThreadPoolExecutor(1)
-    build_stream = torch.cuda.Stream()
+    builder = cf.ThreadPoolExecutor(BATCH_BUILDERS)
+    tls = threading.local()
     host_wait = {"pairs": 0.0, "plan": 0.0}
 
     def build(w):
@@ -852,10 +856,12 @@ def batched_leg(budget=200e9, n_mol=256, repeats=2):
         items = [(mols[i],) + pair_futs[i].result() for i in w]
         host_wait["pairs"] += time.perf_counter() - t
         t = time.perf_counter()
-        with torch.cuda.stream(build_stream):
+        if not hasattr(tls, "stream"):
+            tls.stream = torch.cuda.Stream()
+        with torch.cuda.stream(tls.stream):
             bp = BatchPlan(items)
             q = bp.work_queue(out)
-            build_stream.synchronize()
+            tls.stream.synchronize()
         host_wait["plan"] += time.perf_counter() - t
         return bp, q
 
@@ -865,13 +871,15 @@ def batched_leg(budget=200e9, n_mol=256, repeats=2):
     checksum = checksum_cls = 0.0
     rel_diff = 0.0
     t_wait = t_cmp = 0.0
-    nxt = builder.submit(build, waves[0])
+    # plans of up to BATCH_BUILDERS waves ahead are built concurrently (their inputs are small beside the
+    # shared output buffer)
+    futs = [builder.submit(build, w) for w in waves[:BATCH_BUILDERS]]
     for k, w in enumerate(waves):
         tw = time.perf_counter()
-        bp, q = nxt.result()
+        bp, q = futs[k].result()
         t_wait += time.perf_counter() - tw
-        if k + 1 < len(waves):
-            nxt = builder.submit(build, waves[k + 1])
+        if k + BATCH_BUILDERS < len(waves):
+            futs.append(builder.submit(build, waves[k + BATCH_BUILDERS]))
         stream.wait_stream(torch.cuda.current_stream())
         best = None
         for _ in range(repeats):

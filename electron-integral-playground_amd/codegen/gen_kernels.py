@@ -24,8 +24,6 @@ turns the MD recursions into straight-line C, but targeting gfx950 and restructu
   (reference two_center.cpp:25-212).
 """
 
-from fractions import Fraction
-from math import comb
 
 from .tables import (aux_transform, cart_list, fmt, herm_list, n_out, ncart, pair_out_rows,
                      pair_support, SqrtRational)
@@ -223,68 +221,6 @@ def _emit_R(lines, L, needed0, ty, base="R000_", prefix="r"):
 
 
 # ---------------------------------------------------------------------------------------------
-# Auxiliary side by Hobson + Leibniz.  The spherical auxiliary function's Hermite expansion is
-# Y_m(d/dx) with Y_m(x) = sum_s c_ms x^s a solid harmonic (only |s| = lc), so
-#   V_m(t) = sum_s c_ms R_{t+s} = d^t [ Y_m(x) G(x) ],  G = (2 d/d|x|^2)^lc R_000 = R^lc_000,
-# and by Leibniz
-#   V_m(t) = sum_{t1 <= t} C(t, t1) (d^t1 Y_m)(x) R^lc_{t-t1}(x),
-# which needs the Hermite integrals of order lc only up to |u| = la + lb (instead of all R^0 up to
-# la + lb + lc): far fewer live registers, and no cancellation inside the harmonic combination.
-# ---------------------------------------------------------------------------------------------
-
-# Switched off: the first GPU test run of kernels generated this way ended with a GPU memory fault in a
-# drop-in three-centre test (orbital s-d, auxiliary s-f); the cause has not been found, so the kernels are
-# generated by the plain Hobson combination of R^0 below.
-LEIBNIZ_AUX = False
-
-
-def _falling(n, k):
-    r = 1
-    for i in range(k):
-        r *= n - i
-    return r
-
-
-def harmonic_derivatives(lc, m):
-    """{t1: {monomial: Fraction}}: the derivatives d^t1 Y_m of the auxiliary row m's polynomial
-    Y_m(x) = sum_s q_s x^s (the common sqrt(r_m) is left out), all with at least one non-zero term."""
-    _, terms = aux_transform(lc, True)[m]
-    out = {}
-    for s, (q, j) in terms.items():
-        assert j == 0
-        for a in range(s[0] + 1):
-            for b in range(s[1] + 1):
-                for c in range(s[2] + 1):
-                    f = _falling(s[0], a) * _falling(s[1], b) * _falling(s[2], c)
-                    mono = (s[0] - a, s[1] - b, s[2] - c)
-                    d = out.setdefault((a, b, c), {})
-                    d[mono] = d.get(mono, Fraction(0)) + q * f
-    out = {t1: {k: v for k, v in d.items() if v != 0} for t1, d in out.items()}
-    return {t1: d for t1, d in out.items() if d}
-
-
-def _leq(a, b):
-    return a[0] <= b[0] and a[1] <= b[1] and a[2] <= b[2]
-
-
-def _binom3(t, t1):
-    return comb(t[0], t1[0]) * comb(t[1], t1[1]) * comb(t[2], t1[2])
-
-
-def leibniz_plan(la, lb, lc, ms):
-    """(derivatives per m, needed u = t - t1) for the pair Hermite indices of class (la, lb)."""
-    ts = {t for t, _ in pair_layout(la, lb, True)}
-    ders = {m: harmonic_derivatives(lc, m) for m in ms}
-    need_u = set()
-    for t in ts:
-        for m in ms:
-            for t1 in ders[m]:
-                if _leq(t1, t):
-                    need_u.add((t[0] - t1[0], t[1] - t1[1], t[2] - t1[2]))
-    return ders, need_u
-
-
-# ---------------------------------------------------------------------------------------------
 # Three-centre kernel
 # ---------------------------------------------------------------------------------------------
 
@@ -340,7 +276,7 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64", stage=None, boys_lds=False):
     use_dpp = stage == "dpp" and pair_nnz(la, lb, spherical) <= DPP_MAX_ENTRIES
     use_lds = not use_dpp and STAGE_LDS and stage != "sgpr"
     nij_ = n_out(la, spherical) * n_out(lb, spherical)
-    prefetch = (use_dpp and precision != "f32" and not LEIBNIZ_AUX
+    prefetch = (use_dpp and precision != "f32"
                 and nij_ * (m1 - m0) + _needed_R(la, lb, lc, m0, m1) <= PREFETCH_MAX_DOUBLES)
     f32 = precision == "f32"
     ty = "float" if f32 else "double"
@@ -564,51 +500,7 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64", stage=None, boys_lds=False):
     for n in range(1, L + 1):
         out.append(f"        const {ty} pf_{n} = {'(' + ty + ') pf' if n == 1 else f'pf_{n - 1}'} * m2z;")
         out.append(f"        const {ty} R000_{n} = pf_{n} * F[{n}];")
-    if LEIBNIZ_AUX and lc > 0:
-        ders, need_u = leibniz_plan(la, lb, lc, ms)
-        lab = la + lb
-        for n in range(lab + 1):
-            out.append(f"        const {ty} Q000_{n} = R000_{n + lc};")
-        _emit_R(out, lab, need_u, ty, base="Q000_", prefix="q")
-        # powers of PQ and the monomials the derivatives use
-        monos = sorted({mono for m in ms for d in ders[m].values() for mono in d})
-        for ax in "xyz":
-            top = max([mono["xyz".index(ax)] for mono in monos] + [0])
-            for e in range(2, top + 1):
-                out.append(f"        const {ty} PQ{ax}_{e} = PQ{ax}_{e - 1} * PQ{ax};" if e > 2
-                           else f"        const {ty} PQ{ax}_2 = PQ{ax} * PQ{ax};")
-
-        def pw(ax, e):
-            return None if e == 0 else (f"PQ{ax}" if e == 1 else f"PQ{ax}_{e}")
-
-        mono_name = {}
-        for mono in monos:
-            fac = [f for f in (pw("x", mono[0]), pw("y", mono[1]), pw("z", mono[2])) if f]
-            if not fac:
-                continue
-            if len(fac) == 1:
-                mono_name[mono] = fac[0]
-            else:
-                nm = f"M_{mono[0]}_{mono[1]}_{mono[2]}"
-                out.append(f"        const {ty} {nm} = {' * '.join(fac)};")
-                mono_name[mono] = nm
-        dname = {}
-        for m in ms:
-            r = aux_rows[m][0]
-            for t1, poly in ders[m].items():
-                pieces = []
-                for mono, qv in sorted(poly.items()):
-                    coef = SqrtRational(r, qv).to_float()
-                    term = mono_name.get(mono)
-                    if term is None:
-                        pieces.append(lit(coef))
-                    else:
-                        pieces.append(term if coef == 1.0 else f"{lit(coef)} * {term}")
-                nm = f"D_{m}_{t1[0]}_{t1[1]}_{t1[2]}"
-                out.append(f"        const {ty} {nm} = {' + '.join(pieces)};")
-                dname[(m, t1)] = nm
-    else:
-        _emit_R(out, L, needed, ty)
+    _emit_R(out, L, needed, ty)
     # contraction, t-major
     k = 0
     for t in ts:
@@ -616,20 +508,11 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64", stage=None, boys_lds=False):
         for m in ms:
             r, terms = aux_rows[m]
             pieces = []
-            if LEIBNIZ_AUX and lc > 0:
-                for t1 in ders[m]:
-                    if not _leq(t1, t):
-                        continue
-                    c = _binom3(t, t1)
-                    qn = _r_name(0, (t[0] - t1[0], t[1] - t1[1], t[2] - t1[2]), "q")
-                    prod = f"{dname[(m, t1)]} * {qn}"
-                    pieces.append(prod if c == 1 else f"{lit(float(c))} * {prod}")
-            else:
-                for s, (qf, j) in terms.items():
-                    assert j == 0
-                    coef = SqrtRational(r, qf).to_float()
-                    rn = _r_name(0, (t[0] + s[0], t[1] + s[1], t[2] + s[2]))
-                    pieces.append(rn if coef == 1.0 else f"{lit(coef)} * {rn}")
+            for s, (qf, j) in terms.items():
+                assert j == 0
+                coef = SqrtRational(r, qf).to_float()
+                rn = _r_name(0, (t[0] + s[0], t[1] + s[1], t[2] + s[2]))
+                pieces.append(rn if coef == 1.0 else f"{lit(coef)} * {rn}")
             vn = f"V_{t[0]}_{t[1]}_{t[2]}_{m}"
             out.append(f"        const {ty} {vn} = {' + '.join(pieces)};")
             vnames[m] = vn
@@ -881,11 +764,7 @@ def emit_j2c(li, lj):
 # ---------------------------------------------------------------------------------------------
 
 def _needed_R(la, lb, lc, m0, m1):
-    """Number of Hermite integrals (and, with the Leibniz form, harmonic derivatives) a chunk [m0, m1)
-    of class (la lb|lc) keeps live."""
-    if LEIBNIZ_AUX and lc > 0:
-        ders, need_u = leibniz_plan(la, lb, lc, range(m0, m1))
-        return len(need_u) + sum(len(d) for d in ders.values())
+    """Number of Hermite integrals a chunk [m0, m1) of class (la lb|lc) keeps live."""
     layout = pair_layout(la, lb, True)
     ts = {t for t, _ in layout}
     rows = aux_transform(lc, True)

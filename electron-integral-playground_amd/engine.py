@@ -1340,7 +1340,7 @@ class WorkQueue:
                 ctls.append(ctl)
                 tot_cost.append(sum(float(np.mean(c)) * nn for _, c, nn in lst))
             self.tier_order = [int(t) for t in np.argsort(-np.array(tot_cost), kind="stable")]
-            slots = np.frombuffer(b"".join(self._slots) or bytes(8), dtype=np.uint8)
+            slots = np.frombuffer(bytearray(b"".join(self._slots) or bytes(8)), dtype=np.uint8)
             self.dev = {"slots": h2d(slots), "entries": ents, "ctl": h2d(np.concatenate(ctls))}
         return self.dev
 
