@@ -628,7 +628,8 @@ def plain80_leg(args, omega=0.0):
 def coulomb_fit_timing(plan, packed, J2c, steps):
     """The density-fitted Coulomb matrix consumed from the packed (ij|P) on the device (engine.CoulombFit):
     gamma = (P|ij) D, (P|Q) d = gamma through the float64 Cholesky factor, J = (ij|P) d, for a random
-    symmetric density; the factorisation is timed once, the contraction per step."""
+    symmetric density; the factorisation (Cholesky and L^-1) is timed once, the contraction per step,
+    beside the same fit through two triangular vector solves per step."""
     import torch
     from electron_integral_playground_amd.engine import CoulombFit
     rng = np.random.default_rng(9)
@@ -639,16 +640,23 @@ def coulomb_fit_timing(plan, packed, J2c, steps):
     fit = CoulombFit(plan, J2c=J2c)
     torch.cuda.synchronize()
     t_chol = time.perf_counter() - t0
-    J = fit.from_packed(packed, D)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        J = fit.from_packed(packed, D)
-    torch.cuda.synchronize()
-    t_fit = (time.perf_counter() - t0) / steps
-    res = {"what": "J = (ij|P) (P|Q)^-1 (P|kl) D_kl from the packed output on the device (two float64 "
-                   "matrix-vector products over it and a Cholesky solve)",
-           "cholesky_s": t_chol, "ms_per_fit": t_fit * 1e3,
+
+    def timed(f):
+        J = f.from_packed(packed, D)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            J = f.from_packed(packed, D)
+        torch.cuda.synchronize()
+        return J, (time.perf_counter() - t0) / steps
+
+    J, t_fit = timed(fit)
+    J2, t_trsv = timed(CoulombFit(plan, L=fit.L, inverse=False))
+    diff = float((J - J2).abs().max() / J2.abs().max())
+    res = {"what": "J = (ij|P) (P|Q)^-1 (P|kl) D_kl from the packed output on the device (two streaming "
+                   "passes over it, eip_weighted_column_sums and eip_row_dots, and a Cholesky solve)",
+           "factorisation_s": t_chol, "ms_per_fit": t_fit * 1e3,
+           "ms_per_fit_triangular_solves": t_trsv * 1e3, "rel_diff_inverse_vs_triangular_solves": diff,
            "packed_GB_read_per_fit": 2 * packed.numel() * 8 / 1e9,
            "GB_per_s": 2 * packed.numel() * 8 / t_fit / 1e9, "J_symmetric": bool(torch.equal(J, J.T))}
     log(f"coulomb fit: {res}")

@@ -300,6 +300,94 @@ __global__ void __launch_bounds__(256) copy_probe_kernel(const double* __restric
     for (long long i = (long long) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
+// ---- Consumers of an (ij|P) output on the device: a row-major float64 matrix A[n_rows][ld] (its first
+// n_cols columns).  Both stream A once from HBM with coalesced loads and sum in a fixed order, so their
+// results do not depend on scheduling.
+
+// gamma[c] (+)= sum_r w[r] A[r][c], first pass: block (b, t) sums rows [b * rpb, (b + 1) * rpb) of the
+// column tile t (256 threads x kColPerThread columns, consecutive threads on consecutive columns) into
+// partial[b][c].
+constexpr int kColPerThread = 4;
+constexpr int kColTile = 256 * kColPerThread;
+__global__ void __launch_bounds__(256) weighted_col_sum_partial(const double* __restrict__ A, long long n_rows, int n_cols,
+                                                                long long ld, const double* __restrict__ w,
+                                                                long long rpb, double* __restrict__ partial)
+{
+    const long long r0 = (long long) blockIdx.x * rpb;
+    const long long r1 = r0 + rpb < n_rows ? r0 + rpb : n_rows;
+    const int c0 = (int) blockIdx.y * kColTile + (int) threadIdx.x;
+    double acc[kColPerThread] = {};
+    bool in[kColPerThread];
+#pragma unroll
+    for (int k = 0; k < kColPerThread; ++k) in[k] = c0 + 256 * k < n_cols;
+    // four rows per step: all 16 loads are issued before their FMAs (bytes in flight hide HBM latency)
+    long long r = r0;
+    for (; r + 4 <= r1; r += 4) {
+        double v[4][kColPerThread];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double* __restrict__ row = A + (r + q) * ld + c0;
+#pragma unroll
+            for (int k = 0; k < kColPerThread; ++k) v[q][k] = in[k] ? __builtin_nontemporal_load(row + 256 * k) : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double wr = w[r + q];
+#pragma unroll
+            for (int k = 0; k < kColPerThread; ++k) acc[k] = fma(wr, v[q][k], acc[k]);
+        }
+    }
+    for (; r < r1; ++r) {
+        const double wr = w[r];
+        const double* __restrict__ row = A + r * ld + c0;
+#pragma unroll
+        for (int k = 0; k < kColPerThread; ++k)
+            if (in[k]) acc[k] = fma(wr, __builtin_nontemporal_load(row + 256 * k), acc[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < kColPerThread; ++k)
+        if (in[k]) partial[(long long) blockIdx.x * n_cols + c0 + 256 * k] = acc[k];
+}
+
+// second pass: gamma[c] (+)= sum_b partial[b][c], b in order.
+__global__ void __launch_bounds__(256) weighted_col_sum_final(const double* __restrict__ partial, int n_blocks, int n_cols,
+                                                              int accumulate, double* __restrict__ gamma)
+{
+    const int c = (int) (blockIdx.x * blockDim.x + threadIdx.x);
+    if (c >= n_cols) return;
+    double acc = 0.0;
+    for (int b = 0; b < n_blocks; ++b) acc += partial[(long long) b * n_cols + c];
+    gamma[c] = accumulate ? gamma[c] + acc : acc;
+}
+
+// out[r] = sum_c A[r][c] x[c]: one wave per row (lanes stride the columns, 512 contiguous bytes per step),
+// a butterfly over the lanes, lane 0 stores.
+__global__ void __launch_bounds__(256) row_dot_kernel(const double* __restrict__ A, long long n_rows, int n_cols,
+                                                      long long ld, const double* __restrict__ x,
+                                                      double* __restrict__ out)
+{
+    const int lane = (int) threadIdx.x & 63;
+    const long long waves = (long long) gridDim.x * (blockDim.x / 64);
+    for (long long r = (long long) blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; r < n_rows; r += waves) {
+        const double* __restrict__ row = A + r * ld;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        int c = lane;
+        for (; c + 192 < n_cols; c += 256) {
+            const double v0 = __builtin_nontemporal_load(row + c), v1 = __builtin_nontemporal_load(row + c + 64);
+            const double v2 = __builtin_nontemporal_load(row + c + 128), v3 = __builtin_nontemporal_load(row + c + 192);
+            a0 = fma(v0, x[c], a0);
+            a1 = fma(v1, x[c + 64], a1);
+            a2 = fma(v2, x[c + 128], a2);
+            a3 = fma(v3, x[c + 192], a3);
+        }
+        for (; c < n_cols; c += 64) a0 = fma(__builtin_nontemporal_load(row + c), x[c], a0);
+        double v = (a0 + a1) + (a2 + a3);
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+        if (lane == 0) out[r] = v;
+    }
+}
+
 __global__ void __launch_bounds__(256) zero_rows_kernel(const int* __restrict__ rows, int n_rows, long long ld,
                                                         double* __restrict__ out)
 {
@@ -963,6 +1051,61 @@ int eip_three_center_queue_run(int use_f32, int tier, const void* d_slots, const
     if (!device_accessible(d_slots) || !device_accessible(d_entries) || !device_accessible(d_ctl)) return 4;
     J3cQueueArgs q{static_cast<const J3cQueueSlot*>(d_slots), reinterpret_cast<const J3cQueueEntry*>(d_entries), d_ctl};
     if (eip_launch_j3cq(use_f32 ? 1 : 0, tier, q, grid, (hipStream_t) stream) != 0) return 3;
+    return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+// Rows per block of the first pass of eip_weighted_column_sums and the blocks it uses: about 8192
+// workgroups over the device, at least 64 rows each.
+static void col_sum_shape(long long n_rows, int n_cols, long long* rpb, int* n_blocks)
+{
+    const int tiles = (n_cols + kColTile - 1) / kColTile;
+    long long want = 8192 / (tiles > 0 ? tiles : 1);
+    if (want < 1) want = 1;
+    long long r = (n_rows + want - 1) / want;
+    if (r < 64) r = 64;
+    *rpb = r;
+    *n_blocks = (int) ((n_rows + r - 1) / r);
+}
+
+long long eip_weighted_column_sums_workspace(long long n_rows, int n_cols)
+{
+    if (n_rows <= 0 || n_cols <= 0) return 0;
+    long long rpb = 0;
+    int nb = 0;
+    col_sum_shape(n_rows, n_cols, &rpb, &nb);
+    return (long long) nb * n_cols;
+}
+
+int eip_weighted_column_sums(const double* d_A, long long n_rows, int n_cols, long long ld, const double* d_w,
+                             double* d_work, long long work_doubles, int accumulate, double* d_out, void* stream)
+{
+    if (n_cols <= 0) return 0;
+    if (!d_out || ld < n_cols || n_rows < 0 || (n_rows > 0 && (!d_A || !d_w))) return 4;
+    if (n_rows == 0) {   // no rows: gamma = 0 (or unchanged), by the second pass over no partials
+        hipLaunchKernelGGL(weighted_col_sum_final, dim3((n_cols + 255) / 256), dim3(256), 0, (hipStream_t) stream,
+                           d_out, 0, n_cols, accumulate, d_out);
+        return hipGetLastError() == hipSuccess ? 0 : 2;
+    }
+    long long rpb = 0;
+    int nb = 0;
+    col_sum_shape(n_rows, n_cols, &rpb, &nb);
+    if (!d_work || work_doubles < (long long) nb * n_cols) return 4;
+    const dim3 grid1((unsigned) nb, (unsigned) ((n_cols + kColTile - 1) / kColTile));
+    hipLaunchKernelGGL(weighted_col_sum_partial, grid1, dim3(256), 0, (hipStream_t) stream, d_A, n_rows, n_cols, ld, d_w,
+                       rpb, d_work);
+    hipLaunchKernelGGL(weighted_col_sum_final, dim3((n_cols + 255) / 256), dim3(256), 0, (hipStream_t) stream, d_work,
+                       nb, n_cols, accumulate, d_out);
+    return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int eip_row_dots(const double* d_A, long long n_rows, int n_cols, long long ld, const double* d_x, double* d_out,
+                 void* stream)
+{
+    if (n_rows <= 0) return 0;
+    if (!d_A || !d_x || !d_out || n_cols <= 0 || ld < n_cols) return 4;
+    const long long blocks = (n_rows + 3) / 4;
+    const int grid = (int) (blocks < 8192 ? blocks : 8192);
+    hipLaunchKernelGGL(row_dot_kernel, dim3(grid), dim3(256), 0, (hipStream_t) stream, d_A, n_rows, n_cols, ld, d_x, d_out);
     return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 

@@ -1051,6 +1051,9 @@ def metric_cholesky(J2c):
     return L
 
 
+INVERSE_BLOCK = 512      # right-hand sides per triangular solve when CoulombFit forms L^-1
+
+
 class CoulombFit:
     """Density-fitted Coulomb matrix from (ij|P) consumed on the device as it is produced:
 
@@ -1061,9 +1064,12 @@ class CoulombFit:
     matrix-vector products over it); ``streamed`` consumes compact blocks of shell pairs streamed through
     one device buffer, evaluating the integrals twice (gamma in the first pass, J in the second) so that
     nothing larger than one block is ever materialised -- outputs of terabytes need no more than the
-    block.  All products are float64 library matrix-vector products on the block in HBM."""
+    block.  The two contractions are the library's streaming kernels (eip_weighted_column_sums,
+    eip_row_dots: one pass over the output in HBM, fixed summation order)."""
 
-    def __init__(self, plan, J2c=None, L=None):
+    def __init__(self, plan, J2c=None, L=None, inverse=True):
+        """inverse: also form L^-1 once (a triangular solve against the identity), so each metric solve is
+        two matrix-vector products over it instead of two latency-bound triangular vector solves."""
         self.plan = plan
         torch = plan.torch
         if L is None:
@@ -1071,27 +1077,67 @@ class CoulombFit:
                 J2c = plan.two_center()
             L = metric_cholesky(J2c)
         self.L = L
+        self.Linv = None
+        if inverse:
+            # L^-1 column block by column block (the solver library's workspace stays small): block
+            # [c0, c1) solves L X = I[:, c0:c1], whose rows above c0 are zero
+            n = L.shape[0]
+            self.Linv = torch.zeros((n, n), dtype=L.dtype, device=L.device)
+            for c0 in range(0, n, INVERSE_BLOCK):
+                c1 = min(n, c0 + INVERSE_BLOCK)
+                rhs = torch.zeros((n - c0, c1 - c0), dtype=L.dtype, device=L.device)
+                rhs[:c1 - c0].fill_diagonal_(1.0)
+                self.Linv[c0:, c0:c1] = torch.linalg.solve_triangular(L[c0:, c0:], rhs, upper=False)
         n = plan.n_ao
         i, j = np.tril_indices(n)
         self.tri_i = torch.as_tensor(i, device=plan.device)
         self.tri_j = torch.as_tensor(j, device=plan.device)
 
     def solve(self, gamma):
-        """d = (P|Q)^-1 gamma through the float64 Cholesky factor."""
-        return self.plan.torch.cholesky_solve(gamma.view(-1, 1), self.L).view(-1)
+        """d = (P|Q)^-1 gamma through the float64 Cholesky factor (d = L^-T (L^-1 gamma))."""
+        torch = self.plan.torch
+        if self.Linv is not None:
+            return torch.mv(self.Linv.t(), torch.mv(self.Linv, gamma))
+        return torch.cholesky_solve(gamma.view(-1, 1), self.L).view(-1)
 
     def _weights(self, D):
         torch = self.plan.torch
         return D[self.tri_i, self.tri_j] * torch.where(self.tri_i == self.tri_j, 1.0, 2.0).to(D.dtype)
 
+    def _col_sums(self, A, w, gamma, accumulate, stream=None):
+        """gamma (+)= A^T w by the library's streaming column-sum kernels (A row-major, float64)."""
+        torch = self.plan.torch
+        lib = self.plan.lib
+        n_rows, n_cols = A.shape
+        need = int(lib.eip_weighted_column_sums_workspace(_c_ll(n_rows), n_cols))
+        work = self.__dict__.get("_work")
+        if work is None or work.numel() < need:
+            work = self._work = torch.empty(max(need, 1), dtype=torch.float64, device=A.device)
+        rc = lib.eip_weighted_column_sums(_ptr(A), _c_ll(n_rows), n_cols, _c_ll(A.stride(0)), _ptr(w), _ptr(work),
+                                          _c_ll(work.numel()), int(accumulate), _ptr(gamma), _stream_handle(stream))
+        if rc != 0:
+            raise RuntimeError(f"eip_weighted_column_sums failed with {rc}")
+        return gamma
+
+    def _row_dots(self, A, x, out, stream=None):
+        n_rows, n_cols = A.shape
+        rc = self.plan.lib.eip_row_dots(_ptr(A), _c_ll(n_rows), n_cols, _c_ll(A.stride(0)), _ptr(x), _ptr(out),
+                                        _stream_handle(stream))
+        if rc != 0:
+            raise RuntimeError(f"eip_row_dots failed with {rc}")
+        return out
+
     def gamma_packed(self, packed, D):
         """gamma_P = sum_ij (P|ij) D_ij from the packed output J3c[tri(i, j), P] (i >= j)."""
-        return self.plan.torch.mv(packed.t(), self._weights(D))
+        torch = self.plan.torch
+        gamma = torch.empty(packed.shape[1], dtype=torch.float64, device=packed.device)
+        return self._col_sums(packed, self._weights(D).contiguous(), gamma, False)
 
     def contract_packed(self, packed, d):
         """J_ij = sum_P (ij|P) d_P from the packed output (symmetric (n_ao, n_ao))."""
         torch = self.plan.torch
-        jp = torch.mv(packed, d)
+        jp = self._row_dots(packed, d.contiguous(), torch.empty(packed.shape[0], dtype=torch.float64,
+                                                                 device=packed.device))
         n = self.plan.n_ao
         J = torch.zeros((n, n), dtype=jp.dtype, device=jp.device)
         J[self.tri_i, self.tri_j] = jp
@@ -1138,7 +1184,7 @@ class CoulombFit:
             n = plan.block_rows(pi, sp0, sp1)
             plan.run_block(pi, sp0, sp1, buf, mixed=mixed, stream=stream)
             I, J, w = self._block_index(pi, sp0, sp1)
-            gamma.addmv_(buf[:n].t(), D[I, J] * w)
+            self._col_sums(buf[:n], (D[I, J] * w).contiguous(), gamma, True, stream)
         return gamma
 
     def contract_streamed(self, d, block_bytes, mixed=False, stream=None):
@@ -1151,7 +1197,7 @@ class CoulombFit:
             n = plan.block_rows(pi, sp0, sp1)
             plan.run_block(pi, sp0, sp1, buf, mixed=mixed, stream=stream)
             I, J, _ = self._block_index(pi, sp0, sp1)
-            jr = torch.mv(buf[:n], d)
+            jr = self._row_dots(buf[:n], d.contiguous(), torch.empty(n, dtype=torch.float64, device=d.device), stream)
             out[I, J] = jr
             out[J, I] = jr
         return out
@@ -1218,11 +1264,17 @@ class WorkQueue:
         self.dev = None
 
     def supports(self, la, lb, lc):
-        """Whether the class has bodies in the tier kernels (classes whose bodies need more registers than
-        a persistent loop leaves -- the highest (df|g) and (ff|x) chunks -- keep their own kernels)."""
+        """Whether the class runs in the tier kernels: it has bodies there (classes whose bodies need more
+        registers than a persistent loop leaves -- the highest (df|g) and (ff|x) chunks -- do not) and its
+        current variant for this precision is a vector one (a class tuned to the matrix cores keeps its
+        own, faster kernels)."""
         m0, m1, tier = _c_int(), _c_int(), _c_int()
-        return self.lib.eip_three_center_queue_chunk(la, lb, lc, self.f32, 0, ctypes.byref(m0), ctypes.byref(m1),
-                                                     ctypes.byref(tier)) == 0
+        if self.lib.eip_three_center_queue_chunk(la, lb, lc, self.f32, 0, ctypes.byref(m0), ctypes.byref(m1),
+                                                 ctypes.byref(tier)) != 0:
+            return False
+        if self.f32:
+            return self.lib.eip_three_center_f32_uses_bf16(la, lb, lc) != 1
+        return self.lib.eip_three_center_uses_mfma(la, lb, lc) != 1
 
     def add(self, la, lb, lc, items, n_items, jobs_ptr, costs, ks_end=None, pp=None, job=None, layout=0,
             accumulate=0, omega=0.0):

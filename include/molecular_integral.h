@@ -206,6 +206,20 @@ int eip_three_center_queue_grid(int use_f32, int tier, int waves_per_simd);
 int eip_three_center_queue_run(int use_f32, int tier, const void* d_slots, const int* d_entries, int* d_ctl,
                                int grid, void* stream);
 
+/* Consumers of an (ij|P) output on the device, for the density-fitted Coulomb matrix (no counterpart in the
+ * reference, which stops at the integrals): A is a row-major float64 matrix A[n_rows][ld] (its first n_cols
+ * columns), e.g. the packed output (rows tri(i, j), columns P) or a compact block of shell pairs.  Both
+ * stream A once from HBM and sum in a fixed order (results independent of scheduling).
+ * eip_weighted_column_sums: out[c] = sum_r w[r] A[r][c] (accumulate = 1: out[c] += ...), through a
+ *   caller-provided workspace of at least eip_weighted_column_sums_workspace(n_rows, n_cols) doubles.
+ * eip_row_dots: out[r] = sum_c A[r][c] x[c].
+ * Return 4 on bad arguments. */
+long long eip_weighted_column_sums_workspace(long long n_rows, int n_cols);
+int eip_weighted_column_sums(const double* d_A, long long n_rows, int n_cols, long long ld, const double* d_w,
+                             double* d_work, long long work_doubles, int accumulate, double* d_out, void* stream);
+int eip_row_dots(const double* d_A, long long n_rows, int n_cols, long long ld, const double* d_x, double* d_out,
+                 void* stream);
+
 /* d_dst[i] = d_src[i] for i < n (device pointers): a plain copy kernel of known size, 8 n bytes read and
  * 8 n bytes written, with one 8-byte load and store per lane.  No counterpart in the reference: it
  * calibrates the profiler's HBM byte counters (FETCH_SIZE, WRITE_SIZE) for the traffic figures. */

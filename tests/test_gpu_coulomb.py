@@ -89,3 +89,46 @@ def test_coulomb_fit_streamed_equals_packed(mixed):
           flush=True)
     bar = 1e-10 if mixed else 1e-13
     assert eg <= bar and eJ <= bar
+
+
+@pytest.mark.parametrize("n_rows,n_cols,ld", [(100003, 1537, 1600), (64, 1024, 1024), (1, 7, 9), (0, 300, 300)])
+def test_consumer_kernels_match_torch(n_rows, n_cols, ld):
+    """eip_weighted_column_sums / eip_row_dots against float64 torch products, on ragged shapes (columns
+    not a multiple of the tile, padded rows, a single row, no rows), both column-sum modes."""
+    import ctypes
+    import torch
+    from electron_integral_playground_amd import library
+    lib = library()
+    g = torch.Generator(device="cuda").manual_seed(n_rows + n_cols)
+    full = torch.randn((max(n_rows, 1), ld), dtype=torch.float64, device="cuda", generator=g)
+    A = full[:n_rows, :n_cols]
+    w = torch.randn(n_rows, dtype=torch.float64, device="cuda", generator=g)
+    x = torch.randn(n_cols, dtype=torch.float64, device="cuda", generator=g)
+    V = ctypes.c_void_p
+    s = V(torch.cuda.current_stream().cuda_stream)
+    need = lib.eip_weighted_column_sums_workspace(ctypes.c_longlong(n_rows), n_cols)
+    work = torch.empty(max(need, 1), dtype=torch.float64, device="cuda")
+    gamma = torch.full((n_cols,), 3.0, dtype=torch.float64, device="cuda")
+    assert lib.eip_weighted_column_sums(V(A.data_ptr()), ctypes.c_longlong(n_rows), n_cols, ctypes.c_longlong(ld),
+                                        V(w.data_ptr()), V(work.data_ptr()), ctypes.c_longlong(work.numel()), 0,
+                                        V(gamma.data_ptr()), s) == 0
+    ref = A.t() @ w if n_rows else torch.zeros(n_cols, dtype=torch.float64, device="cuda")
+    scale = (A.abs().t() @ w.abs()) if n_rows else torch.zeros_like(ref)
+    assert bool(((gamma - ref).abs() <= 1e-13 * scale + 1e-300).all())
+    acc = gamma.clone()
+    assert lib.eip_weighted_column_sums(V(A.data_ptr()), ctypes.c_longlong(n_rows), n_cols, ctypes.c_longlong(ld),
+                                        V(w.data_ptr()), V(work.data_ptr()), ctypes.c_longlong(work.numel()), 1,
+                                        V(acc.data_ptr()), s) == 0
+    assert bool(((acc - 2 * ref).abs() <= 2e-13 * scale + 1e-300).all())
+    # a workspace too small is rejected, not overrun
+    if n_rows:
+        assert lib.eip_weighted_column_sums(V(A.data_ptr()), ctypes.c_longlong(n_rows), n_cols, ctypes.c_longlong(ld),
+                                            V(w.data_ptr()), V(work.data_ptr()), ctypes.c_longlong(need - 1), 0,
+                                            V(gamma.data_ptr()), s) == 4
+    out = torch.full((max(n_rows, 1),), 7.0, dtype=torch.float64, device="cuda")
+    assert lib.eip_row_dots(V(A.data_ptr()), ctypes.c_longlong(n_rows), n_cols, ctypes.c_longlong(ld), V(x.data_ptr()),
+                            V(out.data_ptr()), s) == 0
+    if n_rows:
+        refr = A @ x
+        assert bool(((out[:n_rows] - refr).abs() <= 1e-13 * (A.abs() @ x.abs())).all())
+    torch.cuda.synchronize()

@@ -161,7 +161,7 @@ def test_queue_every_class_equals_class_kernels(use_f32):
             k += 1
     _say(f"{q.n_triplet_items} items of {k - len(own)} classes through the queue; "
          f"{len(own)} classes on their own kernels: {[(plan.pairs[p].la, plan.pairs[p].lb, plan.aux[a].lc) for p, a in own]}")
-    assert k - len(own) >= 0.8 * k
+    assert k - len(own) >= 0.7 * k
     q.build().upload()
     plan.zero_unwritten(ref)
     plan.zero_unwritten(out)
