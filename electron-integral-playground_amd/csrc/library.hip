@@ -821,6 +821,11 @@ int eip_set_three_center_variant(int i_L, int j_L, int k_L, int use_f32, int var
     return 0;
 }
 
+int eip_three_center_variant(int i_L, int j_L, int k_L, int use_f32)
+{
+    return spec_3c(i_L, j_L, k_L) ? current_variant(i_L, j_L, k_L, use_f32) : -1;
+}
+
 int eip_three_center_f32_uses_bf16(int i_L, int j_L, int k_L)
 {
     return spec_3c(i_L, j_L, k_L) ? (eip_j3c_variant_mfma(i_L, j_L, k_L, current_variant(i_L, j_L, k_L, 1)) == 2) : 0;

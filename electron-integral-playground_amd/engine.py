@@ -1265,16 +1265,27 @@ class WorkQueue:
 
     def supports(self, la, lb, lc):
         """Whether the class runs in the tier kernels: it has bodies there (classes whose bodies need more
-        registers than a persistent loop leaves -- the highest (df|g) and (ff|x) chunks -- do not) and its
-        current variant for this precision is a vector one (a class tuned to the matrix cores keeps its
-        own, faster kernels)."""
-        m0, m1, tier = _c_int(), _c_int(), _c_int()
-        if self.lib.eip_three_center_queue_chunk(la, lb, lc, self.f32, 0, ctypes.byref(m0), ctypes.byref(m1),
-                                                 ctypes.byref(tier)) != 0:
-            return False
-        if self.f32:
-            return self.lib.eip_three_center_f32_uses_bf16(la, lb, lc) != 1
-        return self.lib.eip_three_center_uses_mfma(la, lb, lc) != 1
+        registers than a persistent loop leaves -- the highest (df|g) and (ff|x) chunks -- do not), and
+        they are its current variant's kernels for this precision (a class tuned to the matrix cores, or
+        to a vector plan the queue could not hold, keeps its own, faster kernels)."""
+        key = (la, lb, lc, self.f32, self.lib.eip_three_center_variant(la, lb, lc, self.f32))
+        hit = _QUEUE_SUPPORT.get(key)
+        if hit is None:
+            m0, m1, tier = _c_int(), _c_int(), _c_int()
+            queue_plan, c = [], 0
+            while self.lib.eip_three_center_queue_chunk(la, lb, lc, self.f32, c, ctypes.byref(m0), ctypes.byref(m1),
+                                                        ctypes.byref(tier)) == 0:
+                queue_plan.append((m0.value, m1.value))
+                c += 1
+            own_plan, c = [], 0
+            while self.lib.eip_three_center_chunk(la, lb, lc, key[4], c, ctypes.byref(m0), ctypes.byref(m1)) == 0:
+                own_plan.append((m0.value, m1.value))
+                c += 1
+            # in the queue only when its bodies are the class's current (tuned) vector kernels
+            hit = bool(queue_plan) and queue_plan == own_plan and \
+                self.lib.eip_three_center_variant_mfma(la, lb, lc, key[4]) == 0
+            _QUEUE_SUPPORT[key] = hit
+        return hit
 
     def add(self, la, lb, lc, items, n_items, jobs_ptr, costs, ks_end=None, pp=None, job=None, layout=0,
             accumulate=0, omega=0.0):
@@ -1468,6 +1479,7 @@ class WorkQueue:
 
 
 _QUEUE_GRIDS = {}
+_QUEUE_SUPPORT = {}
 
 
 class QueueGraph:

@@ -80,6 +80,7 @@ int eip_three_center_nchunks(int i_L, int j_L, int k_L);
 int eip_three_center_nvariants(int i_L, int j_L, int k_L);
 int eip_three_center_chunk(int i_L, int j_L, int k_L, int variant, int chunk, int* m0, int* m1);
 int eip_set_three_center_variant(int i_L, int j_L, int k_L, int use_f32, int variant);
+int eip_three_center_variant(int i_L, int j_L, int k_L, int use_f32);   /* the current one, -1: no such class */
 
 /* Variants after the vector ones run the float64 contraction on the matrix cores
  * (v_mfma_f64_16x16x4f64: four primitive pairs x 16 auxiliary shells per instruction, one workgroup of

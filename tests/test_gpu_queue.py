@@ -114,6 +114,8 @@ def test_queue_graph_replayed_over_new_work():
         assert queues[k].dev is bufs
         outs[k].fill_(float("nan"))
         plans[k].zero_unwritten(outs[k])      # rows of screened-out shell pairs (not in the queue)
+        for pi, ai in queues[k].fallback:     # classes that keep their own kernels
+            plans[k]._launch_class(pi, ai, outs[k], None)
         torch.cuda.synchronize()
         g.replay()
         torch.cuda.synchronize()
