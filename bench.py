@@ -768,7 +768,7 @@ def mixed_leg(threshold=1e-6, block_bytes=40e9, n_atom=None, tune=None, aux_orde
     return res
 
 
-def serving_leg(n_jobs, rate, budget=200e9, mixed=True, part=0, latency_file=None, persistent=True):
+def serving_leg(n_jobs, rate, budget=200e9, mixed=True, part=0, latency_file=None, persistent=False):
     """Poisson stream of integral jobs (atom counts lognormal, median 40, clipped 10-250; 4s3p2d1f
     orbital and s-g auxiliary generators) through the continuous-batching server (each step one work
     queue per precision through the persistent tier kernels, or -- persistent=False -- one launch per
@@ -979,8 +979,9 @@ def main():
                     help="jobs of the short serving run included in the default line")
     ap.add_argument("--serving-rate", type=float, default=1.8, help="Poisson arrival rate, jobs/s")
     ap.add_argument("--serving-plain", action="store_true", help="serve in plain float64 instead of mixed")
-    ap.add_argument("--serving-class-launches", action="store_true",
-                    help="serving steps as one launch per class and precision instead of the persistent queue")
+    ap.add_argument("--serving-persistent", action="store_true",
+                    help="serving steps as one work queue per precision through the persistent tier kernels "
+                         "instead of one launch per class and precision")
     ap.add_argument("--serving-part", type=int, default=0, help="part index of a stream run in several calls")
     ap.add_argument("--serving-latencies", metavar="JSON", help="write every job's latency to this file")
     ap.add_argument("--single-stream", action="store_true",
@@ -1025,7 +1026,7 @@ def main():
     if args.serving:
         print(json.dumps({"serving": serving_leg(args.serving_jobs, args.serving_rate, mixed=not args.serving_plain,
                                                  part=args.serving_part, latency_file=args.serving_latencies,
-                                                 persistent=not args.serving_class_launches)}),
+                                                 persistent=args.serving_persistent)}),
               flush=True)
         return
     if args.batched_only:

@@ -20,15 +20,15 @@ def _drain(server, jobs):
     return server
 
 
-@pytest.mark.parametrize("mixed", [False, True])
-def test_alone_equals_mix(mixed):
+@pytest.mark.parametrize("mixed,persistent", [(False, False), (True, False), (False, True), (True, True)])
+def test_alone_equals_mix(mixed, persistent):
     import torch
     from electron_integral_playground_amd.serving import IntegralServer
-    print(f"case mixed={mixed}", flush=True)
+    print(f"case mixed={mixed} persistent={persistent}", flush=True)
     a = _job(10, 31)
-    alone = _drain(IntegralServer(budget_bytes=8e9, mixed=mixed, keep_outputs=True), [(0, a)])
+    alone = _drain(IntegralServer(budget_bytes=8e9, mixed=mixed, keep_outputs=True, persistent=persistent), [(0, a)])
     others = [(1, _job(12, 32)), (2, _job(10, 33)), (3, _job(11, 34))]
-    mix = IntegralServer(budget_bytes=8e9, mixed=mixed, keep_outputs=True)
+    mix = IntegralServer(budget_bytes=8e9, mixed=mixed, keep_outputs=True, persistent=persistent)
     mix.submit(1, others[0][1])
     mix.poll()                                   # job 1 may already be on the GPU when job 0 arrives
     mix.submit(0, a)
