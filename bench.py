@@ -400,8 +400,10 @@ def headline_leg(args):
         f"blocks={len(blocks)} host pair list {t_pairs:.1f}s; split {stats}")
 
     def step(mixed):
+        # each block's launches (auxiliary classes, and float64/float32 tiers) spread over 6 streams; the
+        # next block (same buffer) waits for all of them
         for b in blocks:
-            plan.run_block(*b, A, mixed=mixed)
+            plan.run_block(*b, A, mixed=mixed, concurrent=not args.single_stream)
 
     for _ in range(max(1, args.warmup)):
         step(False)
@@ -985,7 +987,8 @@ def main():
     ap.add_argument("--serving-part", type=int, default=0, help="part index of a stream run in several calls")
     ap.add_argument("--serving-latencies", metavar="JSON", help="write every job's latency to this file")
     ap.add_argument("--single-stream", action="store_true",
-                    help="plain80: every launch on one stream (profiling runs: kernel durations do not overlap)")
+                    help="every launch of a step on one stream (80-atom block and headline steps; profiling runs: "
+                         "kernel durations do not overlap)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run the library's plain copy kernel %d times over %d MiB (calibrates rocprofv3 "
                          "FETCH_SIZE/WRITE_SIZE)" % (COPY_PROBE_RUNS, COPY_PROBE_BYTES >> 20))

@@ -965,24 +965,33 @@ class ThreeCenterPlan:
         pc = self.pairs[pi]
         return (sp1 - sp0) * (2 * pc.la + 1) * (2 * pc.lb + 1)
 
-    def run_block(self, pi, sp0, sp1, out, mixed=False, stream=None, aux_classes=None):
+    def run_block(self, pi, sp0, sp1, out, mixed=False, stream=None, aux_classes=None, concurrent=False):
         """(ij|P) of shell pairs [sp0, sp1) of class pi, all auxiliary shells, into the compact block
         out[(sp - sp0) * nij + ij, P] (every row written; no zeroing needed).  ``aux_classes`` (indices
-        into self.aux) restricts the launches to those auxiliary classes' columns."""
+        into self.aux) restricts the launches to those auxiliary classes' columns.  ``concurrent`` spreads
+        the launches over N_SIDE_STREAMS + 1 streams by estimated cost (each auxiliary class writes its own
+        columns, and an item-granularity mixed split its own items in each tier); ``stream`` waits for all
+        of them before the call returns."""
         pc = self.pairs[pi]
         nij = (2 * pc.la + 1) * (2 * pc.lb + 1)
         base = out.data_ptr() - 8 * sp0 * nij * self.n_aux
         entries = [self.job_words(pi, ai, base, self.n_aux, self.mixed[(pi, ai)]["perm"] if mixed else None)
                    for ai in range(len(self.aux))]
         jobs = self.job_table(entries)
+        groups = []          # (cost, [launch(stream) in order]): groups are independent of each other
         for ai, ac in enumerate(self.aux):
             if aux_classes is not None and ai not in aux_classes:
                 continue
+            fl = three_center_flops_per_triplet(pc.la, pc.lb, ac.lc) * max(1, ac.n_prim)
             if not mixed:
-                self.launch_work(pi, ai, self._block_items(pi, ai, sp0, sp1)[0], jobs[ai], layout=1, stream=stream)
+                it = self._block_items(pi, ai, sp0, sp1)[0]
+                groups.append((fl * it.shape[0], [lambda s, ai=ai, it=it: self.launch_work(
+                    pi, ai, it, jobs[ai], layout=1, stream=s)]))
                 continue
             if self.checked:
-                self.launch_checked(pi, ai, self._block_items(pi, ai, sp0, sp1)[0], jobs[ai], layout=1, stream=stream)
+                it = self._block_items(pi, ai, sp0, sp1)[0]
+                groups.append((fl * it.shape[0], [lambda s, ai=ai, it=it: self.launch_checked(
+                    pi, ai, it, jobs[ai], layout=1, stream=s)]))
                 continue
             if self.mixed_aux_order == "natural":
                 it64, pp64 = self._block_items(pi, ai, sp0, sp1, "f64")
@@ -992,9 +1001,28 @@ class ThreeCenterPlan:
                 # Q_P-ordered splits stay in shell-pair order: longest-first dispatch measured slower
                 # there (output columns are permuted as well)
                 it64, pp64, it32, pp32, acc32 = self.mixed_items(pi, ai, sp0, sp1)
-            self.launch_work(pi, ai, it64, jobs[ai], pp=pp64, layout=1, accumulate=0, stream=stream)
-            self.launch_work(pi, ai, it32, jobs[ai], pp=pp32, layout=1, accumulate=acc32, use_f32=True,
-                             stream=stream)
+            f64 = lambda s, ai=ai, it=it64, pp=pp64: self.launch_work(pi, ai, it, jobs[ai], pp=pp, layout=1,
+                                                                       accumulate=0, stream=s)
+            f32 = lambda s, ai=ai, it=it32, pp=pp32, acc=acc32: self.launch_work(
+                pi, ai, it, jobs[ai], pp=pp, layout=1, accumulate=acc, use_f32=True, stream=s)
+            n64 = it64.shape[0] if it64 is not None else 0
+            n32 = it32.shape[0] if it32 is not None else 0
+            if acc32:
+                groups.append((fl * (n64 + n32), [f64, f32]))       # the float32 part adds onto the float64
+            else:
+                groups.append((fl * n64, [f64]))
+                groups.append((0.6 * fl * n32, [f32]))
+        if not concurrent or len(groups) < 2:
+            for _, fns in groups:
+                for fn in fns:
+                    fn(stream)
+            return out
+        with _fork(stream, min(N_SIDE_STREAMS, len(groups) - 1)) as sides:
+            lanes = [stream] + sides
+            for k, idx in enumerate(_assign_lpt([c for c, _ in groups], len(lanes))):
+                for i in idx:
+                    for fn in groups[i][1]:
+                        fn(lanes[k])
         return out
 
     def _block_items(self, pi, ai, sp0, sp1, kind="plain"):
