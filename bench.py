@@ -394,16 +394,34 @@ def headline_leg(args):
     plan.prepare_blocks(blocks, mixed=True)
     rows = max(plan.block_rows(*b) for b in blocks)
     A = torch.empty((rows, mol_s.n_aux), dtype=torch.float64, device="cuda")
+    A2 = torch.empty_like(A) if not args.single_stream else None
     stream = torch.cuda.current_stream()
+    lane_sets = [[torch.cuda.Stream() for _ in range(6)] for _ in range(2)]
     n_pp = sum(len(a) for a in arrays.values())
     log(f"headline: n_ao={mol.n_ao} n_aux(slice)={mol_s.n_aux} of {mol.n_aux} primitive pairs={n_pp} "
         f"blocks={len(blocks)} host pair list {t_pairs:.1f}s; split {stats}")
 
     def step(mixed):
-        # each block's launches (auxiliary classes, and float64/float32 tiers) spread over 6 streams; the
-        # next block (same buffer) waits for all of them
-        for b in blocks:
-            plan.run_block(*b, A, mixed=mixed, concurrent=not args.single_stream)
+        """One pass over every block.  Blocks alternate between two 40 GB buffers; each block's launches
+        (auxiliary classes, and float64/float32 tiers) are spread over its buffer's 6 streams, and a block
+        waits only for the previous block of the same buffer (two blocks back), so one block's tail
+        overlaps the next block.  --single-stream: one buffer, every launch in order on one stream."""
+        if args.single_stream:
+            for b in blocks:
+                plan.run_block(*b, A, mixed=mixed)
+            return
+        last = [[], []]
+        for s_ in lane_sets[0] + lane_sets[1]:
+            s_.wait_stream(stream)
+        for k, b in enumerate(blocks):
+            lanes = lane_sets[k % 2]
+            for s_ in lanes:
+                for e in last[k % 2]:
+                    s_.wait_event(e)
+            plan.run_block(*b, (A, A2)[k % 2], mixed=mixed, lanes=lanes)
+            last[k % 2] = [s_.record_event() for s_ in lanes]
+        for s_ in lane_sets[0] + lane_sets[1]:
+            stream.wait_stream(s_)
 
     for _ in range(max(1, args.warmup)):
         step(False)
@@ -475,7 +493,7 @@ def headline_leg(args):
         stats["f32_triplets"] = sum(a - b for a, b in tally.values())
         log("checked split: float64 share per class " +
             ", ".join(f"{''.join(map(str, k))} {b / max(a, 1):.3f}" for k, (a, b) in sorted(tally.items())))
-    del A, B
+    del A, A2, B
     torch.cuda.empty_cache()
     gpu_cols = {name: {} for name in got}
     c0 = 0

@@ -965,13 +965,16 @@ class ThreeCenterPlan:
         pc = self.pairs[pi]
         return (sp1 - sp0) * (2 * pc.la + 1) * (2 * pc.lb + 1)
 
-    def run_block(self, pi, sp0, sp1, out, mixed=False, stream=None, aux_classes=None, concurrent=False):
+    def run_block(self, pi, sp0, sp1, out, mixed=False, stream=None, aux_classes=None, concurrent=False,
+                  lanes=None):
         """(ij|P) of shell pairs [sp0, sp1) of class pi, all auxiliary shells, into the compact block
         out[(sp - sp0) * nij + ij, P] (every row written; no zeroing needed).  ``aux_classes`` (indices
         into self.aux) restricts the launches to those auxiliary classes' columns.  ``concurrent`` spreads
         the launches over N_SIDE_STREAMS + 1 streams by estimated cost (each auxiliary class writes its own
         columns, and an item-granularity mixed split its own items in each tier); ``stream`` waits for all
-        of them before the call returns."""
+        of them before the call returns.  ``lanes`` (a list of streams) spreads them over those streams
+        instead, with no fork or join: the caller orders the lanes against other work (e.g. double-
+        buffered blocks, where block k + 2 waits only for block k)."""
         pc = self.pairs[pi]
         nij = (2 * pc.la + 1) * (2 * pc.lb + 1)
         base = out.data_ptr() - 8 * sp0 * nij * self.n_aux
@@ -1012,6 +1015,16 @@ class ThreeCenterPlan:
             else:
                 groups.append((fl * n64, [f64]))
                 groups.append((0.6 * fl * n32, [f32]))
+        if lanes is not None:
+            # the job table (and any item table) may have just been written on the current stream
+            cur = self.torch.cuda.current_stream(self.device)
+            for ln in lanes:
+                ln.wait_stream(cur)
+            for k, idx in enumerate(_assign_lpt([c for c, _ in groups], len(lanes))):
+                for i in idx:
+                    for fn in groups[i][1]:
+                        fn(lanes[k])
+            return out
         if not concurrent or len(groups) < 2:
             for _, fns in groups:
                 for fn in fns:
