@@ -840,6 +840,28 @@ def serving_leg(n_jobs, rate, budget=200e9, mixed=True, part=0, latency_file=Non
 BATCH_BUILDERS = 4      # host threads building wave plans concurrently
 
 
+def tune_queue_leg(path, budget=200e9, n_mol=256):
+    """Autotune the persistent queue (grid occupancy cap x entry grouping) on the first wave of the
+    batched workload and write the table to ``path`` (engine.autotune_queue)."""
+    import torch
+    from electron_integral_playground_amd import generators, pair_utility
+    from electron_integral_playground_amd.engine import BatchPlan, autotune_queue
+    sizes = generators.batched_sizes(n_mol=n_mol)
+    wave, acc = [], 0
+    for i, n in enumerate(sizes):
+        mol = generators.batched_molecule(i, int(n))
+        b = 8 * (mol.n_ao * (mol.n_ao + 1) // 2 * mol.n_aux + mol.n_aux ** 2)
+        if wave and acc + b > budget:
+            break
+        wave.append((mol, pair_utility.form_pair_data(mol, 1e-14), pair_utility.form_auxiliary_pair_data(mol)))
+        acc += b
+    bp = BatchPlan(wave)
+    out = bp.new_output()
+    times = autotune_queue(bp, out, save=path, log=log)
+    return {"molecules": len(wave), "ms": {f"waves {w}, entry cost {c:.0e}": t for (w, c), t in times.items()}}
+
+
+
 def batched_leg(budget=200e9, n_mol=256, repeats=2):
     """Batched workload: 256 molecules, atom counts lognormal (median 20, sigma 0.5, clipped 4-120), plain
     generators (orbital l <= 2, auxiliary l <= 3), box scaled with the atom count.  Molecules are grouped
@@ -992,6 +1014,8 @@ def main():
     ap.add_argument("--tuning", metavar="JSON", help="apply this variant table on top of the shipped one")
     ap.add_argument("--serving", action="store_true", help="run only the serving workload")
     ap.add_argument("--batched-only", action="store_true", help="run only the batched workload")
+    ap.add_argument("--tune-queue", metavar="JSON",
+                    help="autotune the persistent queue on the first batched wave, write the table to JSON, exit")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched workload")
     ap.add_argument("--serving-jobs", type=int, default=2000)
     ap.add_argument("--no-serving", action="store_true", help="skip the short serving run of the default line")
@@ -1052,6 +1076,9 @@ def main():
         return
     if args.batched_only:
         print(json.dumps({"batched": batched_leg()}), flush=True)
+        return
+    if args.tune_queue:
+        print(json.dumps({"tune_queue": tune_queue_leg(args.tune_queue)}), flush=True)
         return
     if args.profile_plain:
         print(json.dumps(plain80_leg(args) if args.plain80_only else profile_plain_leg(args)), flush=True)

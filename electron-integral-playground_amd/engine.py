@@ -1272,6 +1272,10 @@ QUEUE_RUN = 16           # consecutive entries dealt to one part (items of one s
 # QUEUE_MAX_GROUP items: one atomic fetch per entry, not per item.
 QUEUE_ENTRY_COST = 3.0e4
 QUEUE_MAX_GROUP = 64
+# Tuned queue parameters per precision ({"f64": {"waves_per_simd": w, "entry_cost": c}}), loaded from
+# tuning/queue.json when it exists (autotune_queue writes it).
+QUEUE_TUNING_FILE = __import__("pathlib").Path(__file__).resolve().parent / "tuning" / "queue.json"
+QUEUE_TUNING = __import__("json").loads(QUEUE_TUNING_FILE.read_text()) if QUEUE_TUNING_FILE.exists() else {}
 # Control block words (one 128-byte line per part counter; see eip_three_center_queue_run)
 _CTL_FIRST = QUEUE_PARTS * 32
 _CTL_COUNT = _CTL_FIRST + QUEUE_PARTS
@@ -1288,13 +1292,18 @@ class WorkQueue:
     large enough, so a captured graph can be replayed over new work); ``run`` launches one kernel per tier,
     the tiers concurrently on side streams."""
 
-    def __init__(self, lib=None, device="cuda", use_f32=False, waves_per_simd=0):
+    def __init__(self, lib=None, device="cuda", use_f32=False, waves_per_simd=None, entry_cost=None):
+        """waves_per_simd: cap of the tier kernels' grid occupancy (0: as many waves as fit); entry_cost:
+        work per queue entry cheap items are grouped up to.  None: the tuned values (QUEUE_TUNING, written by
+        autotune_queue) or the defaults."""
         import torch
         self.torch = torch
         self.lib = lib or library()
         self.device = torch.device(device)
         self.f32 = int(bool(use_f32))
-        self.waves_per_simd = int(waves_per_simd)
+        tuned = QUEUE_TUNING.get("f32" if use_f32 else "f64", {})
+        self.waves_per_simd = int(tuned.get("waves_per_simd", 0) if waves_per_simd is None else waves_per_simd)
+        self.entry_cost = float(tuned.get("entry_cost", QUEUE_ENTRY_COST) if entry_cost is None else entry_cost)
         self.slot_bytes = int(self.lib.eip_three_center_queue_slot_bytes())
         self.ctl_words = int(self.lib.eip_three_center_queue_ctl_words())
         self.ntiers = int(self.lib.eip_three_center_queue_tiers(self.f32))
@@ -1358,7 +1367,7 @@ class WorkQueue:
     def build(self):
         """Host arrays: slots (bytes), entries [n][4] int32 (slot, first item, item count, 0) and the
         control block of every tier.  A slot's items are grouped in their stored order into runs of about
-        QUEUE_ENTRY_COST work; runs are ordered longest-first within a slot, slots by decreasing mean item
+        entry_cost work; runs are ordered longest-first within a slot, slots by decreasing mean item
         cost, and the ordered runs are dealt QUEUE_RUN at a time over the 8 parts."""
         entries, ctls, tot_cost = [], [], []
         for t in range(self.ntiers):
@@ -1366,7 +1375,7 @@ class WorkQueue:
             rows = []
             for s, cst, _ in lst:
                 n = len(cst)
-                g = int(np.clip(np.ceil(QUEUE_ENTRY_COST / max(float(cst.mean()), 1e-30)), 1, QUEUE_MAX_GROUP))
+                g = int(np.clip(np.ceil(self.entry_cost / max(float(cst.mean()), 1e-30)), 1, QUEUE_MAX_GROUP))
                 first = np.arange(0, n, g, dtype=np.int64)
                 count = np.minimum(g, n - first)
                 run_cost = np.add.reduceat(cst, first)
@@ -1411,7 +1420,7 @@ class WorkQueue:
                 lst = sorted(self._tier[t], key=lambda e: -float(np.mean(e[1])))
                 slot_ids, n_it, grp, n_run = [], [], [], []
                 for sl, cst, n in lst:
-                    g = int(np.clip(np.ceil(QUEUE_ENTRY_COST / max(float(np.mean(cst)), 1e-30)), 1, QUEUE_MAX_GROUP))
+                    g = int(np.clip(np.ceil(self.entry_cost / max(float(np.mean(cst)), 1e-30)), 1, QUEUE_MAX_GROUP))
                     slot_ids.append(sl); n_it.append(n); grp.append(g); n_run.append((n + g - 1) // g)
                 N = int(sum(n_run))
                 if N:
@@ -1578,7 +1587,8 @@ class QueueGraph:
             for t in range(q.ntiers):
                 q.grid(t)
             self.order = list(getattr(q, "tier_order", range(q.ntiers)))
-            empty = WorkQueue(q.lib, q.device, use_f32=bool(q.f32), waves_per_simd=q.waves_per_simd).build()
+            empty = WorkQueue(q.lib, q.device, use_f32=bool(q.f32), waves_per_simd=q.waves_per_simd,
+                              entry_cost=q.entry_cost).build()
             empty.tier_order = self.order
             empty.upload(into=self.bufs, stream=s)
             empty.run(s)                                   # warm-up outside capture: no work
@@ -1781,10 +1791,10 @@ class BatchPlan:
                                   device=self.device)
         return table, {k: i for i, k in enumerate(keys)}
 
-    def work_queue(self, out, waves_per_simd=0):
+    def work_queue(self, out, waves_per_simd=None, entry_cost=None):
         """The batch's (ij|P) as one WorkQueue (persistent tier kernels) writing into ``out``."""
         table, row = self.job_table(out)
-        q = WorkQueue(self.lib, self.device, waves_per_simd=waves_per_simd)
+        q = WorkQueue(self.lib, self.device, waves_per_simd=waves_per_simd, entry_cost=entry_cost)
         q.job_table = table                      # kept alive with the queue
         q.fallback = []                          # classes without queue bodies: their own kernels
         for (pi, ai), (it, end) in self.items.items():
@@ -1963,3 +1973,36 @@ def autotune(plan, repeats=3, use_f32=False, save=None, log=None, block_bytes=No
         old["variants"].update(chosen)
         path.write_text(_json.dumps(old, indent=1, sort_keys=True))
     return chosen
+
+
+def autotune_queue(batch, out, waves=(0, 2, 1), entry_costs=(1.0e4, 3.0e4, 1.0e5), repeats=3, save=None, log=None):
+    """Pick the persistent queue's grid occupancy cap (waves per SIMD; 0 = as many as fit) and entry
+    grouping (work per entry) for a BatchPlan ``batch`` writing into ``out``: every combination is built,
+    uploaded and timed (HIP events, fastest of ``repeats`` after a warm-up); the fastest is kept in
+    QUEUE_TUNING (and written to ``save``).  Returns {(waves, entry_cost): ms}."""
+    import torch
+    stream = torch.cuda.current_stream()
+    times = {}
+    for w in waves:
+        for c in entry_costs:
+            q = batch.work_queue(out, waves_per_simd=w, entry_cost=c)
+            q.upload()
+            batch.run(out, queue=q, zero=False)
+            best = None
+            for _ in range(repeats):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                batch.run(out, queue=q, zero=False)
+                e1.record(stream)
+                e1.synchronize()
+                best = e0.elapsed_time(e1) if best is None else min(best, e0.elapsed_time(e1))
+            times[(w, c)] = best
+            if log:
+                log(f"  autotune queue waves_per_simd {w} entry_cost {c:.0e}: {best:.2f} ms")
+    (w, c), _ = min(times.items(), key=lambda kv: kv[1])
+    QUEUE_TUNING["f64"] = {"waves_per_simd": int(w), "entry_cost": float(c)}
+    if save:
+        path = _pathlib.Path(save)
+        path.parent.mkdir(parents=True, exist_ok=True)
+        path.write_text(_json.dumps(QUEUE_TUNING, indent=1, sort_keys=True))
+    return times
