@@ -1,10 +1,20 @@
-"""Pick, per three-centre class and precision, how the kernels stage pair-matrix entries ("lds" or
-"dpp"), from autotuner logs of two builds that differ only in the staging mode.
+"""Pick, per three-centre class and precision, how the kernels stage pair-matrix entries, from autotuner
+logs of builds that differ only in the staging mode:
 
-Usage: python -m codegen.choose_stage LDS_LOG[,LDS_LOG...] DPP_LOG[,DPP_LOG...] > codegen/stage_choice.json
-(run from the package directory).  Each log holds lines "autotune <la><lb><lc>_<prec> variant <v>: <t> ms"
+* "lds": the workgroup's pair-matrix rows in LDS, each entry read by a broadcast LDS load where used;
+* "dpp": the rows in registers, 64 entries per lane-parallel load, each entry broadcast by a DPP move
+  (only for rows of at most DPP_MAX_ENTRIES entries);
+* "ldsdpp": the rows in LDS, read 16 entries per lane-parallel LDS load into rotating registers two
+  groups ahead of use, each entry broadcast by a DPP move (any row length).
+
+Usage (from the package directory):
+  python -m codegen.choose_stage LDS_LOG[,...] DPP_LOG[,...] > codegen/stage_choice.json
+  python -m codegen.choose_stage --ldsdpp BASE_LOG[,...] LDSDPP_LOG[,...] < old.json > codegen/stage_choice.json
+The first form picks "lds" or "dpp"; the second moves the classes of an existing table that are on "lds"
+to "ldsdpp", from the logs of a build of that table and of a build whose "lds" classes use "ldsdpp".  Each log holds lines "autotune <la><lb><lc>_<prec> variant <v>: <t> ms"
 as printed by engine.autotune; a class's time is its best variant, summed over the logs (workloads) that
-time it.  "dpp" is chosen when it is at least 2 % faster; otherwise the class keeps "lds".
+time it.  A mode other
+than "lds" is chosen when it is at least 2 % faster than "lds"; otherwise the class keeps "lds".
 """
 
 import json
@@ -41,6 +51,15 @@ def choose(lds_logs, dpp_logs, margin=0.98):
     return {k: pick(k) for k in sorted(lds)}
 
 
+def upgrade_lds(table, base_logs, ldsdpp_logs, margin=0.98):
+    base, alt = best_times(base_logs), best_times(ldsdpp_logs)
+    return {k: ("ldsdpp" if m == "lds" and k in base and k in alt and alt[k] < margin * base[k] else m)
+            for k, m in table.items()}
+
+
 if __name__ == "__main__":
-    table = choose(sys.argv[1].split(","), sys.argv[2].split(","))
+    if sys.argv[1] == "--ldsdpp":
+        table = upgrade_lds(json.load(sys.stdin), sys.argv[2].split(","), sys.argv[3].split(","))
+    else:
+        table = choose(sys.argv[1].split(","), sys.argv[2].split(","))
     print(json.dumps(table, indent=1, sort_keys=True))

@@ -265,16 +265,19 @@ def boys_in_lds(la, lb, lc, precision, boys_lds):
 
 def emit_j3c(la, lb, lc, m0, m1, precision="f64", stage=None, boys_lds=False):
     """Spherical three-centre kernel for aux components [m0, m1) of class (la lb | lc); ``stage`` is
-    "lds", "dpp" or "sgpr" (default STAGE), the way pair-matrix entries reach the lanes; ``boys_lds``:
+    "lds", "dpp", "ldsdpp" or "sgpr" (default STAGE), the way pair-matrix entries reach the lanes; ``boys_lds``:
     the float64 Boys function reads the class's coarse table slice staged in LDS (boys64_lds) instead of
     the fine grid in global memory."""
     spherical = True
     stage = stage or STAGE
-    if stage not in ("lds", "dpp", "sgpr"):
+    if stage not in ("lds", "dpp", "sgpr", "ldsdpp"):
         raise ValueError(f"unknown staging mode {stage!r}")
-    # sgpr: no staging -- the wave-uniform entries are scalar loads used as SGPR operands of the FMAs
+    # sgpr: no staging -- the wave-uniform entries are scalar loads used as SGPR operands of the FMAs;
+    # ldsdpp: rows staged in LDS as for "lds", but read 16 entries per load (each lane of a row of 16 its
+    # own entry) two groups ahead of use, and broadcast from those registers by DPP
     use_dpp = stage == "dpp" and pair_nnz(la, lb, spherical) <= DPP_MAX_ENTRIES
     use_lds = not use_dpp and STAGE_LDS and stage != "sgpr"
+    lds_dpp = use_lds and stage == "ldsdpp"
     nij_ = n_out(la, spherical) * n_out(lb, spherical)
     prefetch = (use_dpp and precision != "f32"
                 and nij_ * (m1 - m0) + _needed_R(la, lb, lc, m0, m1) <= PREFETCH_MAX_DOUBLES)
@@ -503,6 +506,12 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64", stage=None, boys_lds=False):
     _emit_R(out, L, needed, ty)
     # contraction, t-major
     k = 0
+    if lds_dpp:
+        # group g of 16 entries in register Eg_{g % 3}; groups 0 and 1 loaded here, group g + 2 when g starts
+        out.append("        const int l16s = tid & 15;")
+        out.append(f"        {ty} Eg_0 = E[min(l16s, {nnz - 1})];")
+        out.append(f"        {ty} Eg_1 = E[min(16 + l16s, {nnz - 1})];" if nnz > 16 else f"        {ty} Eg_1 = 0;")
+        out.append(f"        {ty} Eg_2 = 0;")
     for t in ts:
         vnames = {}
         for m in ms:
@@ -520,6 +529,11 @@ def emit_j3c(la, lb, lc, m0, m1, precision="f64", stage=None, boys_lds=False):
             row = layout[k][1]
             if use_dpp:
                 out.append(f"        {{ const {ty} e = eip::bcast16<{k % 16}>(Er_{k // 16});")
+            elif lds_dpp:
+                g = k // 16
+                if k % 16 == 0 and 16 * (g + 2) < nnz:
+                    out.append(f"        Eg_{(g + 2) % 3} = E[min({16 * (g + 2)} + l16s, {nnz - 1})];")
+                out.append(f"        {{ const {ty} e = eip::bcast16<{k % 16}>(Eg_{g % 3});")
             else:
                 out.append(f"        {{ const {ty} e = E[{k}];")
             for m in ms:

@@ -52,7 +52,7 @@ STAGE_CHOICE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "stage_c
 
 
 def stage_table(path=STAGE_CHOICE):
-    """{"<la><lb><lc>_<prec>": "lds" | "dpp"}: the measured staging mode per class (choose_stage.py)."""
+    """{"<la><lb><lc>_<prec>": "lds" | "dpp" | "ldsdpp"}: the measured staging mode per class (choose_stage.py)."""
     if not os.path.exists(path):
         return {}
     with open(path) as f:

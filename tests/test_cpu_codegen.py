@@ -36,6 +36,36 @@ def test_dpp_rows_cover_every_entry_once(la, lb, lc, m0, m1, prec):
     assert "__shared__" not in code and "__syncthreads" not in code
 
 
+@pytest.mark.parametrize("la,lb,lc,m0,m1,prec", [(0, 0, 0, 0, 1, "f64"), (0, 2, 1, 0, 3, "f64"),
+                                                 (2, 3, 4, 0, 2, "f64"), (3, 3, 3, 4, 6, "f32")])
+def test_ldsdpp_groups_are_loaded_before_use_and_kept_until_done(la, lb, lc, m0, m1, prec):
+    """Rows staged in LDS and read 16 entries per load: replaying the generated statements in order,
+    every broadcast of entry k reads register Eg_{(k//16) % 3} while it holds group k // 16 (loaded from
+    LDS offset 16 * (k // 16) + lane, clamped to the row), entries are used in order exactly once per
+    contraction, and no group is loaded twice."""
+    _, code = emit_j3c(la, lb, lc, m0, m1, prec, stage="ldsdpp")
+    nnz = pair_nnz(la, lb, True)
+    assert "__shared__" in code
+    held = {}
+    loads = []
+    uses = []
+    for line in code.splitlines():
+        m = re.search(r"\bEg_(\d) = E\[min\((?:(\d+) \+ )?l16s, (\d+)\)\]", line)
+        if m:
+            g = int(m.group(2) or 0) // 16
+            assert int(m.group(3)) == nnz - 1 and int(m.group(2) or 0) == 16 * g
+            held[int(m.group(1))] = g
+            loads.append(g)
+        m = re.search(r"eip::bcast16<(\d+)>\(Eg_(\d)\)", line)
+        if m:
+            j, r = int(m.group(1)), int(m.group(2))
+            k = len(uses)
+            assert (j, r) == (k % 16, (k // 16) % 3), f"entry {k}"
+            assert held.get(r) == k // 16, f"entry {k}: register {r} holds group {held.get(r)}"
+            uses.append(k)
+    assert len(uses) == nnz and sorted(loads) == loads == list(range((nnz + 15) // 16))
+
+
 def test_large_rows_stay_in_lds():
     la, lb = 3, 3
     assert pair_nnz(la, lb, True) > gen_kernels.DPP_MAX_ENTRIES
@@ -65,6 +95,12 @@ def test_choose_stage_needs_a_clear_win(tmp_path):
     table = choose_stage.choose([str(lds)], [str(dpp)])
     # 1.4 < 0.98 * 1.5: dpp; 0.99 is not 2 % faster than 1.0: lds; (ff) rows never leave LDS
     assert table == {"011_f64": "lds", "123_f64": "dpp", "333_f64": "lds"}
+    # the LDS-read + DPP-broadcast mode replaces "lds" only, and only on a 2 % win
+    alt = tmp_path / "ldsdpp.log"
+    alt.write_text("  autotune 011_f64 variant 0: 0.900 ms\n  autotune 123_f64 variant 0: 1.000 ms\n"
+                   "  autotune 333_f64 variant 0: 4.950 ms\n")
+    assert choose_stage.upgrade_lds(table, [str(lds)], [str(alt)]) == {"011_f64": "ldsdpp", "123_f64": "dpp",
+                                                                       "333_f64": "lds"}
 
 
 def _eval_generated(lines, env):

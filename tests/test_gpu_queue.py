@@ -184,14 +184,18 @@ def test_queue_every_class_equals_class_kernels(use_f32):
 
 
 def test_run_persistent_shared_graph():
-    """BatchPlan.run_persistent: one QueueGraph for several batches (captured once), with the zeroing and
-    (P|Q) beside it, equal to the per-class path for every batch."""
+    """BatchPlan.run_persistent: one QueueGraph for several batches (captured again only while its buffers
+    grow to the largest batch), with the zeroing and (P|Q) beside it, equal to the per-class path for
+    every batch."""
     import torch
     from electron_integral_playground_amd.engine import BatchPlan, QueueGraph
     plans = [BatchPlan(_batch(800 + 10 * k, 3 + k, (6, 10, 4, 12))) for k in range(3)]
     graph = QueueGraph()
     s = torch.cuda.Stream()
+    first_pass = None
     for rep in range(2):
+        if rep == 1:
+            first_pass = graph.captures
         for k, bp in enumerate(plans):
             ref, ref2 = bp.new_output(), bp.new_two_center_output()
             bp.run(ref)
@@ -205,5 +209,7 @@ def test_run_persistent_shared_graph():
             torch.cuda.synchronize()
             assert _close(out, ref), f"batch {k} pass {rep}"
             assert torch.equal(out2, ref2), f"batch {k} pass {rep} (P|Q)"
-    _say(f"3 batches x 2 passes through one graph, captured {graph.captures}x")
-    assert graph.captures <= 2
+    _say(f"3 batches x 2 passes through one graph, captured {first_pass}x in the first pass, "
+         f"{graph.captures - first_pass}x in the second")
+    # the buffers only grow: once every batch has been seen, every queue fits and the graph is replayed
+    assert 1 <= first_pass <= 3 and graph.captures == first_pass
