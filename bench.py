@@ -1089,13 +1089,16 @@ def main():
     if args.plain80_only:
         print(json.dumps({"plain80": plain80_leg(args)}), flush=True)
         return
+    # The drop-in loop first, in the state a caller of the C ABI finds the process in: timed after the
+    # headline (its 2 x 40 GB buffers in torch's cache, the CPU baseline's 16-thread OpenMP pool) the same
+    # calls measured 11 ms instead of 4.3-5.2 ms.
+    dropin = None if args.headline_only else dropin_leg()
     head = headline_leg(args)
-    plain80 = batched = serving = omega80 = dropin = None
+    plain80 = batched = serving = omega80 = None
     if not args.headline_only:
         if not args.no_plain80:
             plain80 = plain80_leg(args)
             omega80 = plain80_leg(argparse.Namespace(**{**vars(args), "steps": 2}), omega=0.2)
-        dropin = dropin_leg()
         if not args.no_batched:
             batched = batched_leg()
         if not args.no_serving:
