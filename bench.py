@@ -330,6 +330,42 @@ def trace_summary(stats_csv, run_json):
     return "\n".join(lines)
 
 
+SQ_PASS_A = ("SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES", "SQ_WAIT_ANY", "SQ_WAIT_INST_ANY", "SQ_ACTIVE_INST_VALU",
+             "SQ_INSTS_VALU", "SQ_INSTS_SMEM", "SQ_INSTS_LDS")
+SQ_PASS_B = ("SQ_LDS_BANK_CONFLICT", "SQ_WAIT_INST_LDS", "SQ_ACTIVE_INST_LDS", "SQ_INSTS_VALU_FMA_F64",
+             "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_ADD_F64", "SQ_ACTIVE_INST_SCA", "SQ_INSTS_SALU")
+
+
+def sq_summary(csv_a, csv_b, top=14):
+    """Markdown table: per float64 class (its chunk kernels' dispatches summed), from two rocprofv3 --pmc
+    passes of the same run (counters SQ_PASS_A, then SQ_PASS_B): the fraction of wave cycles spent waiting
+    on s_waitcnt (SQ_WAIT_ANY), stalled on dependencies (SQ_WAIT_INST_ANY) and issuing VALU
+    (SQ_ACTIVE_INST_VALU), the VALU and LDS instructions per wave cycle, and the share of VALU
+    instructions that are float64 FMAs.  Classes ordered by wave cycles, the top ``top`` shown."""
+    tot = {}
+    for path in (csv_a, csv_b):
+        for r in _csv_rows(path):
+            cls = _class_of(r.get("Kernel_Name") or "")
+            if cls is None:
+                continue
+            d = tot.setdefault(cls, {})
+            d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+            if r["Counter_Name"] == "SQ_WAVE_CYCLES":
+                d["dispatches"] = d.get("dispatches", 0) + 1
+    lines = ["| class | dispatches | wave cycles | wait (s_waitcnt) | dependency stall | VALU active | "
+             "VALU instr / wave cycle | FP64 FMA share of VALU | LDS instr / VALU instr | LDS bank conflict / LDS active |",
+             "|---|---|---|---|---|---|---|---|---|---|"]
+    for cls, d in sorted(tot.items(), key=lambda kv: -kv[1].get("SQ_WAVE_CYCLES", 0))[:top]:
+        wc = d.get("SQ_WAVE_CYCLES", 0) or 1
+        valu = d.get("SQ_INSTS_VALU", 0) or 1
+        lds_act = d.get("SQ_ACTIVE_INST_LDS", 0) or 1
+        lines.append(f"| j3c_{cls} | {d.get('dispatches', 0)} | {wc:.3e} | {d.get('SQ_WAIT_ANY', 0) / wc:.2f} | "
+                     f"{d.get('SQ_WAIT_INST_ANY', 0) / wc:.2f} | {d.get('SQ_ACTIVE_INST_VALU', 0) / wc:.2f} | "
+                     f"{d.get('SQ_INSTS_VALU', 0) / wc:.3f} | {d.get('SQ_INSTS_VALU_FMA_F64', 0) / valu:.2f} | "
+                     f"{d.get('SQ_INSTS_LDS', 0) / valu:.3f} | {d.get('SQ_LDS_BANK_CONFLICT', 0) / lds_act:.3f} |")
+    return "\n".join(lines)
+
+
 def pmc_summary(fetch_csv, write_csv, run_json):
     """Markdown table: per class, FETCH_SIZE and WRITE_SIZE per pass (KiB, summed over the class's
     dispatches), each scaled by the factor the copy probe's dispatches give (known bytes / reading), beside
@@ -1043,12 +1079,18 @@ def main():
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "RUN_JSON"),
                     help="write the per-class HBM traffic table from two rocprofv3 --pmc counter_collection.csv "
                          "files of a --profile-plain run whose printed line is in RUN_JSON, then exit")
+    ap.add_argument("--sq-summary", nargs=2, metavar=("PASS_A_CSV", "PASS_B_CSV"),
+                    help="write the per-class SQ counter table (stalls, VALU use, FP64 FMA share) from the two "
+                         "rocprofv3 --pmc counter_collection.csv files of a --profile-plain run, then exit")
     ap.add_argument("--trace-summary", nargs=2, metavar=("KERNEL_STATS_CSV", "RUN_JSON"),
                     help="write the per-class kernel-time table from a rocprofv3 --kernel-trace --stats "
                          "kernel_stats.csv of a --profile-plain run, then exit")
     args = ap.parse_args()
     if args.pmc_summary:
         print(pmc_summary(*args.pmc_summary))
+        return
+    if args.sq_summary:
+        print(sq_summary(*args.sq_summary))
         return
     if args.trace_summary:
         print(trace_summary(*args.trace_summary))
