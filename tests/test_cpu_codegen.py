@@ -198,3 +198,28 @@ def test_coarse_boys_slices_reproduce_mpmath():
                 ref = mpmath.mpf(1) / (2 * m + 1) if T == 0 else \
                     mpmath.gammainc(m + 0.5, 0, T) / (2 * mpmath.mpf(T) ** (m + 0.5))
                 assert abs(F[m] - float(ref)) <= 1e-15 * float(ref), (T, L, m)
+
+
+@pytest.mark.parametrize("la,lb,lc,m0,m1,prec", [(0, 0, 0, 0, 1, "f64"), (1, 2, 3, 0, 7, "f64"),
+                                                 (2, 3, 4, 0, 2, "f64"), (3, 3, 4, 6, 8, "f32")])
+def test_demand_ordered_hermite_integrals_define_before_use(la, lb, lc, m0, m1, prec):
+    """Emitted on demand, the kernel defines the same Hermite integrals with the same expressions as the
+    level-by-level order, each once, every one before its first use; and the first auxiliary-transform
+    term (V) of every Hermite index comes after all the integrals it reads."""
+    _, lev = emit_j3c(la, lb, lc, m0, m1, prec, r_order="level")
+    _, dem = emit_j3c(la, lb, lc, m0, m1, prec, r_order="demand")
+    ty = "float" if prec == "f32" else "double"
+    pat = re.compile(rf"\s*const {ty} (r\d+_\d+_\d+_\d+) = (.*);")
+    defs = lambda code: [m.groups() for m in map(pat.match, code.splitlines()) if m]
+    assert sorted(defs(lev)) == sorted(defs(dem)) and len(set(n for n, _ in defs(dem))) == len(defs(dem))
+    seen = set()
+    for line in dem.splitlines():
+        for ref in re.findall(r"\b(r\d+_\d+_\d+_\d+)\b", line.split("=", 1)[1] if "=" in line else ""):
+            assert ref in seen, f"{ref} read before it is defined: {line.strip()[:80]}"
+        m = pat.match(line)
+        if m:
+            seen.add(m.group(1))
+    # the demand order interleaves: some V is computed before the last Hermite integral
+    first_v = next(i for i, l in enumerate(dem.splitlines()) if re.match(rf"\s*const {ty} V_", l))
+    last_r = max(i for i, l in enumerate(dem.splitlines()) if pat.match(l))
+    assert first_v < last_r or len(defs(dem)) <= 1
