@@ -464,6 +464,21 @@ def headline_leg(args):
         step(True)
     torch.cuda.synchronize()
     log("headline warmup done")
+    if args.tune_step:
+        # variants chosen by the step time itself: float64 on the plain step, float32 on the mixed step,
+        # the classes in order of their share of the step (per-class event pass), the top ones only
+        from electron_integral_playground_amd.engine import autotune_in_step
+
+        def synced(mixed):
+            def run():
+                step(mixed)
+                torch.cuda.synchronize()
+            return run
+        for mixed in (False, True):
+            t = class_times(plan, blocks, A, mixed, stream)
+            top = [k for k, v in sorted(t.items(), key=lambda kv: -kv[1]) if v >= 0.01 * sum(t.values())]
+            log(f"step-tune {'float32' if mixed else 'float64'}: {len(top)} classes")
+            autotune_in_step(plan.lib, top, synced(mixed), use_f32=mixed, save=args.tune_step, log=log)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step(False)
@@ -1047,6 +1062,9 @@ def main():
                     help="autotune the chunk-plan variants (vector and matrix-core) on the headline slice first "
                          "(or, with --mixed-full, on the full workload) and save the table to JSON")
     ap.add_argument("--tune-f32-only", action="store_true", help="with --tune: the float32 tier's variants only")
+    ap.add_argument("--tune-step", metavar="JSON",
+                    help="before the timed headline steps, choose the variants of the classes above 1 %% of the "
+                         "step by the step time itself (engine.autotune_in_step) and write them to JSON")
     ap.add_argument("--tuning", metavar="JSON", help="apply this variant table on top of the shipped one")
     ap.add_argument("--serving", action="store_true", help="run only the serving workload")
     ap.add_argument("--batched-only", action="store_true", help="run only the batched workload")

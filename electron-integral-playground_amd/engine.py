@@ -1975,6 +1975,58 @@ def autotune(plan, repeats=3, use_f32=False, save=None, log=None, block_bytes=No
     return chosen
 
 
+def autotune_in_step(lib, classes, step, use_f32=False, repeats=2, save=None, log=None):
+    """Variant choice by the time of the whole step the classes run in, not of each class alone.
+
+    ``step()`` runs one full step (every class, on however many streams it uses) and returns when it is
+    done; ``classes`` are (la, lb, lc) in the order to tune (the costliest first).  Coordinate descent:
+    for each class in turn, every usable variant of the class (the others at their current choice) is
+    timed over ``repeats`` steps (the fastest counts), and the fastest kept.  Timed alone, a class often
+    prefers the variant with the most waves per SIMD; inside a step whose streams already fill the
+    device with other classes' waves, the variant with the least work can win instead.  Returns
+    {"<la><lb><lc>_<f32|f64>": variant}."""
+    import time
+    chosen = {}
+    prec = "f32" if use_f32 else "f64"
+
+    def timed():
+        best = None
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            step()
+            t = time.perf_counter() - t0
+            best = t if best is None else min(best, t)
+        return best
+
+    for la, lb, lc in classes:
+        key = f"{la}{lb}{lc}_{prec}"
+        cur = lib.eip_three_center_variant(la, lb, lc, int(use_f32))
+        # the current choice stays a candidate even where a kernel of it reaches scratch memory (a table
+        # entry measured faster than the variants that do not)
+        cands = sorted(set(usable_variants(lib, la, lb, lc, use_f32)) | ({cur} if cur >= 0 else set()))
+        if len(cands) < 2:
+            continue
+        res = {}
+        for v in cands:
+            lib.eip_set_three_center_variant(la, lb, lc, int(use_f32), v)
+            res[v] = timed()
+        best = min(res, key=res.get)
+        # keep the current choice unless another is clearly (1 %) faster: step times vary by that much
+        if cur in res and res[cur] <= 1.01 * res[best]:
+            best = cur
+        lib.eip_set_three_center_variant(la, lb, lc, int(use_f32), best)
+        chosen[key] = best
+        if log:
+            log(f"  step-tune {key}: " + ", ".join(f"v{v} {t * 1e3:.1f} ms" for v, t in res.items()) + f" -> v{best}")
+    if save:
+        path = _pathlib.Path(save)
+        path.parent.mkdir(parents=True, exist_ok=True)
+        old = _json.loads(path.read_text()) if path.exists() else {"variants": {}}
+        old["variants"].update(chosen)
+        path.write_text(_json.dumps(old, indent=1, sort_keys=True))
+    return chosen
+
+
 def autotune_queue(batch, out, waves=(0, 2, 1), entry_costs=(1.0e4, 3.0e4, 1.0e5), repeats=3, save=None, log=None):
     """Pick the persistent queue's grid occupancy cap (waves per SIMD; 0 = as many as fit) and entry
     grouping (work per entry) for a BatchPlan ``batch`` writing into ``out``: every combination is built,

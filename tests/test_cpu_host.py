@@ -580,3 +580,43 @@ def test_serving_returns_cached_memory_only_between_steps():
     state["reserved"] = serving.CACHE_SLACK_BYTES / 2
     server._retire()
     assert len(calls) == 2 and not server._release_pending    # little slack: kept
+
+
+def test_step_tuner_keeps_current_choice_unless_clearly_faster(monkeypatch, tmp_path):
+    """engine.autotune_in_step on a fake library and a fake step clock: a class whose current variant is
+    not among the usable ones keeps it when it is fastest; a variant 0.5 % faster than the current one
+    does not replace it, one 5 % faster does; a class with one candidate is left alone; the choices are
+    merged into the saved table."""
+    import time
+    from electron_integral_playground_amd import engine
+
+    class Lib:
+        def __init__(self):
+            self.v = {(2, 3, 4): 2, (1, 2, 3): 0, (2, 2, 4): 1, (0, 0, 0): 0}
+
+        def eip_three_center_variant(self, la, lb, lc, f32):
+            return self.v[(la, lb, lc)]
+
+        def eip_set_three_center_variant(self, la, lb, lc, f32, v):
+            self.v[(la, lb, lc)] = v
+
+    lib = Lib()
+    usable = {(2, 3, 4): [0, 3], (1, 2, 3): [0, 1], (2, 2, 4): [0, 1, 2], (0, 0, 0): [0]}
+    monkeypatch.setattr(engine, "usable_variants", lambda _l, la, lb, lc, f32: usable[(la, lb, lc)])
+    cost = {(2, 3, 4): {0: 0.10, 2: 0.05, 3: 0.12}, (1, 2, 3): {0: 0.0200, 1: 0.0190},
+            (2, 2, 4): {0: 0.040, 1: 0.0402, 2: 0.040}, (0, 0, 0): {0: 0.001}}
+    clock = [0.0]
+    monkeypatch.setattr(time, "perf_counter", lambda: clock[0])
+
+    def step():
+        clock[0] += 0.5 + sum(cost[k][v] for k, v in lib.v.items())
+
+    path = tmp_path / "t.json"
+    path.write_text('{"variants": {"000_f64": 0}}')
+    chosen = engine.autotune_in_step(lib, [(2, 3, 4), (1, 2, 3), (2, 2, 4), (0, 0, 0)], step, save=str(path))
+    assert chosen == {"234_f64": 2, "123_f64": 0, "224_f64": 1}
+    # 5 % of the step: the faster variant wins
+    cost[(1, 2, 3)][1] = 0.0200 - 0.05 * 0.66
+    assert engine.autotune_in_step(lib, [(1, 2, 3)], step)["123_f64"] == 1
+    import json
+    assert json.loads(path.read_text())["variants"] == {"000_f64": 0, "234_f64": 2, "123_f64": 0, "224_f64": 1}
