@@ -252,43 +252,9 @@ class _RDemand:
 
 
 def _emit_R(lines, L, needed0, ty, base="R000_", prefix="r"):
-    """Emit R^0_t for t in needed0 (plus dependencies), from R^n_000 named <base>n."""
-    need = set((0, t) for t in needed0)
-    stack = list(need)
-    deps = {}
-    while stack:
-        n, t = stack.pop()
-        tx, ty_, tz = t
-        if t == (0, 0, 0):
-            deps[(n, t)] = []
-            continue
-        if tx > 0:
-            d = [(n + 1, (tx - 1, ty_, tz))] + ([(n + 1, (tx - 2, ty_, tz))] if tx > 1 else [])
-        elif ty_ > 0:
-            d = [(n + 1, (tx, ty_ - 1, tz))] + ([(n + 1, (tx, ty_ - 2, tz))] if ty_ > 1 else [])
-        else:
-            d = [(n + 1, (tx, ty_, tz - 1))] + ([(n + 1, (tx, ty_, tz - 2))] if tz > 1 else [])
-        deps[(n, t)] = d
-        for x in d:
-            if x not in need:
-                need.add(x)
-                stack.append(x)
-    order = sorted(need, key=lambda x: (-x[0], sum(x[1]), x[1]))
-    for n, t in order:
-        tx, ty_, tz = t
-        if t == (0, 0, 0):
-            lines.append(f"        const {ty} {_r_name(n, t, prefix)} = {base}{n};")
-            continue
-        if tx > 0:
-            a, b, X, k = (tx - 1, ty_, tz), (tx - 2, ty_, tz), "PQx", tx - 1
-        elif ty_ > 0:
-            a, b, X, k = (tx, ty_ - 1, tz), (tx, ty_ - 2, tz), "PQy", ty_ - 1
-        else:
-            a, b, X, k = (tx, ty_, tz - 1), (tx, ty_, tz - 2), "PQz", tz - 1
-        expr = f"{X} * {_r_name(n + 1, a, prefix)}"
-        if k > 0:
-            expr += f" + {fmt(k)}{'f' if ty == 'float' else ''} * {_r_name(n + 1, b, prefix)}"
-        lines.append(f"        const {ty} {_r_name(n, t, prefix)} = {expr};")
+    """Emit R^0_t for t in needed0 (plus dependencies), from R^n_000 named <base>n, level by level."""
+    for n, t in sorted(_R_deps(needed0), key=lambda x: (-x[0], sum(x[1]), x[1])):
+        lines.append(_R_line(n, t, ty, base, prefix))
 
 
 # ---------------------------------------------------------------------------------------------
