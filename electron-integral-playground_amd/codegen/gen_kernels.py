@@ -1293,7 +1293,9 @@ def emit_j3c_bf16(la, lb, lc, m0, m1):
             out.append(f"          const float ae_{N}_{n} = {ex};")
         out.append(f"          Aenv[{N}] = ae_{N}_0;")
     out.append("        }")
-    _emit_R(out, L, needed, "float")
+    rdemand = _RDemand(needed, "float") if R_ORDER == "demand" else None
+    if rdemand is None:
+        _emit_R(out, L, needed, "float")
     for qg in range(nq):
         out.append(f"        // Hermite indices {8 * qg} .. {min(len(ts), 8 * qg + 8) - 1}")
         for m in ms:
@@ -1305,6 +1307,9 @@ def emit_j3c_bf16(la, lb, lc, m0, m1):
                     vals.append("0.0f")
                     continue
                 t = ts[ti]
+                if rdemand is not None:
+                    for s_ in terms:
+                        rdemand.request(out, (t[0] + s_[0], t[1] + s_[1], t[2] + s_[2]))
                 pieces = []
                 for s_, (qf, j_) in terms.items():
                     assert j_ == 0
