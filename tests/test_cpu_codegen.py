@@ -223,3 +223,26 @@ def test_demand_ordered_hermite_integrals_define_before_use(la, lb, lc, m0, m1, 
     first_v = next(i for i, l in enumerate(dem.splitlines()) if re.match(rf"\s*const {ty} V_", l))
     last_r = max(i for i, l in enumerate(dem.splitlines()) if pat.match(l))
     assert first_v < last_r or len(defs(dem)) <= 1
+
+
+@pytest.mark.parametrize("emitter,ty", [("emit_j3c_mfma", "double"), ("emit_j3c_bf16", "float")])
+@pytest.mark.parametrize("cls", [(2, 3, 2, 0, 5), (3, 3, 4, 0, 1)])
+def test_matrix_core_kernels_demand_order(emitter, ty, cls, monkeypatch):
+    """The matrix-core emitters under the demand order: the same Hermite-integral definitions as the
+    level order, each defined before it is read."""
+    emit = getattr(gen_kernels, emitter)
+    monkeypatch.setattr(gen_kernels, "R_ORDER", "level")
+    lev = emit(*cls)[1]
+    monkeypatch.setattr(gen_kernels, "R_ORDER", "demand")
+    dem = emit(*cls)[1]
+    pat = re.compile(rf"\s*const {ty} (r\d+_\d+_\d+_\d+) = (.*);")
+    defs = lambda code: sorted(m.groups() for m in map(pat.match, code.splitlines()) if m)
+    assert defs(lev) == defs(dem) and len(defs(dem)) > 0
+    seen = set()
+    for line in dem.splitlines():
+        rhs = line.split("=", 1)[1] if "=" in line else line
+        for ref in re.findall(r"\b(r\d+_\d+_\d+_\d+)\b", rhs):
+            assert ref in seen, f"{ref} read before it is defined"
+        m = pat.match(line)
+        if m:
+            seen.add(m.group(1))
