@@ -479,16 +479,24 @@ def headline_leg(args):
             top = [k for k, v in sorted(t.items(), key=lambda kv: -kv[1]) if v >= 0.01 * sum(t.values())]
             log(f"step-tune {'float32' if mixed else 'float64'}: {len(top)} classes")
             autotune_in_step(plan.lib, top, synced(mixed), use_f32=mixed, save=args.tune_step, log=log)
+    # step boundaries as events on the launch stream (every lane joins it at the end of a step), so the
+    # per-step durations come without synchronising between steps
+    marks = {False: [], True: []}
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step(False)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / args.steps
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step(True)
+    for mixed in (False, True):
+        if mixed:
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / args.steps
+            t0 = time.perf_counter()
+        marks[mixed].append(stream.record_event(torch.cuda.Event(enable_timing=True)))
+        for _ in range(args.steps):
+            step(mixed)
+            marks[mixed].append(stream.record_event(torch.cuda.Event(enable_timing=True)))
     torch.cuda.synchronize()
     dt_mixed = (time.perf_counter() - t0) / args.steps
+    each = {m: [a.elapsed_time(b) for a, b in zip(ev[:-1], ev[1:])] for m, ev in marks.items()}
+    log("headline steps (ms): plain " + " ".join(f"{x:.0f}" for x in each[False]) + "; mixed "
+        + " ".join(f"{x:.0f}" for x in each[True]))
     n_int = mol_s.n_aux * mol.n_ao * (mol.n_ao + 1) // 2
     log(f"headline step plain {dt * 1e3:.1f} ms -> {n_int / dt:.4e} integrals/s; mixed {dt_mixed * 1e3:.1f} ms "
         f"-> {n_int / dt_mixed:.4e} ({dt / dt_mixed:.3f}x)")
@@ -567,7 +575,8 @@ def headline_leg(args):
              "f32_triplet_fraction": stats["f32_triplets"] / (stats["f32_triplets"] + stats["f64_triplets"]),
              "max_abs_err_vs_plain_fp64": max_abs.item(), "rms_err_vs_plain_fp64": (sum_sq.item() / n_el) ** 0.5,
              "max_abs_value": max_ref.item(), "compared_elements": n_el,
-             "class_seconds_per_step": {"%d%d%d" % k: v for k, v in times32.items()}}
+             "class_seconds_per_step": {"%d%d%d" % k: v for k, v in times32.items()},
+             "ms_each_step": [round(x, 2) for x in each[True]]}
     cpu = None
     if not args.no_cpu:
         cpu, err = cpu_baseline(mol_s, pairs, aux, gpu_cols, blocks_ao)
@@ -580,7 +589,8 @@ def headline_leg(args):
            "n_aux_full": mol.n_aux, "primitive_pairs": n_pp, "blocks_per_step": len(blocks),
            "roofline": roofline, "cpu_baseline": cpu, "mixed": mixed,
            "max_abs_err_vs_cpu_reference_sample": cpu_err,
-           "class_seconds_per_step": {"%d%d%d" % k: v for k, v, _ in per_class}}
+           "class_seconds_per_step": {"%d%d%d" % k: v for k, v, _ in per_class},
+           "ms_each_step": [round(x, 2) for x in each[False]]}
     del plan
     torch.cuda.empty_cache()
     return res
@@ -1179,7 +1189,7 @@ def main():
             "roofline": head["roofline"], "cpu_baseline": head["cpu_baseline"],
             "mixed_precision": head["mixed"],
             "max_abs_err_vs_cpu_reference_sample": head["max_abs_err_vs_cpu_reference_sample"],
-            "class_seconds_per_step": head["class_seconds_per_step"],
+            "class_seconds_per_step": head["class_seconds_per_step"], "ms_each_step": head["ms_each_step"],
             "plain80": plain80, "plain80_omega": omega80, "dropin_loop": dropin, "batched": batched, "serving": serving,
             "three_center_variants": variants}
     print(json.dumps(line), flush=True)
