@@ -1039,15 +1039,12 @@ def emit_j3c_mfma(la, lb, lc, m0, m1):
     for n in range(1, L + 1):
         out.append(f"        const double pf_{n} = {'pf' if n == 1 else f'pf_{n - 1}'} * m2z;")
         out.append(f"        const double R000_{n} = pf_{n} * F[{n}];")
-    rdemand = _RDemand(needed, "double") if R_ORDER == "demand" else None
-    if rdemand is None:
-        _emit_R(out, L, needed, "double")
+    # level order here: with the Hermite integrals emitted next to their use, the VALU work interleaves
+    # with the matrix-core instructions and the (ff|p), (ff|d) kernels ran 1.7-3.6x slower
+    _emit_R(out, L, needed, "double")
     for ti, t in enumerate(ts):
         for m in ms:
             r, terms = aux_rows[m]
-            if rdemand is not None:
-                for s in terms:
-                    rdemand.request(out, (t[0] + s[0], t[1] + s[1], t[2] + s[2]))
             pieces = []
             for s, (qf, j) in terms.items():
                 assert j == 0

@@ -1907,7 +1907,10 @@ def usable_variants(lib, la, lb, lc, f32=False):
 
 
 def apply_tuning(path=TUNING_FILE):
-    """Set every class's variant from the tuning file (if it exists); returns the table."""
+    """Set every class's variant from the tuning file (if it exists); returns the table.  Any explicit
+    call counts as the process's tuning: plans created afterwards do not apply the shipped table over it."""
+    global _TUNING_APPLIED
+    _TUNING_APPLIED = True
     lib = library()
     if not _pathlib.Path(path).exists():
         return {}

@@ -225,11 +225,11 @@ def test_demand_ordered_hermite_integrals_define_before_use(la, lb, lc, m0, m1, 
     assert first_v < last_r or len(defs(dem)) <= 1
 
 
-@pytest.mark.parametrize("emitter,ty", [("emit_j3c_mfma", "double"), ("emit_j3c_bf16", "float")])
+@pytest.mark.parametrize("emitter,ty", [("emit_j3c_bf16", "float")])
 @pytest.mark.parametrize("cls", [(2, 3, 2, 0, 5), (3, 3, 4, 0, 1)])
 def test_matrix_core_kernels_demand_order(emitter, ty, cls, monkeypatch):
-    """The matrix-core emitters under the demand order: the same Hermite-integral definitions as the
-    level order, each defined before it is read."""
+    """The bf16 matrix-core emitter under the demand order: the same Hermite-integral definitions as the
+    level order, each defined before it is read.  (The float64 matrix-core emitter keeps the level order.)"""
     emit = getattr(gen_kernels, emitter)
     monkeypatch.setattr(gen_kernels, "R_ORDER", "level")
     lev = emit(*cls)[1]

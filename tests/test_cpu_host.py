@@ -620,3 +620,24 @@ def test_step_tuner_keeps_current_choice_unless_clearly_faster(monkeypatch, tmp_
     assert engine.autotune_in_step(lib, [(1, 2, 3)], step)["123_f64"] == 1
     import json
     assert json.loads(path.read_text())["variants"] == {"000_f64": 0, "234_f64": 2, "123_f64": 0, "224_f64": 1}
+
+
+def test_explicit_tuning_is_not_overridden_by_the_shipped_table(tmp_path):
+    """A variant table applied explicitly (bench.py --tuning) stays in force when plans are created
+    afterwards: the shipped table is applied once per process only if nothing was applied before."""
+    import json
+    from electron_integral_playground_amd import engine, library
+    lib = library()
+    shipped = json.loads(engine.TUNING_FILE.read_text())["variants"]["332_f64"]
+    other = 0 if shipped != 0 else 1
+    path = tmp_path / "t.json"
+    path.write_text(json.dumps({"variants": {"332_f64": other}}))
+    try:
+        engine._TUNING_APPLIED = False
+        engine.apply_tuning()
+        engine.apply_tuning(path)
+        engine._ensure_tuning()                       # what a new plan does
+        assert lib.eip_three_center_variant(3, 3, 2, 0) == other
+    finally:
+        engine.apply_tuning()
+    assert lib.eip_three_center_variant(3, 3, 2, 0) == shipped
