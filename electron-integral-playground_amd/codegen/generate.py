@@ -366,7 +366,7 @@ TUNING_FILE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__fil
 # Register tiers of the persistent kernels, by the occupancy (waves per SIMD) of the class chunk's own
 # kernel: a tier kernel runs at the occupancy of its largest body, so bodies are grouped with bodies of
 # the same occupancy.
-QUEUE_TIERS = ((3, 99), (2, 2), (1, 1))
+QUEUE_TIERS = ((4, 99), (3, 3), (2, 2), (1, 1))
 
 
 # Largest register footprint (VGPRs + AGPRs per lane, of the chunk's own kernel) of a body in a queue
