@@ -977,8 +977,12 @@ class ThreeCenterPlan:
         buffered blocks, where block k + 2 waits only for block k)."""
         pc = self.pairs[pi]
         nij = (2 * pc.la + 1) * (2 * pc.lb + 1)
-        base = out.data_ptr() - 8 * sp0 * nij * self.n_aux
-        entries = [self.job_words(pi, ai, base, self.n_aux, self.mixed[(pi, ai)]["perm"] if mixed else None)
+        # rows of n_aux columns at the tensor's row stride (a padded buffer may have a longer one)
+        if out.stride(1) != 1 or out.shape[1] < self.n_aux or out.stride(0) < self.n_aux:
+            raise ValueError("run_block: out must hold rows of n_aux contiguous columns")
+        ld = out.stride(0)
+        base = out.data_ptr() - 8 * sp0 * nij * ld
+        entries = [self.job_words(pi, ai, base, ld, self.mixed[(pi, ai)]["perm"] if mixed else None)
                    for ai in range(len(self.aux))]
         jobs = self.job_table(entries)
         groups = []          # (cost, [launch(stream) in order]): groups are independent of each other
