@@ -172,6 +172,29 @@ def test_row_blocks_match_packed_output():
                         assert torch.equal(buf[row], want), (pi, sp, a, b)
 
 
+def test_row_blocks_into_padded_rows():
+    """A block buffer whose rows are longer than n_aux (a column slice of a wider tensor): run_block
+    writes the same values at the tensor's row stride and leaves the padding alone; a buffer without
+    contiguous columns is rejected."""
+    import torch
+    mol, _, plan = _block_plan()
+    blocks = plan.row_blocks(3 * 25 * mol.n_aux * 8)
+    rows = max(plan.block_rows(*b) for b in blocks)
+    ref = torch.empty((rows, mol.n_aux), dtype=torch.float64, device="cuda")
+    wide = torch.full((rows, mol.n_aux + 40), -7.0, dtype=torch.float64, device="cuda")
+    padded = wide[:, :mol.n_aux]
+    for b in blocks[::3]:
+        print(f"case padded block {b}", flush=True)
+        n = plan.block_rows(*b)
+        plan.run_block(*b, ref)
+        plan.run_block(*b, padded)
+        torch.cuda.synchronize()
+        assert torch.equal(padded[:n], ref[:n]), b
+        assert bool((wide[:, mol.n_aux:] == -7.0).all())
+    with pytest.raises(ValueError):
+        plan.run_block(*blocks[0], wide.t())
+
+
 def test_row_blocks_mixed_within_bound():
     """The mixed split (float64 prefix stored, float32 suffix added) per block stays within 1e-10 of
     float64, and a threshold of 0 (everything float64) reproduces float64 exactly."""
